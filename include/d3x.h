@@ -1,0 +1,90 @@
+/* D3 dispersion C API — extensions beside the reference's d3.h.
+ *
+ * These entry points have no counterpart in the reference (src/d3.h); they add what the float
+ * API cannot carry: FP64 inputs/outputs (a float total energy resolves only ~6e-8 relative),
+ * a precision mode, custom functional parameters (the reference has no setter for CUSTOM,
+ * src/d3_buffer.cu:283), per-atom diagnostics, distinct status codes (the reference uses 1 for
+ * every failure, src/d3.cu:134,224), batched evaluation of many cells, device-resident timing
+ * entry points, and parameter-table selection.  Units are those of d3.h (Angstrom in, cutoffs
+ * in bohr, eV / eV/Angstrom / eV/Angstrom^3 out) unless stated.
+ */
+#ifndef D3X_H
+#define D3X_H
+
+#include <stdint.h>
+
+#include "d3.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Arithmetic of the pair and triplet kernels.  FP64: plain double everywhere.  FP32: float
+ * pair/triplet math with compensated (two-sum) per-atom accumulation; per-atom results, the
+ * total energy and the virial are reduced in double. */
+typedef enum { D3X_FP64 = 0, D3X_FP32 = 1 } d3x_precision_t;
+
+/* Status bits returned by d3x_* calls and d3x_last_status (0 = success). */
+#define D3X_ERR_GENERIC 1u   /* any failure (also what compute_..._status returns)        */
+#define D3X_ERR_OVERFLOW 2u  /* a neighbour-list capacity was exceeded                    */
+#define D3X_ERR_INPUT 4u     /* bad argument: element, length, cell, cutoff              */
+#define D3X_ERR_DEVICE 8u    /* HIP runtime error                                        */
+#define D3X_ERR_STATE 16u    /* call order: atoms or cell not set                        */
+
+/* Handle with every option.  atm_cutoff <= 0 disables the three-body term.  damping: 0 zero,
+ * 1 BJ; functional: d3_types.h values; precision: d3x_precision_t. */
+d3_handle_t *d3x_init(const uint16_t *elements, uint64_t num_elements, uint64_t max_length, double cutoff,
+                      double cn_cutoff, double atm_cutoff, int damping, int functional, int precision);
+
+/* params = {s6, s8_zero, s8_bj, sr6, sr8, a1, a2} (the reference's functional_params_t order,
+ * src/d3_types.h:36-44).  Applies to the handle whatever functional it was created with. */
+int d3x_set_functional_params(d3_handle_t *handle, const double params[7]);
+
+/* FP64 variants of set_atoms / set_cell / compute.  coords: length x 3 Angstrom. */
+int d3x_set_atoms_f64(d3_handle_t *handle, const double *coords, const uint16_t *elements, uint64_t length);
+int d3x_set_cell_f64(d3_handle_t *handle, const double cell[9]);
+int d3x_compute_f64(d3_handle_t *handle, double *energy, double *forces, double *stress);
+
+/* Per-atom results of the last compute, caller's atom order; any pointer may be NULL.
+ * cn: coordination numbers; dedcn: dE+/dCN in Hartree (two-body + three-body, the input of the
+ * chain-rule pass); atom_energy: eV. */
+int d3x_get_atom_data(d3_handle_t *handle, double *cn, double *dedcn, double *atom_energy);
+
+/* Device-resident evaluation for timing: runs the whole pipeline on the handle's stream from
+ * the coordinates already on the device and leaves the results on the device (no host
+ * transfer, no synchronisation).  d3x_fetch_results copies them out (synchronising). */
+int d3x_compute_device(d3_handle_t *handle);
+int d3x_fetch_results(d3_handle_t *handle, double *energy, double *forces, double *stress);
+
+/* The HIP stream (hipStream_t) the handle launches on; d3x_set_stream replaces it (NULL =
+ * the handle's own stream). */
+void *d3x_get_stream(d3_handle_t *handle);
+int d3x_set_stream(d3_handle_t *handle, void *stream);
+
+/* Batched evaluation of n_struct independent periodic cells in one pass of launches.
+ * atom_counts[s] atoms each, concatenated coords (Angstrom) and elements, cells (n_struct x 9).
+ * Outputs: energies[n_struct], forces (concatenated, atoms x 3), stresses (n_struct x 9).
+ * The total atom count must not exceed the handle's max_length. */
+int d3x_compute_batch_f64(d3_handle_t *handle, uint64_t n_struct, const uint64_t *atom_counts,
+                          const double *coords, const uint16_t *elements, const double *cells, double *energies,
+                          double *forces, double *stresses);
+
+/* Detailed status of the handle's last call, and the last error message of this thread. */
+uint32_t d3x_last_status(d3_handle_t *handle);
+const char *d3x_last_error(void);
+
+/* Parameter tables (process-wide; handles copy what they need at init).
+ * d3x_table_load: NULL, "" or "synthetic" = built-in synthetic 95 x 5 set (default),
+ * "synthetic-extended" = built-in synthetic 103 x 7 set, anything else = path of a table file
+ * (format in lasp-d3_amd/csrc/d3_tables.hpp).  The synthetic sets have the reference's layout
+ * and invariants but NON-physical values. */
+int d3x_table_load(const char *path);
+const char *d3x_table_name(void);
+int d3x_table_shape(int *nel, int *nref, int *zoff, int *zmax);
+int d3x_table_export(float *c6ab, float *r0ab, float *rcov, float *r2r4);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* D3X_H */

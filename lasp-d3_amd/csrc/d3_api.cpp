@@ -1,0 +1,299 @@
+// C ABI of the D3 library: the reference's d3.h entry points (src/d3.cu:14-252) and the d3x.h
+// extensions, on top of d3::Engine.  Errors never escape as C++ exceptions: they are printed to
+// stderr (as the reference does), recorded for d3x_last_error / d3x_last_status, and turned into
+// NULL / non-zero returns.
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/d3x.h"
+#include "d3_host.hpp"
+#include "d3_tables.hpp"
+
+using d3::Engine;
+using d3::Error;
+
+namespace {
+
+struct Handle {
+    std::unique_ptr<Engine> eng;
+    uint32_t status = 0;
+};
+
+Handle *H(d3_handle_t *h) { return static_cast<Handle *>(h); }
+
+// Default arithmetic of handles made through the reference-compatible init functions:
+// FP64 unless D3_PRECISION=fp32 is set in the environment.
+int default_precision() {
+    const char *p = std::getenv("D3_PRECISION");
+    return (p && (std::strcmp(p, "fp32") == 0 || std::strcmp(p, "FP32") == 0)) ? D3X_FP32 : D3X_FP64;
+}
+
+void report(const char *where, const std::string &msg) {
+    d3::set_last_error(msg);
+    std::fprintf(stderr, "Error: %s: %s\n", where, msg.c_str());
+}
+
+// Runs fn, converting exceptions into a status code stored on the handle.
+template <class F>
+uint32_t guarded(Handle *h, const char *where, F &&fn) {
+    uint32_t st = 0;
+    try {
+        fn();
+    } catch (const Error &e) {
+        st = e.code ? e.code : D3X_ERR_GENERIC;
+        report(where, e.msg);
+    } catch (const std::bad_alloc &) {
+        st = D3X_ERR_GENERIC;
+        report(where, "out of host memory");
+    } catch (const std::exception &e) {
+        st = D3X_ERR_GENERIC;
+        report(where, e.what());
+    }
+    if (h) h->status = st;
+    return st;
+}
+
+// Full evaluation with results copied to the host; a three-body neighbour-list overflow is
+// retried with a larger capacity (capacity grows x4, at most 4 times).
+void evaluate(Engine &e, double *energies, double *forces, double *stresses) {
+    for (int attempt = 0;; ++attempt) {
+        e.compute_device();
+        try {
+            e.fetch(energies, forces, stresses);
+            return;
+        } catch (const Error &err) {
+            if ((err.code & D3X_ERR_OVERFLOW) && attempt < 4) {
+                e.grow_atm_cap();
+                continue;
+            }
+            throw;
+        }
+    }
+}
+
+d3_handle_t *make_handle(const uint16_t *elements, uint64_t ne, uint64_t max_length, double rc, double rccn,
+                         double rc3, int damping, int functional, int precision) {
+    Handle *h = nullptr;
+    uint32_t st = guarded(nullptr, "init_d3_handle", [&] {
+        std::unique_ptr<Handle> hh(new Handle);
+        hh->eng.reset(new Engine(elements, ne, max_length, rc, rccn, rc3, damping, functional, precision));
+        h = hh.release();
+    });
+    return st ? nullptr : h;
+}
+
+}  // namespace
+
+extern "C" {
+
+d3_handle_t *init_d3_handle(uint16_t *elements, uint64_t num_elements, uint64_t max_length, real_t cutoff_radius,
+                            real_t coordination_number_cutoff, damping_type_t damping_type,
+                            functional_t functional_type) {
+    return make_handle(elements, num_elements, max_length, cutoff_radius, coordination_number_cutoff, 0.0,
+                       (int)damping_type, (int)functional_type, default_precision());
+}
+
+d3_handle_t *init_d3_handle_with_atm(uint16_t *elements, uint64_t length_elements, uint64_t max_length,
+                                     real_t cutoff_radius, real_t coordination_number_cutoff, real_t atm_cutoff,
+                                     damping_type_t damping_type, functional_t functional_type) {
+    if (!(atm_cutoff > 0)) {
+        report("init_d3_handle_with_atm", "atm_cutoff must be positive");
+        return nullptr;
+    }
+    return make_handle(elements, length_elements, max_length, cutoff_radius, coordination_number_cutoff, atm_cutoff,
+                       (int)damping_type, (int)functional_type, default_precision());
+}
+
+void set_atoms(d3_handle_t *handle, real_t *coords, uint16_t *elements, uint64_t length) {
+    if (!handle) {
+        report("set_atoms", "handle is NULL");
+        return;
+    }
+    guarded(H(handle), "set_atoms", [&] {
+        std::vector<double> c(3 * length);
+        for (uint64_t k = 0; k < 3 * length; ++k) c[k] = coords[k];
+        H(handle)->eng->set_atoms(c.data(), elements, length);
+    });
+}
+
+void set_cell(d3_handle_t *handle, real_t cell[3][3]) {
+    if (!handle) {
+        report("set_cell", "handle is NULL");
+        return;
+    }
+    guarded(H(handle), "set_cell", [&] {
+        double c[9];
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) c[a * 3 + b] = cell[a][b];
+        H(handle)->eng->set_cell(c);
+    });
+}
+
+void free_d3_handle(d3_handle_t *handle) {
+    if (!handle) {
+        report("free_d3_handle", "handle is NULL");
+        return;
+    }
+    delete H(handle);
+}
+
+void clear_d3_handle(d3_handle_t *handle) {
+    if (!handle) {
+        report("clear_d3_handle", "handle is NULL");
+        return;
+    }
+    guarded(H(handle), "clear_d3_handle", [&] { H(handle)->eng->clear(); });
+}
+
+uint16_t compute_dispersion_energy_from_handle_status(d3_handle_t *handle, real_t *energy, real_t *force,
+                                                      real_t *stress) {
+    if (!handle) {
+        report("compute_dispersion_energy_from_handle", "handle is NULL");
+        return 1;
+    }
+    Handle *h = H(handle);
+    uint32_t st = guarded(h, "compute_dispersion_energy_from_handle", [&] {
+        Engine &e = *h->eng;
+        const uint64_t n = e.num_atoms();
+        std::vector<double> f(3 * n);
+        double E, S[9];
+        evaluate(e, &E, f.data(), S);
+        if (energy) *energy = (real_t)E;
+        if (force)
+            for (uint64_t k = 0; k < 3 * n; ++k) force[k] = (real_t)f[k];
+        if (stress)
+            for (int k = 0; k < 9; ++k) stress[k] = (real_t)S[k];
+    });
+    return st ? 1 : 0;
+}
+
+void compute_dispersion_energy_from_handle(d3_handle_t *handle, real_t *energy, real_t *force, real_t *stress) {
+    uint16_t st = compute_dispersion_energy_from_handle_status(handle, energy, force, stress);
+    if (st != 0) std::fprintf(stderr, "Error: compute_dispersion_energy_from_handle failed with status %d\n", st);
+}
+
+void compute_dispersion_energy(real_t atoms[][3], uint16_t *elements, uint64_t num_atoms, real_t cell[3][3],
+                               real_t cutoff_radius, real_t CN_cutoff_radius, damping_type_t damping_type,
+                               functional_t functional_type, real_t *energy, real_t *force, real_t *stress) {
+    // as src/d3.cu:236-252: a handle over the structure's own elements, used once
+    d3_handle_t *h = init_d3_handle(elements, num_atoms, num_atoms ? num_atoms : 1, cutoff_radius, CN_cutoff_radius,
+                                    damping_type, functional_type);
+    if (!h) return;
+    set_atoms(h, (real_t *)atoms, elements, num_atoms);
+    set_cell(h, cell);
+    compute_dispersion_energy_from_handle(h, energy, force, stress);
+    free_d3_handle(h);
+}
+
+// ------------------------------------------------------------------------------------------ d3x
+
+d3_handle_t *d3x_init(const uint16_t *elements, uint64_t num_elements, uint64_t max_length, double cutoff,
+                      double cn_cutoff, double atm_cutoff, int damping, int functional, int precision) {
+    return make_handle(elements, num_elements, max_length, cutoff, cn_cutoff, atm_cutoff, damping, functional,
+                       precision);
+}
+
+int d3x_set_functional_params(d3_handle_t *handle, const double params[7]) {
+    if (!handle || !params) return (int)(D3X_ERR_GENERIC | D3X_ERR_INPUT);
+    return (int)guarded(H(handle), "d3x_set_functional_params", [&] { H(handle)->eng->set_params(params); });
+}
+
+int d3x_set_atoms_f64(d3_handle_t *handle, const double *coords, const uint16_t *elements, uint64_t length) {
+    if (!handle) return (int)(D3X_ERR_GENERIC | D3X_ERR_INPUT);
+    return (int)guarded(H(handle), "d3x_set_atoms_f64", [&] { H(handle)->eng->set_atoms(coords, elements, length); });
+}
+
+int d3x_set_cell_f64(d3_handle_t *handle, const double cell[9]) {
+    if (!handle || !cell) return (int)(D3X_ERR_GENERIC | D3X_ERR_INPUT);
+    return (int)guarded(H(handle), "d3x_set_cell_f64", [&] { H(handle)->eng->set_cell(cell); });
+}
+
+int d3x_compute_f64(d3_handle_t *handle, double *energy, double *forces, double *stress) {
+    if (!handle) return (int)(D3X_ERR_GENERIC | D3X_ERR_INPUT);
+    return (int)guarded(H(handle), "d3x_compute_f64", [&] {
+        Engine &e = *H(handle)->eng;
+        if (e.num_sys() != 1) throw Error{D3X_ERR_GENERIC | D3X_ERR_STATE, "handle holds a batch; use d3x_compute_batch_f64"};
+        evaluate(e, energy, forces, stress);
+    });
+}
+
+int d3x_get_atom_data(d3_handle_t *handle, double *cn, double *dedcn, double *atom_energy) {
+    if (!handle) return (int)(D3X_ERR_GENERIC | D3X_ERR_INPUT);
+    return (int)guarded(H(handle), "d3x_get_atom_data",
+                        [&] { H(handle)->eng->fetch_atom_data(cn, dedcn, atom_energy); });
+}
+
+int d3x_compute_device(d3_handle_t *handle) {
+    if (!handle) return (int)(D3X_ERR_GENERIC | D3X_ERR_INPUT);
+    return (int)guarded(H(handle), "d3x_compute_device", [&] { H(handle)->eng->compute_device(); });
+}
+
+int d3x_fetch_results(d3_handle_t *handle, double *energy, double *forces, double *stress) {
+    if (!handle) return (int)(D3X_ERR_GENERIC | D3X_ERR_INPUT);
+    return (int)guarded(H(handle), "d3x_fetch_results", [&] { H(handle)->eng->fetch(energy, forces, stress); });
+}
+
+void *d3x_get_stream(d3_handle_t *handle) { return handle ? (void *)H(handle)->eng->stream() : nullptr; }
+
+int d3x_set_stream(d3_handle_t *handle, void *stream) {
+    if (!handle) return (int)(D3X_ERR_GENERIC | D3X_ERR_INPUT);
+    H(handle)->eng->set_stream((hipStream_t)stream);
+    return 0;
+}
+
+int d3x_compute_batch_f64(d3_handle_t *handle, uint64_t n_struct, const uint64_t *atom_counts,
+                          const double *coords, const uint16_t *elements, const double *cells, double *energies,
+                          double *forces, double *stresses) {
+    if (!handle) return (int)(D3X_ERR_GENERIC | D3X_ERR_INPUT);
+    return (int)guarded(H(handle), "d3x_compute_batch_f64", [&] {
+        Engine &e = *H(handle)->eng;
+        e.set_batch(n_struct, atom_counts, coords, elements, cells);
+        evaluate(e, energies, forces, stresses);
+    });
+}
+
+uint32_t d3x_last_status(d3_handle_t *handle) { return handle ? H(handle)->status : D3X_ERR_GENERIC; }
+
+const char *d3x_last_error(void) { return d3::last_error(); }
+
+int d3x_table_load(const char *path) {
+    try {
+        if (!path || !*path || std::strcmp(path, "synthetic") == 0)
+            d3::set_active_table(d3::make_synthetic_table(false));
+        else if (std::strcmp(path, "synthetic-extended") == 0)
+            d3::set_active_table(d3::make_synthetic_table(true));
+        else
+            d3::set_active_table(d3::load_table_file(path));
+        return 0;
+    } catch (const std::exception &e) {
+        report("d3x_table_load", e.what());
+        return (int)(D3X_ERR_GENERIC | D3X_ERR_INPUT);
+    }
+}
+
+const char *d3x_table_name(void) { return d3::active_table().name.c_str(); }
+
+int d3x_table_shape(int *nel, int *nref, int *zoff, int *zmax) {
+    const d3::Table &t = d3::active_table();
+    if (nel) *nel = t.nel;
+    if (nref) *nref = t.nref;
+    if (zoff) *zoff = t.zoff;
+    if (zmax) *zmax = t.zmax;
+    return 0;
+}
+
+int d3x_table_export(float *c6ab, float *r0ab, float *rcov, float *r2r4) {
+    const d3::Table &t = d3::active_table();
+    if (c6ab) std::memcpy(c6ab, t.c6ab.data(), t.c6ab.size() * sizeof(float));
+    if (r0ab) std::memcpy(r0ab, t.r0ab.data(), t.r0ab.size() * sizeof(float));
+    if (rcov) std::memcpy(rcov, t.rcov.data(), t.rcov.size() * sizeof(float));
+    if (r2r4) std::memcpy(r2r4, t.r2r4.data(), t.r2r4.size() * sizeof(float));
+    return 0;
+}
+
+}  // extern "C"

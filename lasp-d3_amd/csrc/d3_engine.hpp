@@ -1,0 +1,108 @@
+// Definitions shared by the host engine (d3_engine.cpp) and the HIP kernels (d3_kernels.hip).
+//
+// HBM layout (one handle = one batch of periodic cells):
+//   inputs (caller order): pos_in[3N] double bohr, type_in[N] u16, sys_in[N] i32
+//   bins: every cell is cut into nb[0] x nb[1] x nb[2] parallelepiped bins; the bins of all
+//         cells are numbered consecutively (SysDev::bin_base); bin_start[nbins+1] i32
+//   sorted atoms (bin order, ascending caller index inside a bin, cells contiguous):
+//         lx/ly/lz[N] T   position relative to the centre of the atom's bin (bohr)
+//         type_s, orig_s (caller index), bin_s, sys_s
+//   per sorted atom: cn[N], w/dw[NREF][N] T (normalised Gaussian CN weights and d/dCN),
+//         e[N], g[N], f[3N], vir[6N] double (energy E+, dE+/dCN, F = grad E+, virial / V)
+//   stencils: for each cell and each cutoff (CN, two-body, ATM) the list of bin offsets d whose
+//         bins can hold a pair within the cutoff, with D = -(d/nb) . H, so that for atoms i, j in
+//         bins b and b+d (wrapped) the image vector is r_i - r_j - T = l_i - l_j + D.
+#pragma once
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+
+namespace d3 {
+
+constexpr int MAX_NREF = 7;
+constexpr double BOHR_A = 0.52917726;            // src/d3.cu:50
+constexpr double HARTREE_EV = 27.211396641308;   // src/d3.cu:204
+constexpr double K1 = 16.0;                      // src/d3_kernel.cuh:17
+constexpr double K3 = 4.0;                       // src/d3_kernel.cuh:19
+
+enum Cut { CUT_CN = 0, CUT_2B = 1, CUT_ATM = 2 };
+
+struct StEntry {
+    int dx, dy, dz, pad;
+};
+
+struct SysDev {
+    double H[9];     // rows = lattice vectors (bohr)
+    double Hinv[9];  // H^-1
+    double inv_vol;  // 1 / det(H) (signed, src/d3_kernel.cu:90-96)
+    int nb[3];       // bins per axis
+    int bin_base;    // global index of this cell's bin (0,0,0)
+    int atom_base;   // first sorted (and caller) atom index of this cell
+    int n_atoms;
+    int st_off[3];   // stencil ranges per cutoff
+    int st_cnt[3];
+};
+
+// Device pointers and scalars of one evaluation; passed by value to every kernel.
+template <class T>
+struct Params {
+    int n;      // atoms in the batch
+    int ne;     // element types of the handle
+    int nref;   // references per element (<= MAX_NREF)
+    int nsys;
+    int damping;
+    T rc_cn2, rc_22, rc_32;  // squared cutoffs (bohr^2)
+    T s6, s8, p1, p2;        // s6, s8 and (sr6, sr8) or (a1, a2)
+    // structure
+    const SysDev *sys;
+    const StEntry *st_d;  // stencil bin offsets
+    const T *st_D;      // stencil image vectors, 4 per entry
+    const int *bin_start;
+    // inputs
+    const double *pos_in;
+    const uint16_t *type_in;
+    const int *sys_in;
+    // binning scratch
+    int *key_in, *bin_count, *cursor, *perm, *perm2;
+    double *wrapped;  // 3N
+    // sorted atoms
+    T *lx, *ly, *lz;
+    uint16_t *type_s;
+    int *orig_s, *bin_s, *sys_s;
+    // per-atom results (sorted order)
+    T *cn, *w, *dw;
+    double *e, *g, *f, *vir;
+    float *npair;  // 3 per atom: two-body, CN, ATM pair counts (for the work model)
+    // type tables (T): c6[ne][ne][nref][nref], cnref[ne][nref] (-1 invalid), rcov[ne],
+    // q4[ne][ne] = r2r4_A r2r4_B, rdamp[ne][ne] (zero: r0ab; BJ: a1 sqrt(3 q4) + a2),
+    // r0[ne][ne] (ATM damping radii)
+    const T *c6, *cnref, *rcov, *q4, *rdamp, *r0;
+    // ATM neighbour scratch (per block)
+    int atm_cap;
+    int *atm_j;
+    T *atm_buf;  // 6 * cap per block: dx, dy, dz, r2, c6, dlc6
+    unsigned *status;
+};
+
+// Host-side launchers (d3_kernels.hip).
+template <class T>
+struct Launch {
+    static void bin_sort(const Params<T> &P, int nbins, int *bin_start_mut, hipStream_t s);
+    static void cn(const Params<T> &P, hipStream_t s);
+    static void twobody(const Params<T> &P, hipStream_t s);
+    static void atm(const Params<T> &P, int nblocks, hipStream_t s);
+    static void chain(const Params<T> &P, hipStream_t s);
+};
+
+// Reduction of per-atom energy / virial / pair counts into per-cell totals (deterministic),
+// and scatter of per-atom results to caller order with unit conversion.
+void launch_reduce(const double *e, const double *vir, const float *npair, const int *chunk_begin,
+                   const int *chunk_end, const int *chunk_sys, int nchunks, const int *sys_chunk_off, int nsys,
+                   double *partial, double *out_sys, hipStream_t s);
+template <class T>
+void launch_finalize(const Params<T> &P, double *out_f, double *out_e, double *out_cn, double *out_g,
+                     hipStream_t s);
+
+constexpr int RED_W = 10;  // per-cell reduced values: E, vir[6], counts[3]
+
+}  // namespace d3

@@ -1,0 +1,666 @@
+// Host side of a D3 handle (declarations in d3_host.hpp).
+#include "d3_host.hpp"
+
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+
+#include "../../include/d3x.h"
+#include "d3_tables.hpp"
+
+namespace d3 {
+
+namespace {
+thread_local std::string g_err;
+
+#define HIPCHK(x)                                                                                         \
+    do {                                                                                                  \
+        hipError_t e_ = (x);                                                                              \
+        if (e_ != hipSuccess)                                                                             \
+            throw Error{D3X_ERR_GENERIC | D3X_ERR_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_)}; \
+    } while (0)
+
+void *dalloc(size_t bytes) {
+    void *p = nullptr;
+    HIPCHK(hipMalloc(&p, bytes ? bytes : 16));
+    return p;
+}
+void dfree(void *&p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+void inv3(const double m[9], double o[9]) {
+    const double det = m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6]) +
+                       m[2] * (m[3] * m[7] - m[4] * m[6]);
+    const double id = 1.0 / det;
+    o[0] = (m[4] * m[8] - m[5] * m[7]) * id;
+    o[1] = (m[2] * m[7] - m[1] * m[8]) * id;
+    o[2] = (m[1] * m[5] - m[2] * m[4]) * id;
+    o[3] = (m[5] * m[6] - m[3] * m[8]) * id;
+    o[4] = (m[0] * m[8] - m[2] * m[6]) * id;
+    o[5] = (m[2] * m[3] - m[0] * m[5]) * id;
+    o[6] = (m[3] * m[7] - m[4] * m[6]) * id;
+    o[7] = (m[1] * m[6] - m[0] * m[7]) * id;
+    o[8] = (m[0] * m[4] - m[1] * m[3]) * id;
+}
+double det3(const double m[9]) {
+    return m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6]) + m[2] * (m[3] * m[7] - m[4] * m[6]);
+}
+[[noreturn]] void fail(uint32_t code, const std::string &msg) { throw Error{code | D3X_ERR_GENERIC, msg}; }
+
+constexpr int CHUNK = 2048;  // atoms per first-level reduction chunk
+
+// Bin offsets d of one cell for cutoff rc: every d for which some pair of points in bins b and
+// b + d can be within rc.  Lower bound of the distance: the larger of the slab bound
+// max_k (|d_k| - 1) h_k / n_k (h_k = perpendicular height of the cell) and |D| - diag, with
+// D = -(d / n) . H the bin-centre offset and diag the longest bin diagonal.
+void make_stencil(const double H[9], const int nb[3], double rc, std::vector<StEntry> &out_d,
+                  std::vector<double> &out_D) {
+    double Hi[9];
+    inv3(H, Hi);
+    double hb[3], e[3][3];
+    for (int k = 0; k < 3; ++k) {
+        const double nk = std::sqrt(Hi[0 * 3 + k] * Hi[0 * 3 + k] + Hi[1 * 3 + k] * Hi[1 * 3 + k] + Hi[2 * 3 + k] * Hi[2 * 3 + k]);
+        hb[k] = (1.0 / nk) / nb[k];
+        for (int c = 0; c < 3; ++c) e[k][c] = H[k * 3 + c] / nb[k];
+    }
+    double diag = 0;
+    for (int s1 = -1; s1 <= 1; s1 += 2)
+        for (int s2 = -1; s2 <= 1; s2 += 2) {
+            double v[3];
+            for (int c = 0; c < 3; ++c) v[c] = e[0][c] + s1 * e[1][c] + s2 * e[2][c];
+            diag = std::max(diag, std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
+        }
+    int R[3];
+    for (int k = 0; k < 3; ++k) R[k] = (int)std::ceil(rc / hb[k]) + 1;
+    const double lim = rc * (1.0 + 1e-10) + 1e-8;
+    for (int dz = -R[2]; dz <= R[2]; ++dz)
+        for (int dy = -R[1]; dy <= R[1]; ++dy)
+            for (int dx = -R[0]; dx <= R[0]; ++dx) {
+                const int d[3] = {dx, dy, dz};
+                double D[3];
+                for (int c = 0; c < 3; ++c) D[c] = -(dx * e[0][c] + dy * e[1][c] + dz * e[2][c]);
+                double slab = 0;
+                for (int k = 0; k < 3; ++k) slab = std::max(slab, std::max(0, std::abs(d[k]) - 1) * hb[k]);
+                const double sph = std::sqrt(D[0] * D[0] + D[1] * D[1] + D[2] * D[2]) - diag;
+                if (std::max(slab, sph) <= lim) {
+                    out_d.push_back(StEntry{dx, dy, dz, 0});
+                    for (int c = 0; c < 3; ++c) out_D.push_back(D[c]);
+                    out_D.push_back(0.0);
+                }
+            }
+}
+
+}  // namespace
+
+void set_last_error(const std::string &msg) { g_err = msg; }
+const char *last_error() { return g_err.c_str(); }
+
+bool functional_params(int functional, double out[7]) {
+    // {s6, s8_zero, s8_bj, sr6, sr8, a1, a2}: the reference's FUNCTIONAL_PARAMS
+    // (src/d3_types.h:46-54), kept as the same float values.
+    static const float tab[7][7] = {
+        {1.0f, 0.722f, 0.7875f, 1.217f, 1.0f, 0.4289f, 4.4407f},  // PBE
+        {1.0f, 0.928f, 1.2177f, 1.287f, 1.0f, 0.4145f, 4.8593f},  // PBE0
+        {1.0f, 1.703f, 1.9889f, 1.261f, 1.0f, 0.3981f, 4.4211f},  // B3LYP
+        {1.0f, 1.682f, 2.6996f, 1.094f, 1.0f, 0.4298f, 4.2359f},  // BLYP
+        {1.0f, 1.683f, 3.2822f, 1.139f, 1.0f, 0.3946f, 4.8516f},  // BP86
+        {1.0f, 1.010f, 2.3550f, 0.923f, 1.0f, 0.5238f, 3.5016f},  // REVPBE
+        {1.0f, 1.0f, 1.0f, 1.0f, 1.0f, 1.0f, 1.0f},               // CUSTOM (until set)
+    };
+    int row;
+    if (functional >= 0 && functional <= 5)
+        row = functional;
+    else if (functional == 99)
+        row = 6;  // the reference indexes FUNCTIONAL_PARAMS[99] out of bounds (src/d3_buffer.cu:283)
+    else
+        return false;
+    for (int k = 0; k < 7; ++k) out[k] = (double)tab[row][k];
+    return true;
+}
+
+Engine::Engine(const uint16_t *elements, uint64_t num_elements, uint64_t max_length, double cutoff,
+               double cn_cutoff, double atm_cutoff, int damping, int functional, int precision)
+    : rc2_(cutoff), rccn_(cn_cutoff), rc3_(atm_cutoff > 0 ? atm_cutoff : 0.0), damping_(damping),
+      functional_(functional), precision_(precision) {
+    if (!elements && num_elements) fail(D3X_ERR_INPUT, "elements is NULL");
+    if (max_length == 0 || max_length > (uint64_t)INT32_MAX / 8) fail(D3X_ERR_INPUT, "max_length out of range");
+    if (!(cutoff > 0) || !(cn_cutoff > 0) || !std::isfinite(cutoff) || !std::isfinite(cn_cutoff) ||
+        !std::isfinite(rc3_))
+        fail(D3X_ERR_INPUT, "cutoffs must be positive and finite");
+    if (damping != 0 && damping != 1) fail(D3X_ERR_INPUT, "damping_type must be 0 (zero) or 1 (BJ)");
+    if (!functional_params(functional, fp_)) fail(D3X_ERR_INPUT, "unknown functional_type");
+    if (precision != D3X_FP64 && precision != D3X_FP32) fail(D3X_ERR_INPUT, "unknown precision");
+    const Table &tab = active_table();
+    table_name_ = tab.name;
+    nref_ = tab.nref;
+    if (nref_ > MAX_NREF) fail(D3X_ERR_INPUT, "table has too many references per element");
+    for (uint64_t k = 0; k < num_elements; ++k) {
+        if (!tab.supports(elements[k]))
+            fail(D3X_ERR_INPUT, "element " + std::to_string(elements[k]) + " is not in the parameter table");
+        elems_.push_back(elements[k]);
+    }
+    std::sort(elems_.begin(), elems_.end());
+    elems_.erase(std::unique(elems_.begin(), elems_.end()), elems_.end());
+    ne_ = (int)elems_.size();
+    if (ne_ == 0) fail(D3X_ERR_INPUT, "no elements given");
+    if (ne_ > 64) fail(D3X_ERR_INPUT, "at most 64 distinct elements per handle");
+
+    // per-handle tables: C6ref rows and factorised CN references (src/d3_buffer.cu:192-277 gathers
+    // the same entries per unique element)
+    const int ne = ne_, nr = nref_;
+    std::vector<double> c6((size_t)ne * ne * nr * nr, 0.0), cnref((size_t)ne * nr, -1.0);
+    for (int A = 0; A < ne; ++A)
+        for (int a = 0; a < nr; ++a)
+            for (int B = 0; B < ne && cnref[A * nr + a] <= -1.0; ++B)
+                for (int b = 0; b < nr; ++b) {
+                    const size_t ix = tab.c6_index(tab.slot(elems_[A]), tab.slot(elems_[B]), a, b);
+                    if (tab.c6ab[ix + 1] > -1.0f && tab.c6ab[ix + 2] > -1.0f) {
+                        cnref[A * nr + a] = tab.c6ab[ix + 1];
+                        break;
+                    }
+                }
+    for (int A = 0; A < ne; ++A)
+        for (int B = 0; B < ne; ++B)
+            for (int a = 0; a < nr; ++a)
+                for (int b = 0; b < nr; ++b) {
+                    const size_t ix = tab.c6_index(tab.slot(elems_[A]), tab.slot(elems_[B]), a, b);
+                    const bool valid = tab.c6ab[ix + 1] > -1.0f && tab.c6ab[ix + 2] > -1.0f;
+                    const bool fvalid = cnref[A * nr + a] > -1.0 && cnref[B * nr + b] > -1.0;
+                    if (valid != fvalid || (valid && ((double)tab.c6ab[ix + 1] != cnref[A * nr + a] ||
+                                                      (double)tab.c6ab[ix + 2] != cnref[B * nr + b])))
+                        fail(D3X_ERR_INPUT, "parameter table CN references do not factorise per element");
+                    if (valid) c6[(((size_t)A * ne + B) * nr + a) * nr + b] = tab.c6ab[ix + 0];
+                }
+    // layout of h_tab_: c6 | cnref | rcov | r2r4 | r0ab  (q4 / rdamp derived at upload)
+    h_tab_.clear();
+    h_tab_.insert(h_tab_.end(), c6.begin(), c6.end());
+    h_tab_.insert(h_tab_.end(), cnref.begin(), cnref.end());
+    for (int A = 0; A < ne; ++A) h_tab_.push_back(tab.rcov[tab.slot(elems_[A])]);
+    for (int A = 0; A < ne; ++A) h_tab_.push_back(tab.r2r4[tab.slot(elems_[A])]);
+    for (int A = 0; A < ne; ++A)
+        for (int B = 0; B < ne; ++B) h_tab_.push_back(tab.r0ab[(size_t)tab.slot(elems_[A]) * tab.nel + tab.slot(elems_[B])]);
+
+    max_len_ = max_length;
+    // Bins about half the CN cutoff wide: the CN / chain-rule stencils then skip most corner
+    // bins, and the two-body stencil's bins stay small against its cutoff.
+    bin_width_ = std::max(4.0, 0.5 * std::min(rccn_, rc2_));
+    sys_.resize(1);
+    HIPCHK(hipStreamCreate(&own_stream_));
+    stream_ = own_stream_;
+    alloc_atoms();
+    d_status = dalloc(sizeof(unsigned));
+    HIPCHK(hipMemset(d_status, 0, sizeof(unsigned)));
+}
+
+Engine::~Engine() {
+    if (own_stream_) (void)hipStreamSynchronize(own_stream_);
+    void **all[] = {&d_pos_in, &d_type_in, &d_sys_in, &d_key, &d_perm, &d_perm2, &d_wrapped, &d_lx, &d_ly, &d_lz,
+                    &d_type_s, &d_orig_s, &d_bin_s, &d_sys_s, &d_cn, &d_w, &d_dw, &d_e, &d_g, &d_f, &d_vir,
+                    &d_npair, &d_out_f, &d_out_e, &d_out_cn, &d_out_g, &d_bin_count, &d_bin_start, &d_cursor,
+                    &d_sys, &d_st_d, &d_st_D, &d_chunk_b, &d_chunk_e, &d_chunk_s, &d_sys_chunk, &d_partial,
+                    &d_out_sys, &d_tab, &d_status, &d_atm_j, &d_atm_buf};
+    for (void **p : all) dfree(*p);
+    if (own_stream_) (void)hipStreamDestroy(own_stream_);
+}
+
+void Engine::alloc_atoms() {
+    const size_t N = max_len_;
+    const size_t ts = precision_ == D3X_FP64 ? sizeof(double) : sizeof(float);
+    d_pos_in = dalloc(3 * N * sizeof(double));
+    d_type_in = dalloc(N * sizeof(uint16_t));
+    d_sys_in = dalloc(N * sizeof(int));
+    d_key = dalloc(N * sizeof(int));
+    d_perm = dalloc(N * sizeof(int));
+    d_perm2 = dalloc(N * sizeof(int));
+    d_wrapped = dalloc(3 * N * sizeof(double));
+    d_lx = dalloc(N * ts);
+    d_ly = dalloc(N * ts);
+    d_lz = dalloc(N * ts);
+    d_type_s = dalloc(N * sizeof(uint16_t));
+    d_orig_s = dalloc(N * sizeof(int));
+    d_bin_s = dalloc(N * sizeof(int));
+    d_sys_s = dalloc(N * sizeof(int));
+    d_cn = dalloc(N * ts);
+    d_w = dalloc((size_t)nref_ * N * ts);
+    d_dw = dalloc((size_t)nref_ * N * ts);
+    d_e = dalloc(N * sizeof(double));
+    d_g = dalloc(N * sizeof(double));
+    d_f = dalloc(3 * N * sizeof(double));
+    d_vir = dalloc(6 * N * sizeof(double));
+    d_npair = dalloc(3 * N * sizeof(float));
+    d_out_f = dalloc(3 * N * sizeof(double));
+    d_out_e = dalloc(N * sizeof(double));
+    d_out_cn = dalloc(N * sizeof(double));
+    d_out_g = dalloc(N * sizeof(double));
+}
+
+int Engine::type_of(uint16_t Z) const {
+    auto it = std::lower_bound(elems_.begin(), elems_.end(), Z);
+    if (it == elems_.end() || *it != Z) return -1;
+    return (int)(it - elems_.begin());
+}
+
+void Engine::set_params(const double p[7]) {
+    for (int k = 0; k < 7; ++k)
+        if (!std::isfinite(p[k])) fail(D3X_ERR_INPUT, "functional parameters must be finite");
+    std::memcpy(fp_, p, sizeof(fp_));
+    tables_dirty_ = true;
+}
+
+void Engine::set_atoms(const double *coords_A, const uint16_t *Z, uint64_t n) {
+    if (n > max_len_)
+        fail(D3X_ERR_INPUT, "length " + std::to_string(n) + " exceeds max_length " + std::to_string(max_len_));
+    if (n && (!coords_A || !Z)) fail(D3X_ERR_INPUT, "coords or elements is NULL");
+    std::vector<uint16_t> types(n);
+    std::vector<double> pos(3 * n);
+    for (uint64_t i = 0; i < n; ++i) {
+        const int t = type_of(Z[i]);
+        if (t < 0) fail(D3X_ERR_INPUT, "element " + std::to_string(Z[i]) + " was not given at init");
+        types[i] = (uint16_t)t;
+        for (int c = 0; c < 3; ++c) {
+            const double v = coords_A[3 * i + c];
+            if (!std::isfinite(v)) fail(D3X_ERR_INPUT, "non-finite coordinate");
+            pos[3 * i + c] = v / BOHR_A;  // Angstrom -> bohr, src/d3.cu:50-56
+        }
+    }
+    if (sys_.size() != 1) {
+        sys_.assign(1, Cell{});
+    }
+    std::vector<int> sid(n, 0);
+    if (n) {
+        HIPCHK(hipMemcpyAsync(d_pos_in, pos.data(), pos.size() * sizeof(double), hipMemcpyHostToDevice, stream_));
+        HIPCHK(hipMemcpyAsync(d_type_in, types.data(), n * sizeof(uint16_t), hipMemcpyHostToDevice, stream_));
+        HIPCHK(hipMemcpyAsync(d_sys_in, sid.data(), n * sizeof(int), hipMemcpyHostToDevice, stream_));
+        HIPCHK(hipStreamSynchronize(stream_));
+    }
+    sys_[0].n = n;
+    n_ = n;
+    atoms_set_ = true;
+    geom_dirty_ = true;
+}
+
+void Engine::set_cell(const double cell_A[9]) {
+    double H[9];
+    for (int k = 0; k < 9; ++k) {
+        if (!std::isfinite(cell_A[k])) fail(D3X_ERR_INPUT, "non-finite cell");
+        H[k] = cell_A[k] / BOHR_A;
+    }
+    if (!(std::fabs(det3(H)) > 1e-12)) fail(D3X_ERR_INPUT, "singular cell");
+    if (sys_.size() != 1) sys_.assign(1, Cell{});
+    std::memcpy(sys_[0].H, H, sizeof(H));
+    sys_[0].set = true;
+    sys_[0].n = n_;
+    geom_dirty_ = true;
+}
+
+void Engine::set_batch(uint64_t nsys, const uint64_t *counts, const double *coords_A, const uint16_t *Z,
+                       const double *cells_A) {
+    if (nsys == 0 || !counts || !cells_A) fail(D3X_ERR_INPUT, "empty batch");
+    uint64_t tot = 0;
+    for (uint64_t s = 0; s < nsys; ++s) tot += counts[s];
+    if (tot > max_len_) fail(D3X_ERR_INPUT, "batch atom count exceeds max_length");
+    if (tot && (!coords_A || !Z)) fail(D3X_ERR_INPUT, "coords or elements is NULL");
+    std::vector<Cell> cells(nsys);
+    for (uint64_t s = 0; s < nsys; ++s) {
+        for (int k = 0; k < 9; ++k) {
+            if (!std::isfinite(cells_A[9 * s + k])) fail(D3X_ERR_INPUT, "non-finite cell");
+            cells[s].H[k] = cells_A[9 * s + k] / BOHR_A;
+        }
+        if (!(std::fabs(det3(cells[s].H)) > 1e-12)) fail(D3X_ERR_INPUT, "singular cell");
+        cells[s].set = true;
+        cells[s].n = counts[s];
+    }
+    std::vector<uint16_t> types(tot);
+    std::vector<double> pos(3 * tot);
+    std::vector<int> sid(tot);
+    uint64_t i = 0;
+    for (uint64_t s = 0; s < nsys; ++s)
+        for (uint64_t k = 0; k < counts[s]; ++k, ++i) {
+            const int t = type_of(Z[i]);
+            if (t < 0) fail(D3X_ERR_INPUT, "element " + std::to_string(Z[i]) + " was not given at init");
+            types[i] = (uint16_t)t;
+            sid[i] = (int)s;
+            for (int c = 0; c < 3; ++c) {
+                if (!std::isfinite(coords_A[3 * i + c])) fail(D3X_ERR_INPUT, "non-finite coordinate");
+                pos[3 * i + c] = coords_A[3 * i + c] / BOHR_A;
+            }
+        }
+    if (tot) {
+        HIPCHK(hipMemcpyAsync(d_pos_in, pos.data(), pos.size() * sizeof(double), hipMemcpyHostToDevice, stream_));
+        HIPCHK(hipMemcpyAsync(d_type_in, types.data(), tot * sizeof(uint16_t), hipMemcpyHostToDevice, stream_));
+        HIPCHK(hipMemcpyAsync(d_sys_in, sid.data(), tot * sizeof(int), hipMemcpyHostToDevice, stream_));
+        HIPCHK(hipStreamSynchronize(stream_));
+    }
+    sys_ = std::move(cells);
+    n_ = tot;
+    atoms_set_ = true;
+    geom_dirty_ = true;
+}
+
+void Engine::clear() {
+    // Every buffer that a compute reads is rewritten by the compute before use (the reference's
+    // clear zeroes accumulators it adds into, src/d3_buffer.cu:479-494); only the status resets.
+    HIPCHK(hipMemsetAsync(d_status, 0, sizeof(unsigned), stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+}
+
+int Engine::stencil_count(int cut) const {
+    int c = 0;
+    for (const auto &s : sysdev_) c += s.st_cnt[cut];
+    return c;
+}
+
+void Engine::build_geometry() {
+    sysdev_.clear();
+    st_d_.clear();
+    st_D_.clear();
+    chunk_b_.clear();
+    chunk_e_.clear();
+    chunk_s_.clear();
+    sys_chunk_off_.assign(1, 0);
+    int bin_base = 0, atom_base = 0;
+    for (size_t s = 0; s < sys_.size(); ++s) {
+        const Cell &C = sys_[s];
+        if (!C.set) fail(D3X_ERR_STATE, "cell not set");
+        SysDev d{};
+        std::memcpy(d.H, C.H, sizeof(d.H));
+        inv3(C.H, d.Hinv);
+        d.inv_vol = 1.0 / det3(C.H);
+        int64_t nbt = 1;
+        for (int k = 0; k < 3; ++k) {
+            const double nk = std::sqrt(d.Hinv[0 * 3 + k] * d.Hinv[0 * 3 + k] + d.Hinv[1 * 3 + k] * d.Hinv[1 * 3 + k] +
+                                        d.Hinv[2 * 3 + k] * d.Hinv[2 * 3 + k]);
+            const double h = 1.0 / nk;
+            d.nb[k] = std::max(1, (int)std::floor(h / bin_width_));
+            nbt *= d.nb[k];
+        }
+        if (nbt > (1 << 22)) fail(D3X_ERR_INPUT, "cell too large for the bin grid");
+        d.bin_base = bin_base;
+        d.atom_base = atom_base;
+        d.n_atoms = (int)C.n;
+        const double rcs[3] = {rccn_, rc2_, rc3_};
+        for (int cut = 0; cut < 3; ++cut) {
+            d.st_off[cut] = (int)st_d_.size();
+            if (rcs[cut] > 0) make_stencil(C.H, d.nb, rcs[cut], st_d_, st_D_);
+            d.st_cnt[cut] = (int)st_d_.size() - d.st_off[cut];
+        }
+        for (int a = 0; a < (int)C.n; a += CHUNK) {
+            chunk_b_.push_back(atom_base + a);
+            chunk_e_.push_back(atom_base + std::min<int>((int)C.n, a + CHUNK));
+            chunk_s_.push_back((int)s);
+        }
+        sys_chunk_off_.push_back((int)chunk_b_.size());
+        bin_base += (int)nbt;
+        atom_base += (int)C.n;
+        sysdev_.push_back(d);
+    }
+    nbins_ = bin_base;
+}
+
+void Engine::upload_geometry() {
+    const size_t ts = precision_ == D3X_FP64 ? sizeof(double) : sizeof(float);
+    if ((size_t)nbins_ + 1 > bins_cap_) {
+        dfree(d_bin_count);
+        dfree(d_bin_start);
+        dfree(d_cursor);
+        bins_cap_ = (size_t)nbins_ + 1;
+        d_bin_count = dalloc(bins_cap_ * sizeof(int));
+        d_bin_start = dalloc(bins_cap_ * sizeof(int));
+        d_cursor = dalloc(bins_cap_ * sizeof(int));
+    }
+    if (sysdev_.size() > sys_cap_) {
+        dfree(d_sys);
+        dfree(d_out_sys);
+        dfree(d_sys_chunk);
+        sys_cap_ = sysdev_.size();
+        d_sys = dalloc(sys_cap_ * sizeof(SysDev));
+        d_out_sys = dalloc(sys_cap_ * RED_W * sizeof(double));
+        d_sys_chunk = dalloc((sys_cap_ + 1) * sizeof(int));
+    }
+    if (st_d_.size() > st_cap_) {
+        dfree(d_st_d);
+        dfree(d_st_D);
+        st_cap_ = st_d_.size();
+        d_st_d = dalloc(st_cap_ * sizeof(StEntry));
+        d_st_D = dalloc(st_cap_ * 4 * sizeof(double));
+    }
+    if (chunk_b_.size() > chunk_cap_ || chunk_cap_ == 0) {
+        dfree(d_chunk_b);
+        dfree(d_chunk_e);
+        dfree(d_chunk_s);
+        dfree(d_partial);
+        chunk_cap_ = std::max<size_t>(1, chunk_b_.size());
+        d_chunk_b = dalloc(chunk_cap_ * sizeof(int));
+        d_chunk_e = dalloc(chunk_cap_ * sizeof(int));
+        d_chunk_s = dalloc(chunk_cap_ * sizeof(int));
+        d_partial = dalloc(chunk_cap_ * RED_W * sizeof(double));
+    }
+    HIPCHK(hipMemcpyAsync(d_sys, sysdev_.data(), sysdev_.size() * sizeof(SysDev), hipMemcpyHostToDevice, stream_));
+    if (!st_d_.empty()) {
+        HIPCHK(hipMemcpyAsync(d_st_d, st_d_.data(), st_d_.size() * sizeof(StEntry), hipMemcpyHostToDevice, stream_));
+        std::vector<unsigned char> buf(st_D_.size() * ts);
+        if (ts == sizeof(double))
+            std::memcpy(buf.data(), st_D_.data(), buf.size());
+        else
+            for (size_t k = 0; k < st_D_.size(); ++k) reinterpret_cast<float *>(buf.data())[k] = (float)st_D_[k];
+        HIPCHK(hipMemcpyAsync(d_st_D, buf.data(), buf.size(), hipMemcpyHostToDevice, stream_));
+        HIPCHK(hipStreamSynchronize(stream_));
+    }
+    if (!chunk_b_.empty()) {
+        HIPCHK(hipMemcpyAsync(d_chunk_b, chunk_b_.data(), chunk_b_.size() * sizeof(int), hipMemcpyHostToDevice, stream_));
+        HIPCHK(hipMemcpyAsync(d_chunk_e, chunk_e_.data(), chunk_e_.size() * sizeof(int), hipMemcpyHostToDevice, stream_));
+        HIPCHK(hipMemcpyAsync(d_chunk_s, chunk_s_.data(), chunk_s_.size() * sizeof(int), hipMemcpyHostToDevice, stream_));
+    }
+    HIPCHK(hipMemcpyAsync(d_sys_chunk, sys_chunk_off_.data(), sys_chunk_off_.size() * sizeof(int),
+                          hipMemcpyHostToDevice, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+}
+
+void Engine::ensure_atm_scratch() {
+    if (rc3_ <= 0) return;
+    // neighbour capacity: three times the mean count within rc3 plus a margin; a centre above it
+    // raises D3X_ERR_OVERFLOW and the caller retries with a larger capacity
+    double dens = 0;
+    for (size_t s = 0; s < sys_.size(); ++s) dens = std::max(dens, (double)sys_[s].n / std::fabs(det3(sys_[s].H)));
+    const double mean = dens * 4.18879020478639 * rc3_ * rc3_ * rc3_;
+    int cap = (int)std::min(262144.0, 3.0 * mean + 1024.0);
+    cap = std::max(cap, atm_cap_);
+    int nb = atm_blocks_req_ > 0 ? atm_blocks_req_ : 1024;
+    nb = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)nb, n_));
+    atm_cap_ = cap;
+    atm_blocks_ = nb;
+    // the kernel indexes scratch as blockIdx.x * atm_cap_: reallocate whenever either grows
+    if ((size_t)cap * nb > atm_alloc_entries_ || !d_atm_j) {
+        dfree(d_atm_j);
+        dfree(d_atm_buf);
+        const size_t ts = precision_ == D3X_FP64 ? sizeof(double) : sizeof(float);
+        atm_alloc_entries_ = (size_t)cap * nb;
+        d_atm_j = dalloc(atm_alloc_entries_ * sizeof(int));
+        d_atm_buf = dalloc(atm_alloc_entries_ * 6 * ts);
+    }
+}
+
+template <class T>
+Params<T> Engine::params() {
+    Params<T> P{};
+    P.n = (int)n_;
+    P.ne = ne_;
+    P.nref = nref_;
+    P.nsys = (int)sysdev_.size();
+    P.damping = damping_;
+    P.rc_cn2 = (T)(rccn_ * rccn_);
+    P.rc_22 = (T)(rc2_ * rc2_);
+    P.rc_32 = (T)(rc3_ * rc3_);
+    P.s6 = (T)fp_[0];
+    if (damping_ == 0) {
+        P.s8 = (T)fp_[1];
+        P.p1 = (T)fp_[3];
+        P.p2 = (T)fp_[4];
+    } else {
+        P.s8 = (T)fp_[2];
+        P.p1 = (T)fp_[5];
+        P.p2 = (T)fp_[6];
+    }
+    P.sys = (const SysDev *)d_sys;
+    P.st_d = (const StEntry *)d_st_d;
+    P.st_D = (const T *)d_st_D;
+    P.bin_start = (const int *)d_bin_start;
+    P.pos_in = (const double *)d_pos_in;
+    P.type_in = (const uint16_t *)d_type_in;
+    P.sys_in = (const int *)d_sys_in;
+    P.key_in = (int *)d_key;
+    P.bin_count = (int *)d_bin_count;
+    P.cursor = (int *)d_cursor;
+    P.perm = (int *)d_perm;
+    P.perm2 = (int *)d_perm2;
+    P.wrapped = (double *)d_wrapped;
+    P.lx = (T *)d_lx;
+    P.ly = (T *)d_ly;
+    P.lz = (T *)d_lz;
+    P.type_s = (uint16_t *)d_type_s;
+    P.orig_s = (int *)d_orig_s;
+    P.bin_s = (int *)d_bin_s;
+    P.sys_s = (int *)d_sys_s;
+    P.cn = (T *)d_cn;
+    P.w = (T *)d_w;
+    P.dw = (T *)d_dw;
+    P.e = (double *)d_e;
+    P.g = (double *)d_g;
+    P.f = (double *)d_f;
+    P.vir = (double *)d_vir;
+    P.npair = (float *)d_npair;
+    const T *tab = (const T *)d_tab;
+    P.c6 = tab + tab_off_[0];
+    P.cnref = tab + tab_off_[1];
+    P.rcov = tab + tab_off_[2];
+    P.q4 = tab + tab_off_[3];
+    P.rdamp = tab + tab_off_[4];
+    P.r0 = tab + tab_off_[5];
+    P.atm_cap = atm_cap_;
+    P.atm_j = (int *)d_atm_j;
+    P.atm_buf = (T *)d_atm_buf;
+    P.status = (unsigned *)d_status;
+    return P;
+}
+
+template <class T>
+void Engine::run() {
+    if (tables_dirty_) {
+        // c6 | cnref | rcov | q4 | rdamp | r0 in T
+        const int ne = ne_, nr = nref_;
+        const size_t nc6 = (size_t)ne * ne * nr * nr, ncr = (size_t)ne * nr;
+        const double *c6 = h_tab_.data(), *cnref = c6 + nc6, *rcov = cnref + ncr, *r2r4 = rcov + ne,
+                     *r0ab = r2r4 + ne;
+        std::vector<T> t;
+        tab_off_[0] = t.size();
+        for (size_t k = 0; k < nc6; ++k) t.push_back((T)c6[k]);
+        tab_off_[1] = t.size();
+        for (size_t k = 0; k < ncr; ++k) t.push_back((T)cnref[k]);
+        tab_off_[2] = t.size();
+        for (int A = 0; A < ne; ++A) t.push_back((T)rcov[A]);
+        tab_off_[3] = t.size();
+        for (int A = 0; A < ne; ++A)
+            for (int B = 0; B < ne; ++B) t.push_back((T)(r2r4[A] * r2r4[B]));
+        tab_off_[4] = t.size();
+        for (int A = 0; A < ne; ++A)
+            for (int B = 0; B < ne; ++B) {
+                // damping radius: zero -> r0ab; BJ -> a1 sqrt(3 r2r4_A r2r4_B) + a2 (src/d3_kernel.cu:1083,388)
+                const double rd = damping_ == 0 ? r0ab[A * ne + B] : fp_[5] * std::sqrt(3.0 * r2r4[A] * r2r4[B]) + fp_[6];
+                t.push_back((T)rd);
+            }
+        tab_off_[5] = t.size();
+        for (int k = 0; k < ne * ne; ++k) t.push_back((T)r0ab[k]);
+        dfree(d_tab);
+        d_tab = dalloc(t.size() * sizeof(T));
+        HIPCHK(hipMemcpyAsync(d_tab, t.data(), t.size() * sizeof(T), hipMemcpyHostToDevice, stream_));
+        HIPCHK(hipStreamSynchronize(stream_));
+        tables_dirty_ = false;
+    }
+    ensure_atm_scratch();
+    Params<T> P = params<T>();
+    launch_fill_u32((unsigned *)d_status, 1, 0u, stream_);
+    Launch<T>::bin_sort(P, nbins_, (int *)d_bin_start, stream_);
+    Launch<T>::cn(P, stream_);
+    Launch<T>::twobody(P, stream_);
+    if (rc3_ > 0) Launch<T>::atm(P, atm_blocks_, stream_);
+    Launch<T>::chain(P, stream_);
+    launch_reduce((const double *)d_e, (const double *)d_vir, (const float *)d_npair, (const int *)d_chunk_b,
+                  (const int *)d_chunk_e, (const int *)d_chunk_s, (int)chunk_b_.size(), (const int *)d_sys_chunk,
+                  (int)sysdev_.size(), (double *)d_partial, (double *)d_out_sys, stream_);
+    launch_finalize(P, (double *)d_out_f, (double *)d_out_e, (double *)d_out_cn, (double *)d_out_g, stream_);
+    HIPCHK(hipGetLastError());
+}
+
+void Engine::compute_device() {
+    if (!atoms_set_) fail(D3X_ERR_STATE, "set_atoms has not been called");
+    for (const Cell &c : sys_)
+        if (!c.set) fail(D3X_ERR_STATE, "set_cell has not been called");
+    if (geom_dirty_) {
+        build_geometry();
+        upload_geometry();
+        geom_dirty_ = false;
+    }
+    if (precision_ == D3X_FP64)
+        run<double>();
+    else
+        run<float>();
+}
+
+void Engine::fetch(double *energies, double *forces, double *stresses) {
+    const size_t ns = sysdev_.size();
+    std::vector<double> red(ns * RED_W);
+    unsigned status = 0;
+    HIPCHK(hipMemcpyAsync(red.data(), d_out_sys, red.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    if (forces && n_)
+        HIPCHK(hipMemcpyAsync(forces, d_out_f, 3 * n_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipMemcpyAsync(&status, d_status, sizeof(unsigned), hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    if (status & 2u) {
+        HIPCHK(hipMemsetAsync(d_status, 0, sizeof(unsigned), stream_));
+        throw Error{D3X_ERR_GENERIC | D3X_ERR_OVERFLOW, "three-body neighbour list capacity exceeded"};
+    }
+    const double sfac = HARTREE_EV / (BOHR_A * BOHR_A * BOHR_A);  // src/d3.cu:213-219
+    for (size_t s = 0; s < ns; ++s) {
+        const double *r = &red[s * RED_W];
+        if (energies) energies[s] = -HARTREE_EV * r[0];
+        if (stresses) {
+            const double v[6] = {r[1], r[2], r[3], r[4], r[5], r[6]};  // xx yy zz xy xz yz
+            double *o = stresses + 9 * s;
+            o[0] = v[0] * sfac;
+            o[1] = v[3] * sfac;
+            o[2] = v[4] * sfac;
+            o[3] = v[3] * sfac;
+            o[4] = v[1] * sfac;
+            o[5] = v[5] * sfac;
+            o[6] = v[4] * sfac;
+            o[7] = v[5] * sfac;
+            o[8] = v[2] * sfac;
+        }
+    }
+}
+
+void Engine::fetch_atom_data(double *cn, double *dedcn, double *atom_energy) {
+    if (n_ == 0) return;
+    if (cn) HIPCHK(hipMemcpyAsync(cn, d_out_cn, n_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    if (dedcn) HIPCHK(hipMemcpyAsync(dedcn, d_out_g, n_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    if (atom_energy) HIPCHK(hipMemcpyAsync(atom_energy, d_out_e, n_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+}
+
+void Engine::fetch_counts(double *counts) {
+    const size_t ns = sysdev_.size();
+    std::vector<double> red(ns * RED_W);
+    HIPCHK(hipMemcpyAsync(red.data(), d_out_sys, red.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    for (size_t s = 0; s < ns; ++s)
+        for (int k = 0; k < 3; ++k) counts[3 * s + k] = red[s * RED_W + 7 + k];
+}
+
+template Params<double> Engine::params<double>();
+template Params<float> Engine::params<float>();
+
+}  // namespace d3

@@ -1,0 +1,134 @@
+// Host side of one D3 handle: parameters, device buffers, cell geometry (bins and stencils)
+// and the launch sequence.  Replaces the reference's Device_Buffer (src/d3_buffer.cu:151-634)
+// and the orchestration of src/d3.cu:130-226.
+#pragma once
+#include <algorithm>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "d3_engine.hpp"
+
+namespace d3 {
+
+struct Error {
+    uint32_t code;
+    std::string msg;
+};
+
+class Engine {
+   public:
+    Engine(const uint16_t *elements, uint64_t num_elements, uint64_t max_length, double cutoff, double cn_cutoff,
+           double atm_cutoff, int damping, int functional, int precision);
+    ~Engine();
+    Engine(const Engine &) = delete;
+    Engine &operator=(const Engine &) = delete;
+
+    void set_params(const double p[7]);
+    // One cell: coordinates in Angstrom (caller order), atomic numbers.
+    void set_atoms(const double *coords_A, const uint16_t *Z, uint64_t n);
+    void set_cell(const double cell_A[9]);
+    // Several cells at once (replaces the single-cell state).
+    void set_batch(uint64_t nsys, const uint64_t *counts, const double *coords_A, const uint16_t *Z,
+                   const double *cells_A);
+    void clear();
+    // Enqueues the full pipeline on the stream (no synchronisation).
+    void compute_device();
+    // Synchronises and copies results: per-cell energy (eV) and stress (eV/A^3, 9 per cell),
+    // forces (eV/A, caller order).  Throws Error on device status.
+    void fetch(double *energies, double *forces, double *stresses);
+    void fetch_atom_data(double *cn, double *dedcn, double *atom_energy);
+    // Pair counts of the last evaluation per cell: two-body ordered pairs, CN ordered pairs,
+    // ATM centre-pairs (each unique triangle counted three times).
+    void fetch_counts(double *counts);
+
+    uint64_t num_atoms() const { return n_; }
+    uint64_t max_length() const { return max_len_; }
+    int num_sys() const { return (int)sys_.size(); }
+    hipStream_t stream() const { return stream_; }
+    void set_stream(hipStream_t s) { stream_ = s ? s : own_stream_; }
+    int precision() const { return precision_; }
+    bool use_atm() const { return rc3_ > 0; }
+    // Bin width (bohr) used for new cells; tunable.
+    void set_bin_width(double w) { bin_width_ = w; geom_dirty_ = true; }
+    double bin_width() const { return bin_width_; }
+    void set_atm_blocks(int nb) { atm_blocks_req_ = nb; }
+    // Quadruples the three-body neighbour capacity (after a D3X_ERR_OVERFLOW).
+    void grow_atm_cap() { atm_cap_ = std::max(atm_cap_, 1024) * 4; }
+    int atm_cap() const { return atm_cap_; }
+    const std::vector<SysDev> &sysdev() const { return sysdev_; }
+    int stencil_count(int cut) const;
+
+   private:
+    struct Cell {
+        double H[9] = {0};  // bohr
+        bool set = false;
+        uint64_t n = 0;
+    };
+    template <class T>
+    Params<T> params();
+    template <class T>
+    void run();
+    void build_geometry();
+    void upload_geometry();
+    void alloc_atoms();
+    void ensure_atm_scratch();
+    int type_of(uint16_t Z) const;
+
+    // configuration
+    std::vector<uint16_t> elems_;  // sorted unique atomic numbers of the handle
+    int ne_ = 0, nref_ = 0;
+    uint64_t max_len_ = 0;
+    double rc2_, rccn_, rc3_;
+    int damping_, functional_, precision_;
+    double fp_[7];
+    double bin_width_;
+    int atm_blocks_req_ = 0;
+    std::string table_name_;
+
+    // state
+    uint64_t n_ = 0;
+    std::vector<Cell> sys_;
+    bool atoms_set_ = false, geom_dirty_ = true, tables_dirty_ = true;
+    std::vector<SysDev> sysdev_;
+    std::vector<StEntry> st_d_;
+    std::vector<double> st_D_;
+    std::vector<int> chunk_b_, chunk_e_, chunk_s_, sys_chunk_off_;
+    int nbins_ = 0;
+    std::vector<uint16_t> h_types_;
+    std::vector<double> h_tab_;  // type tables in double (converted to T at upload)
+
+    // device
+    hipStream_t own_stream_ = nullptr, stream_ = nullptr;
+    void *d_pos_in = nullptr, *d_type_in = nullptr, *d_sys_in = nullptr;
+    void *d_key = nullptr, *d_perm = nullptr, *d_perm2 = nullptr, *d_wrapped = nullptr;
+    void *d_lx = nullptr, *d_ly = nullptr, *d_lz = nullptr, *d_type_s = nullptr, *d_orig_s = nullptr,
+         *d_bin_s = nullptr, *d_sys_s = nullptr;
+    void *d_cn = nullptr, *d_w = nullptr, *d_dw = nullptr, *d_e = nullptr, *d_g = nullptr, *d_f = nullptr,
+         *d_vir = nullptr, *d_npair = nullptr;
+    void *d_out_f = nullptr, *d_out_e = nullptr, *d_out_cn = nullptr, *d_out_g = nullptr;
+    void *d_bin_count = nullptr, *d_bin_start = nullptr, *d_cursor = nullptr;
+    size_t bins_cap_ = 0;
+    void *d_sys = nullptr, *d_st_d = nullptr, *d_st_D = nullptr;
+    size_t sys_cap_ = 0, st_cap_ = 0;
+    void *d_chunk_b = nullptr, *d_chunk_e = nullptr, *d_chunk_s = nullptr, *d_sys_chunk = nullptr,
+         *d_partial = nullptr, *d_out_sys = nullptr;
+    size_t chunk_cap_ = 0, sysred_cap_ = 0;
+    void *d_tab = nullptr;
+    size_t tab_off_[6] = {0};
+    void *d_status = nullptr;
+    void *d_atm_j = nullptr, *d_atm_buf = nullptr;
+    int atm_blocks_ = 0, atm_cap_ = 0;
+    size_t atm_alloc_entries_ = 0;  // allocated scratch entries (>= atm_blocks_ * atm_cap_)
+};
+
+void launch_fill_u32(unsigned *p, int n, unsigned v, hipStream_t s);
+
+// Thread-local last error message of the C API.
+void set_last_error(const std::string &msg);
+const char *last_error();
+
+// Functional parameter table {s6, s8_zero, s8_bj, sr6, sr8, a1, a2}; functional_t order.
+bool functional_params(int functional, double out[7]);
+
+}  // namespace d3

@@ -1,0 +1,859 @@
+// HIP kernels of the D3 pipeline for gfx950 (CDNA4, wave64).
+//
+// Pipeline per evaluation (all on one stream, no host synchronisation in between):
+//   bin sort   k_zero -> k_bin -> k_scan -> k_scatter -> k_canon -> k_gather
+//   k_cn       coordination numbers + normalised CN weights per atom
+//   k_2b       two-body energy, dE/dCN, force, virial
+//   k_atm      Axilrod-Teller-Muto three-body term (adds to energy, dE/dCN, force, virial)
+//   k_chain    dE/dCN chain-rule force and virial
+//   reduce / finalize (d3_reduce.hip)
+// The formulas restate the reference's src/d3_kernel.cu; each kernel cites its counterpart.
+// T = double is the plain FP64 path, T = float the FP32 path with compensated accumulation.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+
+#include "d3_engine.hpp"
+
+namespace d3 {
+namespace {
+
+constexpr int WAVE = 64;
+constexpr int BLOCK = 256;
+constexpr int WPB = BLOCK / WAVE;
+constexpr int QCAP = 2 * WAVE;  // per-wave compaction queue (entries)
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & (WAVE - 1); }
+__device__ __forceinline__ uint64_t ballot64(bool p) { return __ballot(p); }
+__device__ __forceinline__ int mbcnt64(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+// Orders LDS traffic between the lanes of one wave (the queue is private to the wave).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+__device__ __forceinline__ int wrapi(int x, int n) {
+    int r = x % n;
+    return r < 0 ? r + n : r;
+}
+
+__device__ __forceinline__ double xsqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float xsqrt(float x) { return sqrtf(x); }
+__device__ __forceinline__ double xexp(double x) { return exp(x); }
+__device__ __forceinline__ float xexp(float x) { return expf(x); }
+__device__ __forceinline__ double xcbrt(double x) { return cbrt(x); }
+__device__ __forceinline__ float xcbrt(float x) { return cbrtf(x); }
+__device__ __forceinline__ double xrsqrt(double x) { return rsqrt(x); }
+__device__ __forceinline__ float xrsqrt(float x) { return rsqrtf(x); }
+template <class T>
+__device__ __forceinline__ T finite_or_zero(T x) {
+    return isfinite(x) ? x : T(0);
+}
+
+// Per-lane accumulator: plain sum in FP64; two-sum compensated (hi, lo) in FP32.
+template <class T>
+struct Acc {
+    T s;
+    __device__ void init() { s = T(0); }
+    __device__ void add(T x) { s += x; }
+};
+template <>
+struct Acc<float> {
+    float hi, lo;
+    __device__ void init() { hi = lo = 0.0f; }
+    __device__ void add(float x) {
+        float s = hi + x;
+        float bp = s - hi;
+        lo += (hi - (s - bp)) + (x - bp);
+        hi = s;
+    }
+};
+
+// Wave-wide total of an accumulator, returned in double in every lane (xor butterfly over the
+// 64 lanes; the FP32 variant keeps the exact two-sum error of every step).
+__device__ __forceinline__ double wave_total(const Acc<double> &a) {
+    double v = a.s;
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+    return v;
+}
+__device__ __forceinline__ double wave_total(const Acc<float> &a) {
+    float hi = a.hi, lo = a.lo;
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) {
+        float oh = __shfl_xor(hi, o, WAVE), ol = __shfl_xor(lo, o, WAVE);
+        float s = hi + oh;
+        float bp = s - hi;
+        float err = (hi - (s - bp)) + (oh - bp);
+        hi = s;
+        lo = lo + ol + err;
+    }
+    return (double)hi + (double)lo;
+}
+__device__ __forceinline__ float wave_sumf(float v) {
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+    return v;
+}
+
+// Geometry of one centre atom: its cell, bin coordinates and local position.
+template <class T>
+struct Centre {
+    int i, sid, ti, bx, by, bz, nx, ny, nz, bin_base;
+    T x, y, z;
+    const SysDev *S;
+    __device__ void load(const Params<T> &P, int i_) {
+        i = i_;
+        sid = P.sys_s[i];
+        S = &P.sys[sid];
+        nx = S->nb[0];
+        ny = S->nb[1];
+        nz = S->nb[2];
+        bin_base = S->bin_base;
+        const int b = P.bin_s[i] - bin_base;
+        bx = b % nx;
+        by = (b / nx) % ny;
+        bz = b / (nx * ny);
+        x = P.lx[i];
+        y = P.ly[i];
+        z = P.lz[i];
+        ti = P.type_s[i];
+    }
+    // Atom range of the bin at stencil entry s, and the centre's position plus the image vector.
+    __device__ void bin(const Params<T> &P, int s, int &j0, int &j1, T &ox, T &oy, T &oz) const {
+        const StEntry d = P.st_d[s];
+        const int cx = wrapi(bx + d.dx, nx), cy = wrapi(by + d.dy, ny), cz = wrapi(bz + d.dz, nz);
+        const int nb = bin_base + cx + nx * (cy + ny * cz);
+        j0 = P.bin_start[nb];
+        j1 = P.bin_start[nb + 1];
+        ox = x + P.st_D[4 * s + 0];
+        oy = y + P.st_D[4 * s + 1];
+        oz = z + P.st_D[4 * s + 2];
+    }
+};
+
+// ---------------------------------------------------------------------------------------------
+// Bin sort: counting sort of atoms by bin, then a canonical (ascending caller index) order
+// inside every bin so that all later sums run in a fixed order.  Replaces the host counting
+// sort of src/d3_buffer.cu:496-634 (wrap to [0,1) with the edge fix of :556-560).
+// ---------------------------------------------------------------------------------------------
+__global__ void k_fill_i32(int *p, int n, int v) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = v;
+}
+
+template <class T>
+__global__ void k_bin(Params<T> P) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const SysDev &S = P.sys[P.sys_in[i]];
+    const double r[3] = {P.pos_in[3 * i], P.pos_in[3 * i + 1], P.pos_in[3 * i + 2]};
+    int b[3];
+    double fl[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double s = r[0] * S.Hinv[0 * 3 + k] + r[1] * S.Hinv[1 * 3 + k] + r[2] * S.Hinv[2 * 3 + k];
+        fl[k] = floor(s);
+        const double u = s - fl[k];
+        int bk = (int)(u * S.nb[k]);
+        if (bk >= S.nb[k]) {  // u rounded to 1.0: the atom belongs to bin 0 of the next image
+            bk = 0;
+            fl[k] += 1.0;
+        }
+        if (bk < 0) bk = 0;
+        b[k] = bk;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+        P.wrapped[3 * i + c] = r[c] - fl[0] * S.H[0 * 3 + c] - fl[1] * S.H[1 * 3 + c] - fl[2] * S.H[2 * 3 + c];
+    const int key = S.bin_base + b[0] + S.nb[0] * (b[1] + S.nb[1] * b[2]);
+    P.key_in[i] = key;
+    atomicAdd(&P.bin_count[key], 1);
+}
+
+// Exclusive scan of bin_count into bin_start[0..nbins] by one block.
+__global__ void k_scan(const int *count, int *start, int *cursor, int nbins) {
+    __shared__ int part[1024];
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < nbins; base += 1024) {
+        const int t = base + threadIdx.x;
+        const int v = t < nbins ? count[t] : 0;
+        part[threadIdx.x] = v;
+        __syncthreads();
+        for (int o = 1; o < 1024; o <<= 1) {  // Hillis-Steele inclusive scan
+            int add = threadIdx.x >= o ? part[threadIdx.x - o] : 0;
+            __syncthreads();
+            part[threadIdx.x] += add;
+            __syncthreads();
+        }
+        if (t < nbins) {
+            const int ex = carry + part[threadIdx.x] - v;
+            start[t] = ex;
+            cursor[t] = ex;
+        }
+        __syncthreads();
+        if (threadIdx.x == 1023) carry += part[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) start[nbins] = carry;
+}
+
+template <class T>
+__global__ void k_scatter(Params<T> P) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const int slot = atomicAdd(&P.cursor[P.key_in[i]], 1);
+    P.perm[slot] = i;
+}
+
+// One block per bin: rank of every entry among the bin's entries (ascending caller index).
+__global__ void k_canon(const int *bin_start, const int *perm, int *perm2) {
+    const int b = blockIdx.x;
+    const int s = bin_start[b], e = bin_start[b + 1];
+    for (int t = s + threadIdx.x; t < e; t += blockDim.x) {
+        const int v = perm[t];
+        int rank = 0;
+        for (int u = s; u < e; ++u) rank += perm[u] < v;
+        perm2[s + rank] = v;
+    }
+}
+
+template <class T>
+__global__ void k_gather(Params<T> P) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P.n) return;
+    const int i = P.perm2[p];
+    const int key = P.key_in[i];
+    const int sid = P.sys_in[i];
+    const SysDev &S = P.sys[sid];
+    const int b = key - S.bin_base;
+    const int b0 = b % S.nb[0], b1 = (b / S.nb[0]) % S.nb[1], b2 = b / (S.nb[0] * S.nb[1]);
+    const double u0 = (b0 + 0.5) / S.nb[0], u1 = (b1 + 0.5) / S.nb[1], u2 = (b2 + 0.5) / S.nb[2];
+    double l[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+        l[c] = P.wrapped[3 * i + c] - (u0 * S.H[0 * 3 + c] + u1 * S.H[1 * 3 + c] + u2 * S.H[2 * 3 + c]);
+    P.lx[p] = (T)l[0];
+    P.ly[p] = (T)l[1];
+    P.lz[p] = (T)l[2];
+    P.type_s[p] = P.type_in[i];
+    P.orig_s[p] = i;
+    P.bin_s[p] = key;
+    P.sys_s[p] = sid;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Coordination number, src/d3_kernel.cu:544-574 (calculate_CN) and 582-683: one wave per
+// centre atom, lanes stride over the atoms of each stencil bin.  CN_i = sum over
+// 0 < r^2 <= rc_CN^2 of 1 / (1 + exp(-k1 ((Rcov_i + Rcov_j) / r - 1))).  The same wave then
+// forms the centre's normalised Gaussian weights of the C6 interpolation:
+//   w_a = exp(-k3 (CN - CNref_a)^2) / sum_b exp(-k3 (CN - CNref_b)^2), a valid iff CNref_a > -1
+//   dw_a/dCN = w_a (l_a - sum_b w_b l_b), l_a = -2 k3 (CN - CNref_a)
+// With CN references that depend only on (element, reference) -- checked at handle init -- the
+// reference's 25-term Z/W of src/d3_kernel.cu:709-800 equals w_A^T C6ref w_B exactly.
+// ---------------------------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_cn(Params<T> P) {
+    const int lane = lane_id();
+    const int i = blockIdx.x * WPB + (threadIdx.x >> 6);
+    if (i >= P.n) return;
+    Centre<T> C;
+    C.load(P, i);
+    const T rci = P.rcov[C.ti];
+    Acc<T> acc;
+    acc.init();
+    float cnt = 0.0f;
+    const int s0 = C.S->st_off[CUT_CN], s1 = s0 + C.S->st_cnt[CUT_CN];
+    for (int s = s0; s < s1; ++s) {
+        int j0, j1;
+        T ox, oy, oz;
+        C.bin(P, s, j0, j1, ox, oy, oz);
+        for (int j = j0 + lane; j < j1; j += WAVE) {
+            const T dx = ox - P.lx[j], dy = oy - P.ly[j], dz = oz - P.lz[j];
+            const T r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 <= P.rc_cn2 && r2 > T(0)) {
+                const T r = xsqrt(r2);
+                const T ex = xexp(-T(K1) * ((rci + P.rcov[P.type_s[j]]) / r - T(1)));
+                acc.add(T(1) / (T(1) + ex));
+                cnt += 1.0f;
+            }
+        }
+    }
+    const double cnd = wave_total(acc);
+    cnt = wave_sumf(cnt);
+    if (lane == 0) {
+        const T cn = (T)cnd;
+        P.cn[i] = cn;
+        P.npair[3 * i + 1] = cnt;
+        T ex[MAX_NREF], la[MAX_NREF];
+        T mx = -T(FLT_MAX);
+        for (int a = 0; a < P.nref; ++a) {
+            const T ref = P.cnref[C.ti * P.nref + a];
+            const T dcn = cn - ref;
+            ex[a] = (ref > T(-1)) ? -T(K3) * dcn * dcn : -T(FLT_MAX);
+            la[a] = -T(2.0 * K3) * dcn;
+            mx = ex[a] > mx ? ex[a] : mx;
+        }
+        T sum = T(0);
+        for (int a = 0; a < P.nref; ++a) {
+            ex[a] = (ex[a] > -T(FLT_MAX)) ? xexp(ex[a] - mx) : T(0);
+            sum += ex[a];
+        }
+        T mean = T(0);
+        for (int a = 0; a < P.nref; ++a) {
+            ex[a] /= sum;
+            mean += ex[a] * la[a];
+        }
+        for (int a = 0; a < P.nref; ++a) {
+            P.w[a * P.n + i] = ex[a];
+            P.dw[a * P.n + i] = ex[a] * (la[a] - mean);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Two-body term, src/d3_kernel.cu:902-986 (calculate_twobody) and 997-1197 (two_body_kernel).
+// One wave per centre; lanes test 64 candidates at a time, in-range pairs (0 < r^2 < rc^2,
+// strict) are compacted through an LDS queue with ballot/mbcnt, and every 64 queued pairs are
+// evaluated by the full wave.  Per pair (A = type i, B = type j):
+//   C6 = w_i^T C6ref_AB w_j = y_i[B] . w_j with y_i[B] = C6ref_AB^T w_i (per-centre, in LDS)
+//   dC6/dCN_i = dy_i[B] . w_j,  C8 = 3 C6 r2r4_A r2r4_B
+//   E+_i += (s6 C6 f6 / r^6 + s8 C8 f8 / r^8) / 2,  dE/dCN_i += s6 f6 dC6 / r^6 + s8 f8 dC8 / r^8
+//   phi = (1/r) d/dr (s6 C6 f6 r^-6 + s8 C8 f8 r^-8),  F_i += phi d,  vir += -phi d d^T / 2
+// Damping (src/d3_kernel.cu:359-424): zero f_n = 1/(1 + 6 (s_rn R0/r)^a_n), a = 14, 16, R0 = r0ab;
+// BJ f_n = r^n / (r^n + (a1 sqrt(3 r2r4_A r2r4_B) + a2)^n).
+// ---------------------------------------------------------------------------------------------
+template <class T>
+struct Queue {
+    int j[QCAP];
+    T dx[QCAP], dy[QCAP], dz[QCAP];
+};
+
+template <class T>
+struct Acc2b {
+    Acc<T> e, g, f[3], v[6];
+    __device__ void init() {
+        e.init();
+        g.init();
+        for (int k = 0; k < 3; ++k) f[k].init();
+        for (int k = 0; k < 6; ++k) v[k].init();
+    }
+};
+
+template <class T>
+__device__ __forceinline__ void pair_2b(const Params<T> &P, int j, T dx, T dy, T dz, const T *y, const T *dy_,
+                                        const T *rq, const T *rR, Acc2b<T> &A) {
+    const int tj = P.type_s[j];
+    T c6 = T(0), dc6 = T(0);
+    for (int b = 0; b < P.nref; ++b) {
+        const T wj = P.w[b * P.n + j];
+        c6 += y[tj * P.nref + b] * wj;
+        dc6 += dy_[tj * P.nref + b] * wj;
+    }
+    const T q = rq[tj], R = rR[tj];
+    const T c8 = T(3) * c6 * q, dc8 = T(3) * dc6 * q;
+    const T r2 = dx * dx + dy * dy + dz * dz;
+    const T ir = xrsqrt(r2);
+    const T ir2 = ir * ir;
+    const T ir6 = ir2 * ir2 * ir2, ir8 = ir6 * ir2;
+    T f6, f8, df6, df8;
+    if (P.damping == 0) {
+        const T t = R * ir;
+        const T b6 = P.p1 * t, b8 = P.p2 * t;
+        const T b6_2 = b6 * b6, b6_4 = b6_2 * b6_2, b6_8 = b6_4 * b6_4, b6_14 = b6_8 * b6_4 * b6_2;
+        const T b8_2 = b8 * b8, b8_4 = b8_2 * b8_2, b8_8 = b8_4 * b8_4, b8_16 = b8_8 * b8_8;
+        f6 = T(1) / (T(1) + T(6) * b6_14);
+        f8 = T(1) / (T(1) + T(6) * b8_16);
+        df6 = T(84) * f6 * f6 * b6_14 * ir;
+        df8 = T(96) * f8 * f8 * b8_16 * ir;
+    } else {
+        const T a2 = R * R, a6 = a2 * a2 * a2, a8 = a6 * a2;
+        const T r6 = r2 * r2 * r2, r8 = r6 * r2;
+        const T d6 = T(1) / (r6 + a6), d8 = T(1) / (r8 + a8);
+        f6 = r6 * d6;
+        f8 = r8 * d8;
+        df6 = T(6) * a6 * r6 * ir * d6 * d6;
+        df8 = T(8) * a8 * r8 * ir * d8 * d8;
+    }
+    f6 = finite_or_zero(f6);
+    f8 = finite_or_zero(f8);
+    df6 = finite_or_zero(df6);
+    df8 = finite_or_zero(df8);
+    const T e6 = P.s6 * c6 * f6 * ir6, e8 = P.s8 * c8 * f8 * ir8;
+    A.e.add(T(0.5) * (e6 + e8));
+    A.g.add(P.s6 * f6 * dc6 * ir6 + P.s8 * f8 * dc8 * ir8);
+    const T phi = (P.s6 * c6 * (df6 - T(6) * f6 * ir) * ir6 + P.s8 * c8 * (df8 - T(8) * f8 * ir) * ir8) * ir;
+    A.f[0].add(phi * dx);
+    A.f[1].add(phi * dy);
+    A.f[2].add(phi * dz);
+    const T h = T(-0.5) * phi;
+    A.v[0].add(h * dx * dx);
+    A.v[1].add(h * dy * dy);
+    A.v[2].add(h * dz * dz);
+    A.v[3].add(h * dx * dy);
+    A.v[4].add(h * dx * dz);
+    A.v[5].add(h * dy * dz);
+}
+
+template <class T>
+__host__ __device__ constexpr size_t twobody_wave_lds(int ne, int nref) {
+    return ((sizeof(Queue<T>) + (size_t)(2 * ne * nref + 2 * ne) * sizeof(T)) + 15) / 16 * 16;
+}
+
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_2b(Params<T> P) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int wv = threadIdx.x >> 6, lane = lane_id();
+    const int i = blockIdx.x * WPB + wv;
+    if (i >= P.n) return;
+    const int ynr = P.ne * P.nref;
+    unsigned char *base = smem + wv * twobody_wave_lds<T>(P.ne, P.nref);
+    Queue<T> &Q = *reinterpret_cast<Queue<T> *>(base);
+    T *y = reinterpret_cast<T *>(base + sizeof(Queue<T>));
+    T *dyv = y + ynr, *rq = dyv + ynr, *rR = rq + P.ne;
+
+    Centre<T> C;
+    C.load(P, i);
+    T wi[MAX_NREF], dwi[MAX_NREF];
+    for (int a = 0; a < MAX_NREF; ++a) {
+        wi[a] = a < P.nref ? P.w[a * P.n + i] : T(0);
+        dwi[a] = a < P.nref ? P.dw[a * P.n + i] : T(0);
+    }
+    for (int idx = lane; idx < ynr; idx += WAVE) {
+        const int B = idx / P.nref, bb = idx % P.nref;
+        T sy = T(0), sdy = T(0);
+        for (int a = 0; a < P.nref; ++a) {
+            const T c = P.c6[((C.ti * P.ne + B) * P.nref + a) * P.nref + bb];
+            sy += c * wi[a];
+            sdy += c * dwi[a];
+        }
+        y[idx] = sy;
+        dyv[idx] = sdy;
+    }
+    for (int B = lane; B < P.ne; B += WAVE) {
+        rq[B] = P.q4[C.ti * P.ne + B];
+        rR[B] = P.rdamp[C.ti * P.ne + B];
+    }
+    wave_sync();
+
+    Acc2b<T> A;
+    A.init();
+    int qn = 0;
+    float cnt = 0.0f;
+    const int s0 = C.S->st_off[CUT_2B], s1 = s0 + C.S->st_cnt[CUT_2B];
+    for (int s = s0; s < s1; ++s) {
+        int j0, j1;
+        T ox, oy, oz;
+        C.bin(P, s, j0, j1, ox, oy, oz);
+        for (int jb = j0; jb < j1; jb += WAVE) {
+            const int j = jb + lane;
+            bool in = false;
+            T dx = T(0), dy = T(0), dz = T(0);
+            if (j < j1) {
+                dx = ox - P.lx[j];
+                dy = oy - P.ly[j];
+                dz = oz - P.lz[j];
+                const T r2 = dx * dx + dy * dy + dz * dz;
+                in = r2 < P.rc_22 && r2 > T(0);
+            }
+            const uint64_t m = ballot64(in);
+            if (m == 0) continue;
+            if (in) {
+                const int pos = qn + mbcnt64(m);
+                Q.j[pos] = j;
+                Q.dx[pos] = dx;
+                Q.dy[pos] = dy;
+                Q.dz[pos] = dz;
+            }
+            qn += __popcll(m);
+            if (qn >= WAVE) {
+                wave_sync();
+                pair_2b(P, Q.j[lane], Q.dx[lane], Q.dy[lane], Q.dz[lane], y, dyv, rq, rR, A);
+                cnt += 1.0f;
+                const int rest = qn - WAVE;
+                int mj = 0;
+                T mx = T(0), my = T(0), mz = T(0);
+                if (lane < rest) {
+                    mj = Q.j[WAVE + lane];
+                    mx = Q.dx[WAVE + lane];
+                    my = Q.dy[WAVE + lane];
+                    mz = Q.dz[WAVE + lane];
+                }
+                wave_sync();
+                if (lane < rest) {
+                    Q.j[lane] = mj;
+                    Q.dx[lane] = mx;
+                    Q.dy[lane] = my;
+                    Q.dz[lane] = mz;
+                }
+                wave_sync();
+                qn = rest;
+            }
+        }
+    }
+    wave_sync();
+    if (lane < qn) {
+        pair_2b(P, Q.j[lane], Q.dx[lane], Q.dy[lane], Q.dz[lane], y, dyv, rq, rR, A);
+        cnt += 1.0f;
+    }
+    const double E = wave_total(A.e), G = wave_total(A.g);
+    double F[3], V[6];
+    for (int k = 0; k < 3; ++k) F[k] = wave_total(A.f[k]);
+    for (int k = 0; k < 6; ++k) V[k] = wave_total(A.v[k]);
+    cnt = wave_sumf(cnt);
+    if (lane == 0) {
+        const double iv = C.S->inv_vol;
+        P.e[i] = E;
+        P.g[i] = G;
+        for (int k = 0; k < 3; ++k) P.f[3 * i + k] = F[k];
+        for (int k = 0; k < 6; ++k) P.vir[6 * i + k] = V[k] * iv;
+        P.npair[3 * i + 0] = cnt;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// dE/dCN chain rule, src/d3_kernel.cu:1511-1556 and 1571-1698: one wave per centre over the CN
+// pair set, F_i += (g_i + g_j) c'(r)/r d,  vir += -(g_i + g_j) c'(r)/r d d^T / 2,
+// c'(r)/r = -k1 e (Rcov_i + Rcov_j) / ((1 + e)^2 r^3).
+// ---------------------------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_chain(Params<T> P) {
+    const int lane = lane_id();
+    const int i = blockIdx.x * WPB + (threadIdx.x >> 6);
+    if (i >= P.n) return;
+    Centre<T> C;
+    C.load(P, i);
+    const T rci = P.rcov[C.ti];
+    const T gi = (T)P.g[i];
+    Acc<T> F[3], V[6];
+    for (int k = 0; k < 3; ++k) F[k].init();
+    for (int k = 0; k < 6; ++k) V[k].init();
+    const int s0 = C.S->st_off[CUT_CN], s1 = s0 + C.S->st_cnt[CUT_CN];
+    for (int s = s0; s < s1; ++s) {
+        int j0, j1;
+        T ox, oy, oz;
+        C.bin(P, s, j0, j1, ox, oy, oz);
+        for (int j = j0 + lane; j < j1; j += WAVE) {
+            const T dx = ox - P.lx[j], dy = oy - P.ly[j], dz = oz - P.lz[j];
+            const T r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 <= P.rc_cn2 && r2 > T(0)) {
+                const T ir = xrsqrt(r2);
+                const T rr = rci + P.rcov[P.type_s[j]];
+                const T ex = xexp(-T(K1) * (rr * ir - T(1)));
+                const T ope = T(1) + ex;
+                const T dcn = -T(K1) * ex * rr * ir * ir * ir / (ope * ope);
+                const T sc = finite_or_zero((gi + (T)P.g[j]) * dcn);
+                F[0].add(sc * dx);
+                F[1].add(sc * dy);
+                F[2].add(sc * dz);
+                const T h = T(-0.5) * sc;
+                V[0].add(h * dx * dx);
+                V[1].add(h * dy * dy);
+                V[2].add(h * dz * dz);
+                V[3].add(h * dx * dy);
+                V[4].add(h * dx * dz);
+                V[5].add(h * dy * dz);
+            }
+        }
+    }
+    double Fd[3], Vd[6];
+    for (int k = 0; k < 3; ++k) Fd[k] = wave_total(F[k]);
+    for (int k = 0; k < 6; ++k) Vd[k] = wave_total(V[k]);
+    if (lane == 0) {
+        const double iv = C.S->inv_vol;
+        for (int k = 0; k < 3; ++k) P.f[3 * i + k] += Fd[k];
+        for (int k = 0; k < 6; ++k) P.vir[6 * i + k] += Vd[k] * iv;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Axilrod-Teller-Muto three-body term, src/d3_kernel.cu:1199-1509.  One block per centre i
+// (persistent blocks loop over centres).  Phase 1 collects the centre's neighbour images with
+// 1e-12 <= r^2 <= rc3^2 (block-wide compaction, deterministic order) into per-block scratch,
+// with C6_ij and dlnC6_ij/dCN_i.  Phase 2 visits every unordered neighbour pair {j, k} with
+// 1e-12 <= r_jk^2 <= rc3^2: each thread owns one k (registers), the j side is staged through
+// LDS tiles.  Every periodic triangle is visited once per vertex, so the centre takes e/3 of
+// its energy, the gradient of e with respect to its own position, its own dE/dCN part and half
+// of the virial of its two incident edges (the totals are exact; forces of index-degenerate
+// triangles are physically consistent, unlike the reference's unscaled forces).
+//   e = C9 A f,  C9 = -sqrt(C6_ij C6_ik C6_jk),  A = (1 + 3 cos cos cos) / (r_ij r_ik r_jk)^3
+//   f = 1 / (1 + 6 x^16),  x = (4/3) (R0_ij R0_ik R0_jk)^(1/3) / (r_ij r_ik r_jk)^(1/3)
+// With a = r_ij^2, b = r_ik^2, c = r_jk^2, Q = (a+b-c)(a-b+c)(-a+b+c):
+//   A = (abc)^(-3/2) + (3/8) Q (abc)^(-5/2)
+//   t_ij = (1/r_ij) de/dr_ij = 2 C9 (dA/da f + 16 A x^16 f^2 / a)
+//   F_i += t_ij d_ij + t_ik d_ik,  dE/dCN_i += (C9/2)(dlnC6_ij + dlnC6_ik) A f
+// ---------------------------------------------------------------------------------------------
+constexpr int ATM_TILE = BLOCK;
+
+template <class T>
+struct AtmTile {
+    T dx[ATM_TILE], dy[ATM_TILE], dz[ATM_TILE], r2[ATM_TILE], c6[ATM_TILE], dl[ATM_TILE];
+    T w[MAX_NREF][ATM_TILE];
+    int t[ATM_TILE];
+};
+
+// Sum over the block's waves of a per-wave total (already equal in every lane of a wave);
+// red holds WPB entries of LDS.  Fixed order, so the result is deterministic.
+__device__ __forceinline__ double block_sum(double wave_tot, double *red) {
+    const int wv = threadIdx.x >> 6, lane = lane_id();
+    __syncthreads();
+    if (lane == 0) red[wv] = wave_tot;
+    __syncthreads();
+    double s = 0.0;
+    for (int k = 0; k < WPB; ++k) s += red[k];
+    return s;
+}
+
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_atm(Params<T> P) {
+    __shared__ AtmTile<T> J;
+    __shared__ T yc[2 * 64 * MAX_NREF];  // centre's y_i[B][b] (C6 row) -- ne * nref <= 64 * MAX_NREF
+    __shared__ int wcount[WPB];
+    __shared__ double red[WPB];
+    const int tid = threadIdx.x, wv = tid >> 6, lane = lane_id();
+    const int cap = P.atm_cap;
+    int *lj = P.atm_j + (size_t)blockIdx.x * cap;
+    T *lb = P.atm_buf + (size_t)blockIdx.x * cap * 6;
+    T *Ldx = lb, *Ldy = lb + cap, *Ldz = lb + 2 * cap, *Lr2 = lb + 3 * cap, *Lc6 = lb + 4 * cap, *Ldl = lb + 5 * cap;
+    const int nref = P.nref, ne = P.ne;
+
+    for (int i = blockIdx.x; i < P.n; i += gridDim.x) {
+        Centre<T> C;
+        C.load(P, i);
+        // centre C6 rows y_i[B] = C6ref_AB^T w_i and dy_i[B] = C6ref_AB^T dw_i
+        __syncthreads();
+        for (int idx = tid; idx < ne * nref; idx += BLOCK) {
+            const int B = idx / nref, bb = idx % nref;
+            T sy = T(0), sdy = T(0);
+            for (int a = 0; a < nref; ++a) {
+                const T c = P.c6[((C.ti * ne + B) * nref + a) * nref + bb];
+                sy += c * P.w[a * P.n + i];
+                sdy += c * P.dw[a * P.n + i];
+            }
+            yc[idx] = sy;
+            yc[ne * nref + idx] = sdy;
+        }
+        // phase 1: neighbour list
+        int count = 0;
+        const int s0 = C.S->st_off[CUT_ATM], s1 = s0 + C.S->st_cnt[CUT_ATM];
+        for (int s = s0; s < s1; ++s) {
+            int j0, j1;
+            T ox, oy, oz;
+            C.bin(P, s, j0, j1, ox, oy, oz);
+            for (int jb = j0; jb < j1; jb += BLOCK) {
+                const int j = jb + tid;
+                bool in = false;
+                T dx = T(0), dy = T(0), dz = T(0), r2 = T(0);
+                if (j < j1) {
+                    dx = ox - P.lx[j];
+                    dy = oy - P.ly[j];
+                    dz = oz - P.lz[j];
+                    r2 = dx * dx + dy * dy + dz * dz;
+                    in = r2 <= P.rc_32 && r2 >= T(1e-12);
+                }
+                const uint64_t m = ballot64(in);
+                __syncthreads();
+                if (lane == 0) wcount[wv] = __popcll(m);
+                __syncthreads();
+                int off = count;
+                int tot = 0;
+                for (int k = 0; k < WPB; ++k) {
+                    if (k < wv) off += wcount[k];
+                    tot += wcount[k];
+                }
+                if (in) {
+                    const int pos = off + mbcnt64(m);
+                    if (pos < cap) {
+                        lj[pos] = j;
+                        Ldx[pos] = dx;
+                        Ldy[pos] = dy;
+                        Ldz[pos] = dz;
+                        Lr2[pos] = r2;
+                    }
+                }
+                count += tot;
+            }
+        }
+        __syncthreads();
+        if (count > cap) {
+            if (tid == 0) atomicOr(P.status, 2u);
+            continue;  // uniform: the centre is skipped and flagged
+        }
+        const int n = count;
+        // C6_ij and dln C6_ij / dCN_i per neighbour
+        for (int p = tid; p < n; p += BLOCK) {
+            const int j = lj[p], tj = P.type_s[j];
+            T c6 = T(0), dc6 = T(0);
+            for (int b = 0; b < nref; ++b) {
+                const T wj = P.w[b * P.n + j];
+                c6 += yc[tj * nref + b] * wj;
+                dc6 += yc[ne * nref + tj * nref + b] * wj;
+            }
+            Lc6[p] = c6;
+            Ldl[p] = c6 != T(0) ? dc6 / c6 : T(0);
+        }
+        __syncthreads();
+
+        // phase 2: triangles
+        Acc<T> aE, aG, aF[3], aV[6];
+        aE.init();
+        aG.init();
+        for (int k = 0; k < 3; ++k) aF[k].init();
+        for (int k = 0; k < 6; ++k) aV[k].init();
+        float cnt = 0.0f;
+        const T rc32 = P.rc_32;
+        for (int kb = 0; kb * ATM_TILE < n; ++kb) {
+            const int k = kb * ATM_TILE + tid;
+            const bool kval = k < n;
+            T kx = T(0), ky = T(0), kz = T(0), kr2 = T(1), kc6 = T(0), kdl = T(0);
+            T kw[MAX_NREF];
+            int tk = 0;
+            for (int a = 0; a < MAX_NREF; ++a) kw[a] = T(0);
+            if (kval) {
+                const int jk = lj[k];
+                kx = Ldx[k];
+                ky = Ldy[k];
+                kz = Ldz[k];
+                kr2 = Lr2[k];
+                kc6 = Lc6[k];
+                kdl = Ldl[k];
+                tk = P.type_s[jk];
+                for (int a = 0; a < nref; ++a) kw[a] = P.w[a * P.n + jk];
+            }
+            const T r0ik = P.r0[C.ti * ne + tk];
+            for (int jt = 0; jt <= kb; ++jt) {
+                __syncthreads();
+                const int p = jt * ATM_TILE + tid;
+                if (p < n) {
+                    const int jj = lj[p];
+                    J.dx[tid] = Ldx[p];
+                    J.dy[tid] = Ldy[p];
+                    J.dz[tid] = Ldz[p];
+                    J.r2[tid] = Lr2[p];
+                    J.c6[tid] = Lc6[p];
+                    J.dl[tid] = Ldl[p];
+                    J.t[tid] = P.type_s[jj];
+                    for (int a = 0; a < nref; ++a) J.w[a][tid] = P.w[a * P.n + jj];
+                }
+                __syncthreads();
+                if (!kval) continue;
+                int jend = (jt == kb) ? tid : ATM_TILE;
+                const int rem = n - jt * ATM_TILE;
+                if (jend > rem) jend = rem;
+                for (int q = 0; q < jend; ++q) {
+                    const T jx = J.dx[q], jy = J.dy[q], jz = J.dz[q];
+                    const T ex = kx - jx, ey = ky - jy, ez = kz - jz;
+                    const T c = ex * ex + ey * ey + ez * ez;
+                    if (c > rc32 || c < T(1e-12)) continue;
+                    const T a = J.r2[q], b = kr2;
+                    const int tj = J.t[q];
+                    // C6_jk = w_j^T C6ref_{tj tk} w_k
+                    T c6jk = T(0);
+                    const T *ct = P.c6 + (size_t)(tj * ne + tk) * nref * nref;
+                    for (int aa = 0; aa < nref; ++aa) {
+                        T u = T(0);
+                        for (int bb = 0; bb < nref; ++bb) u += ct[aa * nref + bb] * kw[bb];
+                        c6jk += J.w[aa][q] * u;
+                    }
+                    const T C9 = -xsqrt(J.c6[q] * kc6 * c6jk);
+                    const T abc = a * b * c;
+                    const T iR = xrsqrt(abc);
+                    const T iR3 = iR * iR * iR;
+                    const T iabc = T(1) / abc;
+                    const T u1 = a + b - c, u2 = a - b + c, u3 = -a + b + c;
+                    const T Qv = u1 * u2 * u3;
+                    const T iR5 = iR3 * iabc;
+                    const T Aang = iR3 + T(0.375) * Qv * iR5;
+                    const T dQa = u2 * u3 + u1 * u3 - u1 * u2;
+                    const T dQb = u2 * u3 - u1 * u3 + u1 * u2;
+                    const T ia = T(1) / a, ib = T(1) / b;
+                    const T dAa = T(-1.5) * iR3 * ia + T(0.375) * (dQa * iR5 - T(2.5) * Qv * iR5 * ia);
+                    const T dAb = T(-1.5) * iR3 * ib + T(0.375) * (dQb * iR5 - T(2.5) * Qv * iR5 * ib);
+                    const T r0p = P.r0[C.ti * ne + tj] * r0ik * P.r0[tj * ne + tk];
+                    const T x2 = T(16.0 / 9.0) * xcbrt(r0p * r0p * iabc);
+                    const T x4 = x2 * x2, x8 = x4 * x4, x16 = x8 * x8;
+                    const T fd = T(1) / (T(1) + T(6) * x16);
+                    const T e = C9 * Aang * fd;
+                    const T dfa = T(16) * x16 * fd * fd;
+                    const T tij = T(2) * C9 * (dAa * fd + Aang * dfa * ia);
+                    const T tik = T(2) * C9 * (dAb * fd + Aang * dfa * ib);
+                    aE.add(e * T(1.0 / 3.0));
+                    aG.add(T(0.5) * C9 * Aang * fd * (J.dl[q] + kdl));
+                    aF[0].add(tij * jx + tik * kx);
+                    aF[1].add(tij * jy + tik * ky);
+                    aF[2].add(tij * jz + tik * kz);
+                    const T hj = T(-0.5) * tij, hk = T(-0.5) * tik;
+                    aV[0].add(hj * jx * jx + hk * kx * kx);
+                    aV[1].add(hj * jy * jy + hk * ky * ky);
+                    aV[2].add(hj * jz * jz + hk * kz * kz);
+                    aV[3].add(hj * jx * jy + hk * kx * ky);
+                    aV[4].add(hj * jx * jz + hk * kx * kz);
+                    aV[5].add(hj * jy * jz + hk * ky * kz);
+                    cnt += 1.0f;
+                }
+            }
+        }
+        const double E = block_sum(wave_total(aE), red), G = block_sum(wave_total(aG), red);
+        double F[3], V[6];
+        for (int k = 0; k < 3; ++k) F[k] = block_sum(wave_total(aF[k]), red);
+        for (int k = 0; k < 6; ++k) V[k] = block_sum(wave_total(aV[k]), red);
+        const double cn3 = block_sum((double)wave_sumf(cnt), red);
+        if (tid == 0) {
+            const double iv = C.S->inv_vol;
+            P.e[i] += E;
+            P.g[i] += G;
+            for (int k = 0; k < 3; ++k) P.f[3 * i + k] += F[k];
+            for (int k = 0; k < 6; ++k) P.vir[6 * i + k] += V[k] * iv;
+            P.npair[3 * i + 2] = (float)cn3;
+        }
+    }
+}
+
+}  // namespace
+
+template <class T>
+void Launch<T>::bin_sort(const Params<T> &P, int nbins, int *bin_start_mut, hipStream_t s) {
+    const int tb = 256;
+    hipLaunchKernelGGL(k_fill_i32, dim3((nbins + tb - 1) / tb), dim3(tb), 0, s, P.bin_count, nbins, 0);
+    if (P.n > 0) hipLaunchKernelGGL(k_bin<T>, dim3((P.n + tb - 1) / tb), dim3(tb), 0, s, P);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, P.bin_count, bin_start_mut, P.cursor, nbins);
+    if (P.n > 0) {
+        hipLaunchKernelGGL(k_scatter<T>, dim3((P.n + tb - 1) / tb), dim3(tb), 0, s, P);
+        hipLaunchKernelGGL(k_canon, dim3(nbins), dim3(tb), 0, s, P.bin_start, P.perm, P.perm2);
+        hipLaunchKernelGGL(k_gather<T>, dim3((P.n + tb - 1) / tb), dim3(tb), 0, s, P);
+    }
+}
+
+template <class T>
+void Launch<T>::cn(const Params<T> &P, hipStream_t s) {
+    if (P.n == 0) return;
+    hipLaunchKernelGGL(k_cn<T>, dim3((P.n + WPB - 1) / WPB), dim3(BLOCK), 0, s, P);
+}
+
+template <class T>
+void Launch<T>::twobody(const Params<T> &P, hipStream_t s) {
+    if (P.n == 0) return;
+    const size_t lds = WPB * twobody_wave_lds<T>(P.ne, P.nref);
+    hipLaunchKernelGGL(k_2b<T>, dim3((P.n + WPB - 1) / WPB), dim3(BLOCK), lds, s, P);
+}
+
+template <class T>
+void Launch<T>::chain(const Params<T> &P, hipStream_t s) {
+    if (P.n == 0) return;
+    hipLaunchKernelGGL(k_chain<T>, dim3((P.n + WPB - 1) / WPB), dim3(BLOCK), 0, s, P);
+}
+
+template <class T>
+void Launch<T>::atm(const Params<T> &P, int nblocks, hipStream_t s) {
+    if (P.n == 0) return;
+    hipLaunchKernelGGL(k_atm<T>, dim3(nblocks), dim3(BLOCK), 0, s, P);
+}
+
+template struct Launch<double>;
+template struct Launch<float>;
+
+}  // namespace d3

@@ -1,0 +1,130 @@
+"""GPU parity: the HIP path through the C ABI against the FP64 oracle on the same inputs.
+
+Bars: plain FP64 path within 1e-12 relative energy and 1e-9
+eV/Angstrom forces of the oracle; stress within 1e-12 relative to its largest component (plus
+1e-12 eV/A^3 absolute); CN and dE/dCN to 1e-10 relative.  The FP32 path is held to 2e-6 relative
+energy and 2e-5 of the largest force component; its measured error is printed per system.
+"""
+
+import numpy as np
+import pytest
+
+import d3_lib
+import d3_oracle
+import systems
+
+pytestmark = pytest.mark.gpu
+
+SYSTEMS = systems.parity_systems()
+
+
+def _oracle(s, table):
+    return d3_oracle.compute(s["pos"], s["Z"], s["cell"], table, cutoff=s["cutoff"], cn_cutoff=s["cn_cutoff"],
+                             atm_cutoff=s.get("atm", 0.0), damping=s["damping"], functional=s["functional"])
+
+
+def _gpu(s, precision):
+    h = d3_lib.Handle(s["Z"], len(s["Z"]), cutoff=s["cutoff"], cn_cutoff=s["cn_cutoff"],
+                      atm_cutoff=s.get("atm", 0.0), damping=s["damping"], functional=s["functional"],
+                      precision=precision)
+    h.set_atoms(s["pos"], s["Z"])
+    h.set_cell(s["cell"])
+    e, f, st = h.compute()
+    cn, g, ea = h.atom_data()
+    h.close()
+    return dict(energy=e, forces=f, stress=st, cn=cn, dedcn=g, atom_energy=ea)
+
+
+_cache = {}
+
+
+def oracle_result(s, table):
+    if s["name"] not in _cache:
+        _cache[s["name"]] = _oracle(s, table)
+    return _cache[s["name"]]
+
+
+@pytest.mark.parametrize("s", SYSTEMS, ids=[s["name"] for s in SYSTEMS])
+def test_fp64_matches_oracle(s, table):
+    print(f"case {s['name']} n={len(s['Z'])} fp64", flush=True)
+    o = oracle_result(s, table)
+    g = _gpu(s, d3_lib.D3X_FP64)
+    assert abs(g["energy"] - o["energy"]) <= 1e-12 * abs(o["energy"]), (g["energy"], o["energy"])
+    assert np.max(np.abs(g["forces"] - o["forces"])) <= 1e-9
+    smax = np.max(np.abs(o["stress"]))
+    assert np.max(np.abs(g["stress"] - o["stress"])) <= 1e-12 * smax + 1e-12
+    # per-atom values: relative to each array's largest magnitude, with a floor far below any
+    # physical size (values near zero carry the rounding noise of the oracle's Z/W
+    # cancellation; dE/dCN vanishes when one reference dominates the weights)
+    for k, floor in (("cn", 1e-14), ("dedcn", 1e-20), ("atom_energy", 1e-15)):
+        scale = np.max(np.abs(o[k]))
+        assert np.max(np.abs(g[k] - o[k])) <= 1e-11 * scale + floor, k
+
+
+@pytest.mark.parametrize("s", SYSTEMS, ids=[s["name"] for s in SYSTEMS])
+def test_fp32_error_vs_oracle(s, table):
+    print(f"case {s['name']} n={len(s['Z'])} fp32", flush=True)
+    o = oracle_result(s, table)
+    g = _gpu(s, d3_lib.D3X_FP32)
+    rel_e = abs(g["energy"] - o["energy"]) / abs(o["energy"])
+    fmax = np.max(np.abs(o["forces"]))
+    df = np.max(np.abs(g["forces"] - o["forces"]))
+    smax = np.max(np.abs(o["stress"]))
+    ds = np.max(np.abs(g["stress"] - o["stress"]))
+    print(f"  fp32 rel dE {rel_e:.2e}  max dF {df:.2e} eV/A (|F|max {fmax:.2e})  max dS {ds:.2e} (|S|max {smax:.2e})")
+    assert rel_e <= 2e-6
+    assert df <= 2e-5 * fmax + 1e-7
+    assert ds <= 2e-5 * smax + 1e-9
+
+
+def test_batch_equals_single(table):
+    """One batched evaluation of several cells equals the cells evaluated one by one."""
+    cases = [systems.si64(seed=11), systems.triclinic_mixed(seed=12), systems.water_small(seed=13)]
+    # common handle parameters: union of elements, one cutoff set
+    els = np.unique(np.concatenate([c["Z"] for c in cases]))
+    kw = dict(cutoff=60.0, cn_cutoff=30.0, atm_cutoff=25.0, damping=1, functional=0)
+    total = sum(len(c["Z"]) for c in cases)
+    hb = d3_lib.Handle(els, total, **kw)
+    eb, fb, sb = hb.compute_batch([len(c["Z"]) for c in cases], np.concatenate([c["pos"] for c in cases]),
+                                  np.concatenate([c["Z"] for c in cases]), np.stack([c["cell"] for c in cases]))
+    off = 0
+    for k, c in enumerate(cases):
+        hs = d3_lib.Handle(els, len(c["Z"]), **kw)
+        hs.set_atoms(c["pos"], c["Z"])
+        hs.set_cell(c["cell"])
+        e, f, st = hs.compute()
+        n = len(c["Z"])
+        assert abs(e - eb[k]) <= 1e-13 * abs(e)
+        assert np.max(np.abs(f - fb[off:off + n])) <= 1e-12
+        assert np.max(np.abs(st - sb[k])) <= 1e-13 * np.max(np.abs(st)) + 1e-15
+        off += n
+
+
+def test_drop_in_float_api(table):
+    """The reference-compatible float API (d3_cffi.D3Calculator) returns the FP64 results rounded to float."""
+    import d3_cffi
+    s = systems.SMALL
+    calc = d3_cffi.D3Calculator(list(s["Z"]), len(s["Z"]), cutoff_radius=40.0, cn_cutoff_radius=40.0,
+                                damping_type=0, functional_type=1)
+    calc.set_atoms(s["pos"].astype(np.float32), list(s["Z"]))
+    calc.set_cell(s["cell"].astype(np.float32))
+    e, f, st = calc.compute()
+    calc.clear()
+    o = oracle_result(s, table)
+    # float32 rounding of Angstrom inputs (the test coordinates are float32 values already)
+    assert abs(e - o["energy"]) <= 1e-6 * abs(o["energy"])
+    assert np.max(np.abs(f - o["forces"])) <= 1e-6 * np.max(np.abs(o["forces"])) + 1e-9
+    assert f.dtype == np.float32 and f.shape == (10, 3) and st.shape == (3, 3)
+
+
+def test_repeat_is_deterministic(table):
+    """ResultStable (test/d3_test.cpp:89-129): repeated evaluations agree bit for bit."""
+    s = systems.si64(seed=21)
+    h = d3_lib.Handle(s["Z"], len(s["Z"]), cutoff=s["cutoff"], cn_cutoff=s["cn_cutoff"], atm_cutoff=s["atm"],
+                      damping=1, functional=0)
+    h.set_atoms(s["pos"], s["Z"])
+    h.set_cell(s["cell"])
+    e0, f0, s0 = h.compute()
+    for _ in range(8):
+        e, f, st = h.compute()
+        assert e == e0 and np.array_equal(f, f0) and np.array_equal(st, s0)
