@@ -1,0 +1,245 @@
+#!/usr/bin/env python3
+"""Throughput benchmark of the D3 engine on one MI355X.
+
+A step is one full evaluation -- energy, forces and 3x3 stress with two-body, CN, ATM
+three-body and the dE/dCN chain rule -- of one synthetic periodic structure whose coordinates
+are already resident in HBM (d3x_compute_device on a handle set up beforehand).  The headline
+value is the FP32 (compensated) path in atoms/s; the plain FP64 path is timed on the same input
+for the "vs plain FP64" comparison, and the FP32 error against it is reported.
+
+Workloads (seeded generators in lasp-d3_amd/src/d3_structures.py; PBE-D3(BJ), two-body cutoff
+sqrt(9000) bohr, CN and ATM cutoff 40 bohr):
+  water  4096 rigid H2O, 12,288 atoms, 49.7 A cubic cell (default)
+  si512  diamond Si 4x4x4, 512 atoms
+  cu1m   fcc Cu 63^3, 1,000,188 atoms
+  batch  256 triclinic cells of 50-5000 atoms (~300k atoms), one batched evaluation
+
+Prints ONE JSON line.  roofline: the dominant kernel's algorithmic FP32 work per launch
+(work model: 17 flops per unique CN pair, 193 per unique two-body pair, 330 per unique
+triangle, 45 per unique CN pair for the chain rule) over its HIP-event-timed average launch
+duration, against the 157.3 TFLOP/s FP32 vector peak.  cpu_baseline: the FP64 OpenMP oracle
+timed on a bounded random sample of centre atoms on this host (scaled per atom).
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _sub in ("lasp-d3_amd/src", "oracle"):
+    sys.path.insert(0, os.path.join(ROOT, _sub))
+
+import numpy as np  # noqa: E402
+
+import d3_lib  # noqa: E402
+import d3_structures as S  # noqa: E402
+
+RC2 = float(np.sqrt(9000.0))
+RCCN = 40.0
+RC3 = 40.0
+PEAK_FP32 = 157.3  # TFLOP/s, MI355X vector FP32
+FLOPS = dict(cn=17.0, two_body=193.0, atm=330.0, chain=45.0)
+METRIC = "atoms/s E+F+stress (2b+3b) vs plain FP64 GPU; fraction of FP32 VALU peak"
+
+
+class HipEvents:
+    """hipEvent timing on an arbitrary stream through the HIP runtime's C API."""
+
+    def __init__(self):
+        self.hip = C.CDLL("libamdhip64.so")
+
+    def create(self):
+        ev = C.c_void_p()
+        assert self.hip.hipEventCreate(C.byref(ev)) == 0
+        return ev
+
+    def record(self, ev, stream):
+        assert self.hip.hipEventRecord(ev, C.c_void_p(stream)) == 0
+
+    def elapsed_ms(self, a, b):
+        assert self.hip.hipEventSynchronize(b) == 0
+        ms = C.c_float()
+        assert self.hip.hipEventElapsedTime(C.byref(ms), a, b) == 0
+        return float(ms.value)
+
+
+def workload(name: str):
+    if name == "water":
+        pos, Z, cell = S.water_box(4096, seed=0)
+        return dict(kind="single", pos=pos, Z=Z, cell=cell, desc="water-4096H2O")
+    if name == "si512":
+        pos, Z, cell = S.si_diamond(4, jitter=0.02, seed=0)
+        return dict(kind="single", pos=pos, Z=Z, cell=cell, desc="si-diamond-512")
+    if name == "cu1m":
+        pos, Z, cell = S.fcc_cu(63, jitter=0.05, seed=0)
+        return dict(kind="single", pos=pos, Z=Z, cell=cell, desc="fcc-cu-1000188")
+    if name == "batch":
+        counts, pos, Z, cells = S.triclinic_batch(256, seed=0)
+        return dict(kind="batch", counts=counts, pos=pos, Z=Z, cells=cells, desc="triclinic-batch-256")
+    raise SystemExit(f"unknown workload {name}")
+
+
+def make_handle(w, precision):
+    n = len(w["Z"])
+    h = d3_lib.Handle(np.unique(w["Z"]), n, cutoff=RC2, cn_cutoff=RCCN, atm_cutoff=RC3, damping=1, functional=0,
+                      precision=precision)
+    if w["kind"] == "single":
+        h.set_atoms(w["pos"], w["Z"])
+        h.set_cell(w["cell"])
+        h.compute()  # sizes scratch (retries a neighbour-list overflow) and leaves inputs resident
+    else:
+        h.compute_batch(w["counts"], w["pos"], w["Z"], w["cells"])
+    return h
+
+
+def fetch(h, w):
+    if w["kind"] == "single":
+        return h.fetch()
+    e = np.zeros(len(w["counts"]))
+    f = np.zeros((len(w["Z"]), 3))
+    s = np.zeros((len(w["counts"]), 9))
+    rc = h.lib.d3x_fetch_results(h.h, d3_lib.ptr(e, C.c_double), d3_lib.ptr(f, C.c_double),
+                                 d3_lib.ptr(s, C.c_double))
+    if rc:
+        raise RuntimeError(d3_lib.last_error())
+    return e, f, s
+
+
+def time_steps(h, ev, steps, warmup):
+    for _ in range(warmup):
+        h.compute_device()
+    st = h.stream()
+    a, b = ev.create(), ev.create()
+    ev.hip.hipStreamSynchronize(C.c_void_p(st))
+    t0 = time.perf_counter()
+    ev.record(a, st)
+    for _ in range(steps):
+        h.compute_device()
+    ev.record(b, st)
+    ms = ev.elapsed_ms(a, b)
+    wall = time.perf_counter() - t0
+    return ms / steps, wall
+
+
+def phase_profile(h, runs=2):
+    h.set_timing(True)
+    for _ in range(runs):
+        h.compute_device()
+    ph, n = h.phase_times()
+    h.set_timing(False)
+    return {k: v / max(n, 1) for k, v in ph.items()}
+
+
+def cpu_baseline(w, table, cn, g, target_s=15.0):
+    """FP64 oracle on a bounded random sample of centre atoms, all host threads of this job."""
+    import d3_oracle
+    if w["kind"] != "single":
+        return None
+    n = len(w["Z"])
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    rng = np.random.Generator(np.random.PCG64(1234))
+    kw = dict(cutoff=RC2, cn_cutoff=RCCN, atm_cutoff=RC3, damping=1, functional=0, nthreads=threads)
+    probe = rng.choice(n, size=min(n, threads), replace=False)
+    t1, _ = d3_oracle.time_centers(w["pos"], w["Z"], w["cell"], table, probe, cn, g, **kw)
+    reps = max(1, int(target_s / max(t1, 1e-6)))
+    m = int(min(n, threads * reps))
+    sample = rng.choice(n, size=m, replace=False)
+    t, _ = d3_oracle.time_centers(w["pos"], w["Z"], w["cell"], table, sample, cn, g, **kw)
+    return dict(value=m / t, unit="atoms/s", cores=threads, kind="port",
+                sample=f"{m} random centre atoms of {n} (all four passes per centre, FP64 oracle, "
+                       f"{t:.1f} s); throughput scales per atom because every atom's work is one centre's")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="water", choices=["water", "si512", "cu1m", "batch"])
+    ap.add_argument("--fp64-steps", type=int, default=None)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    w = workload(args.workload)
+    n = len(w["Z"])
+    ev = HipEvents()
+    print(f"[bench] workload {w['desc']} atoms={n}", file=sys.stderr, flush=True)
+
+    h32 = make_handle(w, d3_lib.D3X_FP32)
+    ms32, wall32 = time_steps(h32, ev, args.steps, args.warmup)
+    e32, f32, s32 = fetch(h32, w)
+    print(f"[bench] fp32 {ms32:.2f} ms/step", file=sys.stderr, flush=True)
+    prof32 = phase_profile(h32)
+    counts = h32.counts(len(w["counts"]) if w["kind"] == "batch" else 1).sum(axis=0)
+
+    h64 = make_handle(w, d3_lib.D3X_FP64)
+    k64 = args.fp64_steps or args.steps
+    ms64, _ = time_steps(h64, ev, k64, 1)
+    e64, f64, s64 = fetch(h64, w)
+    print(f"[bench] fp64 {ms64:.2f} ms/step", file=sys.stderr, flush=True)
+    prof64 = phase_profile(h64, runs=1)
+    if w["kind"] == "single":
+        cn64, g64, _ = h64.atom_data()
+
+    e32a, e64a = np.atleast_1d(e32), np.atleast_1d(e64)
+    err = dict(energy_rel=float(np.max(np.abs(e32a - e64a) / np.abs(e64a))),
+               force_max_abs=float(np.max(np.abs(f32 - f64))),
+               force_rel_to_max=float(np.max(np.abs(f32 - f64)) / max(np.max(np.abs(f64)), 1e-300)),
+               stress_max_abs=float(np.max(np.abs(np.asarray(s32) - np.asarray(s64)))))
+
+    # work model: unique pairs / triangles from the GPU's own counts
+    uniq = dict(two_body=counts[0] / 2, cn=counts[1] / 2, chain=counts[1] / 2, atm=counts[2] / 3)
+    flops = {k: FLOPS[k] * uniq[k] for k in FLOPS}
+    dom = max(("cn", "two_body", "atm", "chain"), key=lambda k: prof32[k])
+    achieved = flops[dom] / (prof32[dom] * 1e-3) / 1e12
+    total_flops = sum(flops.values())
+
+    cpu = None
+    if not args.no_cpu and w["kind"] == "single":
+        import d3_oracle  # noqa: F401
+        cpu = cpu_baseline(w, d3_lib.table(), cn64, g64)
+
+    out = {
+        "metric": METRIC,
+        "value": n / (ms32 * 1e-3),
+        "unit": "atoms/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms32,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": w["desc"], "atoms": n, "functional": "PBE-D3(BJ)", "rc_2body_bohr": RC2,
+                   "rc_cn_bohr": RCCN, "rc_atm_bohr": RC3, "terms": "2b+CN+ATM+chain", "precision": "fp32",
+                   "table": d3_lib.table()["name"]},
+        "fp64_plain": {"atoms_per_s": n / (ms64 * 1e-3), "ms_per_step": ms64, "steps": k64},
+        "speedup_fp32_vs_fp64": ms64 / ms32,
+        "fp32_error_vs_fp64": err,
+        "phase_ms_fp32": prof32,
+        "phase_ms_fp64": prof64,
+        "work": {"unique_2b_pairs": uniq["two_body"], "unique_cn_pairs": uniq["cn"],
+                 "unique_triangles": uniq["atm"], "flops_per_eval": total_flops,
+                 "tflops_whole_step": total_flops / (ms32 * 1e-3) / 1e12},
+        "roofline": {"bound": "valu", "kernel": {"atm": "k_atm<float>", "two_body": "k_2b<float>",
+                                                  "cn": "k_cn<float>", "chain": "k_chain<float>"}[dom],
+                     "achieved": achieved, "peak": PEAK_FP32, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32,
+                     "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -56,6 +56,18 @@ int d3x_get_atom_data(d3_handle_t *handle, double *cn, double *dedcn, double *at
 int d3x_compute_device(d3_handle_t *handle);
 int d3x_fetch_results(d3_handle_t *handle, double *energy, double *forces, double *stress);
 
+/* Work counts of the last evaluation, 3 per cell: two-body pairs within the cutoff (each
+ * unique pair counted twice, once per atom), CN pairs (same convention), and three-body
+ * centre-pairs (each unique triangle counted three times, once per vertex). */
+int d3x_get_counts(d3_handle_t *handle, double *counts);
+
+/* Per-phase GPU timing with HIP events on the handle's stream.  d3x_set_timing(h, 1) enables
+ * and resets; d3x_phase_times fills ms[6] with the summed milliseconds of the phases bin sort,
+ * CN, two-body, three-body, chain rule, reduction over the timed evaluations and returns their
+ * number (negative on error).  Timing synchronises once per evaluation. */
+int d3x_set_timing(d3_handle_t *handle, int enable);
+int d3x_phase_times(d3_handle_t *handle, double ms[6]);
+
 /* The HIP stream (hipStream_t) the handle launches on; d3x_set_stream replaces it (NULL =
  * the handle's own stream). */
 void *d3x_get_stream(d3_handle_t *handle);

@@ -238,6 +238,23 @@ int d3x_fetch_results(d3_handle_t *handle, double *energy, double *forces, doubl
     return (int)guarded(H(handle), "d3x_fetch_results", [&] { H(handle)->eng->fetch(energy, forces, stress); });
 }
 
+int d3x_get_counts(d3_handle_t *handle, double *counts) {
+    if (!handle || !counts) return (int)(D3X_ERR_GENERIC | D3X_ERR_INPUT);
+    return (int)guarded(H(handle), "d3x_get_counts", [&] { H(handle)->eng->fetch_counts(counts); });
+}
+
+int d3x_set_timing(d3_handle_t *handle, int enable) {
+    if (!handle) return (int)(D3X_ERR_GENERIC | D3X_ERR_INPUT);
+    return (int)guarded(H(handle), "d3x_set_timing", [&] { H(handle)->eng->set_timing(enable != 0); });
+}
+
+int d3x_phase_times(d3_handle_t *handle, double ms[6]) {
+    if (!handle || !ms) return -1;
+    int n = -1;
+    uint32_t st = guarded(H(handle), "d3x_phase_times", [&] { n = H(handle)->eng->phase_times(ms); });
+    return st ? -1 : n;
+}
+
 void *d3x_get_stream(d3_handle_t *handle) { return handle ? (void *)H(handle)->eng->stream() : nullptr; }
 
 int d3x_set_stream(d3_handle_t *handle, void *stream) {

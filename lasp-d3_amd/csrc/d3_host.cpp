@@ -204,6 +204,8 @@ Engine::~Engine() {
                     &d_sys, &d_st_d, &d_st_D, &d_chunk_b, &d_chunk_e, &d_chunk_s, &d_sys_chunk, &d_partial,
                     &d_out_sys, &d_tab, &d_status, &d_atm_j, &d_atm_buf};
     for (void **p : all) dfree(*p);
+    for (auto &e : ev_)
+        if (e) (void)hipEventDestroy(e);
     if (own_stream_) (void)hipStreamDestroy(own_stream_);
 }
 
@@ -582,17 +584,57 @@ void Engine::run() {
     }
     ensure_atm_scratch();
     Params<T> P = params<T>();
+    if (timing_) harvest();
     launch_fill_u32((unsigned *)d_status, 1, 0u, stream_);
+    mark(0);
     Launch<T>::bin_sort(P, nbins_, (int *)d_bin_start, stream_);
+    mark(1);
     Launch<T>::cn(P, stream_);
+    mark(2);
     Launch<T>::twobody(P, stream_);
+    mark(3);
     if (rc3_ > 0) Launch<T>::atm(P, atm_blocks_, stream_);
+    mark(4);
     Launch<T>::chain(P, stream_);
+    mark(5);
     launch_reduce((const double *)d_e, (const double *)d_vir, (const float *)d_npair, (const int *)d_chunk_b,
                   (const int *)d_chunk_e, (const int *)d_chunk_s, (int)chunk_b_.size(), (const int *)d_sys_chunk,
                   (int)sysdev_.size(), (double *)d_partial, (double *)d_out_sys, stream_);
     launch_finalize(P, (double *)d_out_f, (double *)d_out_e, (double *)d_out_cn, (double *)d_out_g, stream_);
+    mark(6);
+    if (timing_) timing_pending_ = true;
     HIPCHK(hipGetLastError());
+}
+
+void Engine::mark(int k) {
+    if (timing_) HIPCHK(hipEventRecord(ev_[k], stream_));
+}
+
+void Engine::harvest() {
+    if (!timing_pending_) return;
+    HIPCHK(hipEventSynchronize(ev_[NPHASE]));
+    for (int k = 0; k < NPHASE; ++k) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, ev_[k], ev_[k + 1]));
+        phase_ms_[k] += ms;
+    }
+    ++timed_runs_;
+    timing_pending_ = false;
+}
+
+void Engine::set_timing(bool on) {
+    if (on && !ev_[0])
+        for (auto &e : ev_) HIPCHK(hipEventCreate(&e));
+    timing_ = on;
+    timing_pending_ = false;
+    timed_runs_ = 0;
+    for (double &m : phase_ms_) m = 0;
+}
+
+int Engine::phase_times(double ms[NPHASE]) {
+    if (timing_) harvest();
+    for (int k = 0; k < NPHASE; ++k) ms[k] = phase_ms_[k];
+    return timed_runs_;
 }
 
 void Engine::compute_device() {

@@ -59,6 +59,13 @@ class Engine {
     const std::vector<SysDev> &sysdev() const { return sysdev_; }
     int stencil_count(int cut) const;
 
+    // Per-phase timing with HIP events on the handle's stream (bin sort, CN, two-body, ATM,
+    // chain rule, reduction).  enable resets the totals; phase_times harvests pending events and
+    // returns the summed milliseconds per phase and the number of timed evaluations.
+    static constexpr int NPHASE = 6;
+    void set_timing(bool on);
+    int phase_times(double ms[NPHASE]);
+
    private:
     struct Cell {
         double H[9] = {0};  // bohr
@@ -120,6 +127,13 @@ class Engine {
     void *d_atm_j = nullptr, *d_atm_buf = nullptr;
     int atm_blocks_ = 0, atm_cap_ = 0;
     size_t atm_alloc_entries_ = 0;  // allocated scratch entries (>= atm_blocks_ * atm_cap_)
+
+    bool timing_ = false, timing_pending_ = false;
+    hipEvent_t ev_[NPHASE + 1] = {};
+    double phase_ms_[NPHASE] = {0};
+    int timed_runs_ = 0;
+    void mark(int k);
+    void harvest();
 };
 
 void launch_fill_u32(unsigned *p, int n, unsigned v, hipStream_t s);

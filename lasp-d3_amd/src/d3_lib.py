@@ -60,6 +60,9 @@ SIGNATURES = {
     "d3x_get_atom_data": (C.c_int, [_vp, _dp, _dp, _dp]),
     "d3x_compute_device": (C.c_int, [_vp]),
     "d3x_fetch_results": (C.c_int, [_vp, _dp, _dp, _dp]),
+    "d3x_get_counts": (C.c_int, [_vp, _dp]),
+    "d3x_set_timing": (C.c_int, [_vp, C.c_int]),
+    "d3x_phase_times": (C.c_int, [_vp, _dp]),
     "d3x_get_stream": (_vp, [_vp]),
     "d3x_set_stream": (C.c_int, [_vp, _vp]),
     "d3x_compute_batch_f64": (C.c_int, [_vp, _u64, _u64p, _dp, _u16p, _dp, _dp, _dp, _dp]),
@@ -201,6 +204,23 @@ class Handle:
                                                    ptr(f, C.c_double), ptr(s, C.c_double)), "d3x_compute_batch_f64")
         self.n = len(z)
         return e, f, s.reshape(ns, 3, 3)
+
+    def counts(self, nsys: int = 1) -> np.ndarray:
+        c = np.zeros((nsys, 3))
+        self._check(self.lib.d3x_get_counts(self.h, ptr(c, C.c_double)), "d3x_get_counts")
+        return c
+
+    def set_timing(self, on: bool):
+        self._check(self.lib.d3x_set_timing(self.h, int(on)), "d3x_set_timing")
+
+    PHASES = ("bin_sort", "cn", "two_body", "atm", "chain", "reduce")
+
+    def phase_times(self):
+        ms = np.zeros(6)
+        n = self.lib.d3x_phase_times(self.h, ptr(ms, C.c_double))
+        if n < 0:
+            raise D3Error("d3x_phase_times", -1)
+        return dict(zip(self.PHASES, ms)), n
 
     def stream(self) -> int:
         return self.lib.d3x_get_stream(self.h) or 0

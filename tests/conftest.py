@@ -4,7 +4,7 @@ import sys
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-for sub in ("lasp-d3_amd/src", "oracle", "tests"):
+for sub in ("lasp-d3_amd/src", "lasp-d3_amd/tables", "oracle", "tests"):
     p = os.path.join(ROOT, sub)
     if p not in sys.path:
         sys.path.insert(0, p)
