@@ -197,7 +197,7 @@ def main():
                stress_max_abs=float(np.max(np.abs(np.asarray(s32) - np.asarray(s64)))))
 
     # work model: unique pairs / triangles from the GPU's own counts
-    uniq = dict(two_body=counts[0] / 2, cn=counts[1] / 2, chain=counts[1] / 2, atm=counts[2] / 3)
+    uniq = dict(two_body=counts[0] / 2, cn=counts[1] / 2, chain=counts[1] / 2, atm=counts[2])
     flops = {k: FLOPS[k] * uniq[k] for k in FLOPS}
     dom = max(("cn", "two_body", "atm", "chain"), key=lambda k: prof32[k])
     achieved = flops[dom] / (prof32[dom] * 1e-3) / 1e12

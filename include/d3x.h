@@ -58,7 +58,7 @@ int d3x_fetch_results(d3_handle_t *handle, double *energy, double *forces, doubl
 
 /* Work counts of the last evaluation, 3 per cell: two-body pairs within the cutoff (each
  * unique pair counted twice, once per atom), CN pairs (same convention), and three-body
- * centre-pairs (each unique triangle counted three times, once per vertex). */
+ * triangles with every edge within the ATM cutoff (each counted once). */
 int d3x_get_counts(d3_handle_t *handle, double *counts);
 
 /* Per-phase GPU timing with HIP events on the handle's stream.  d3x_set_timing(h, 1) enables

@@ -1,0 +1,134 @@
+// Device-side helpers shared by the HIP kernel files: wave64 primitives, math overloads for
+// float/double, the compensated accumulator and the centre-atom geometry of a stencil scan.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+
+#include "d3_engine.hpp"
+
+namespace d3 {
+namespace {
+
+constexpr int WAVE = 64;
+constexpr int BLOCK = 256;
+constexpr int WPB = BLOCK / WAVE;
+constexpr int QCAP = 2 * WAVE;  // per-wave compaction queue (entries)
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & (WAVE - 1); }
+__device__ __forceinline__ uint64_t ballot64(bool p) { return __ballot(p); }
+__device__ __forceinline__ int mbcnt64(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+// Orders LDS traffic between the lanes of one wave (the queue is private to the wave).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+__device__ __forceinline__ int wrapi(int x, int n) {
+    int r = x % n;
+    return r < 0 ? r + n : r;
+}
+
+__device__ __forceinline__ double xsqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float xsqrt(float x) { return sqrtf(x); }
+__device__ __forceinline__ double xexp(double x) { return exp(x); }
+__device__ __forceinline__ float xexp(float x) { return expf(x); }
+__device__ __forceinline__ double xcbrt(double x) { return cbrt(x); }
+__device__ __forceinline__ float xcbrt(float x) { return cbrtf(x); }
+__device__ __forceinline__ double xrsqrt(double x) { return rsqrt(x); }
+__device__ __forceinline__ float xrsqrt(float x) { return rsqrtf(x); }
+template <class T>
+__device__ __forceinline__ T finite_or_zero(T x) {
+    return isfinite(x) ? x : T(0);
+}
+
+// Per-lane accumulator: plain sum in FP64; two-sum compensated (hi, lo) in FP32.
+template <class T>
+struct Acc {
+    T s;
+    __device__ void init() { s = T(0); }
+    __device__ void add(T x) { s += x; }
+};
+template <>
+struct Acc<float> {
+    float hi, lo;
+    __device__ void init() { hi = lo = 0.0f; }
+    __device__ void add(float x) {
+        float s = hi + x;
+        float bp = s - hi;
+        lo += (hi - (s - bp)) + (x - bp);
+        hi = s;
+    }
+};
+
+// Wave-wide total of an accumulator, returned in double in every lane (xor butterfly over the
+// 64 lanes; the FP32 variant keeps the exact two-sum error of every step).
+__device__ __forceinline__ double wave_total(const Acc<double> &a) {
+    double v = a.s;
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+    return v;
+}
+__device__ __forceinline__ double wave_total(const Acc<float> &a) {
+    float hi = a.hi, lo = a.lo;
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) {
+        float oh = __shfl_xor(hi, o, WAVE), ol = __shfl_xor(lo, o, WAVE);
+        float s = hi + oh;
+        float bp = s - hi;
+        float err = (hi - (s - bp)) + (oh - bp);
+        hi = s;
+        lo = lo + ol + err;
+    }
+    return (double)hi + (double)lo;
+}
+// Value of one lane's accumulator.
+__device__ __forceinline__ double acc_value(const Acc<double> &a) { return a.s; }
+__device__ __forceinline__ double acc_value(const Acc<float> &a) { return (double)a.hi + (double)a.lo; }
+
+__device__ __forceinline__ float wave_sumf(float v) {
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+    return v;
+}
+
+// Geometry of one centre atom: its cell, bin coordinates and local position.
+template <class T>
+struct Centre {
+    int i, sid, ti, bx, by, bz, nx, ny, nz, bin_base;
+    T x, y, z;
+    const SysDev *S;
+    __device__ void load(const Params<T> &P, int i_) {
+        i = i_;
+        sid = P.sys_s[i];
+        S = &P.sys[sid];
+        nx = S->nb[0];
+        ny = S->nb[1];
+        nz = S->nb[2];
+        bin_base = S->bin_base;
+        const int b = P.bin_s[i] - bin_base;
+        bx = b % nx;
+        by = (b / nx) % ny;
+        bz = b / (nx * ny);
+        x = P.lx[i];
+        y = P.ly[i];
+        z = P.lz[i];
+        ti = P.type_s[i];
+    }
+    // Atom range of the bin at stencil entry s, and the centre's position plus the image vector.
+    __device__ void bin(const Params<T> &P, int s, int &j0, int &j1, T &ox, T &oy, T &oz) const {
+        const StEntry d = P.st_d[s];
+        const int cx = wrapi(bx + d.dx, nx), cy = wrapi(by + d.dy, ny), cz = wrapi(bz + d.dz, nz);
+        const int nb = bin_base + cx + nx * (cy + ny * cz);
+        j0 = P.bin_start[nb];
+        j1 = P.bin_start[nb + 1];
+        ox = x + P.st_D[4 * s + 0];
+        oy = y + P.st_D[4 * s + 1];
+        oz = z + P.st_D[4 * s + 2];
+    }
+};
+
+}  // namespace
+}  // namespace d3

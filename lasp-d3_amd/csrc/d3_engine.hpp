@@ -77,12 +77,23 @@ struct Params {
     // q4[ne][ne] = r2r4_A r2r4_B, rdamp[ne][ne] (zero: r0ab; BJ: a1 sqrt(3 q4) + a2),
     // r0[ne][ne] (ATM damping radii)
     const T *c6, *cnref, *rcov, *q4, *rdamp, *r0;
-    // ATM neighbour scratch (per block)
+    // ATM: per-block scratch for one centre's forward neighbours (capacity atm_cap):
+    //   atm_i32: j, type (2 x cap); atm_T: record fields (ATM_NF x cap);
+    //   atm_acc: per-neighbour accumulators F[3], dE/dCN, E/3 (5 x cap, double)
+    // and per-atom results (sorted order): fa[3N], ga[N], eas[N] filled with atomics (every
+    // triangle's gradient, dE/dCN and e/3 for each vertex), eat[N] = sum of e over the
+    // triangles anchored at each centre (deterministic total).
     int atm_cap;
-    int *atm_j;
-    T *atm_buf;  // 6 * cap per block: dx, dy, dz, r2, c6, dlc6
+    int *atm_i32;
+    T *atm_T;
+    double *atm_acc;
+    double *fa, *ga, *eas, *eat;
     unsigned *status;
 };
+
+constexpr int ATM_NF = 10;  // record fields per ATM neighbour
+// doubles of ATM accumulator scratch per block: 5 per neighbour slot + 11 x 64 for the centre
+__host__ __device__ constexpr size_t atm_acc_stride(int cap) { return 5 * (size_t)cap + 11 * 64; }
 
 // Host-side launchers (d3_kernels.hip).
 template <class T>
@@ -90,13 +101,19 @@ struct Launch {
     static void bin_sort(const Params<T> &P, int nbins, int *bin_start_mut, hipStream_t s);
     static void cn(const Params<T> &P, hipStream_t s);
     static void twobody(const Params<T> &P, hipStream_t s);
-    static void atm(const Params<T> &P, int nblocks, hipStream_t s);
     static void chain(const Params<T> &P, hipStream_t s);
 };
 
+// Three-body term (d3_atm.hip): zeroes the ATM per-atom arrays, runs the triangle kernel on
+// nblocks persistent one-wave blocks (when P.rc_32 > 0) and merges forces and dE/dCN into
+// P.f / P.g.
+template <class T>
+void launch_atm(const Params<T> &P, int nblocks, hipStream_t s);
+size_t atm_lds_bytes(int ne, int nref, size_t tsize);
+
 // Reduction of per-atom energy / virial / pair counts into per-cell totals (deterministic),
 // and scatter of per-atom results to caller order with unit conversion.
-void launch_reduce(const double *e, const double *vir, const float *npair, const int *chunk_begin,
+void launch_reduce(const double *e, const double *eat, const double *vir, const float *npair, const int *chunk_begin,
                    const int *chunk_end, const int *chunk_sys, int nchunks, const int *sys_chunk_off, int nsys,
                    double *partial, double *out_sys, hipStream_t s);
 template <class T>

@@ -137,7 +137,8 @@ Engine::Engine(const uint16_t *elements, uint64_t num_elements, uint64_t max_len
     const Table &tab = active_table();
     table_name_ = tab.name;
     nref_ = tab.nref;
-    if (nref_ > MAX_NREF) fail(D3X_ERR_INPUT, "table has too many references per element");
+    if (nref_ != 5 && nref_ != 7)  // the reference's two table layouts (src/constants_include.h:14-27)
+        fail(D3X_ERR_INPUT, "table must have 5 or 7 references per element");
     for (uint64_t k = 0; k < num_elements; ++k) {
         if (!tab.supports(elements[k]))
             fail(D3X_ERR_INPUT, "element " + std::to_string(elements[k]) + " is not in the parameter table");
@@ -148,6 +149,8 @@ Engine::Engine(const uint16_t *elements, uint64_t num_elements, uint64_t max_len
     ne_ = (int)elems_.size();
     if (ne_ == 0) fail(D3X_ERR_INPUT, "no elements given");
     if (ne_ > 64) fail(D3X_ERR_INPUT, "at most 64 distinct elements per handle");
+    if (rc3_ > 0 && atm_lds_bytes(ne_, nref_, precision == D3X_FP64 ? 8 : 4) > 65536)
+        fail(D3X_ERR_INPUT, "too many distinct elements for the three-body kernel's LDS tile");
 
     // per-handle tables: C6ref rows and factorised CN references (src/d3_buffer.cu:192-277 gathers
     // the same entries per unique element)
@@ -202,7 +205,7 @@ Engine::~Engine() {
                     &d_type_s, &d_orig_s, &d_bin_s, &d_sys_s, &d_cn, &d_w, &d_dw, &d_e, &d_g, &d_f, &d_vir,
                     &d_npair, &d_out_f, &d_out_e, &d_out_cn, &d_out_g, &d_bin_count, &d_bin_start, &d_cursor,
                     &d_sys, &d_st_d, &d_st_D, &d_chunk_b, &d_chunk_e, &d_chunk_s, &d_sys_chunk, &d_partial,
-                    &d_out_sys, &d_tab, &d_status, &d_atm_j, &d_atm_buf};
+                    &d_out_sys, &d_tab, &d_status, &d_atm_i32, &d_atm_T, &d_atm_acc, &d_atm_pa};
     for (void **p : all) dfree(*p);
     for (auto &e : ev_)
         if (e) (void)hipEventDestroy(e);
@@ -234,6 +237,7 @@ void Engine::alloc_atoms() {
     d_f = dalloc(3 * N * sizeof(double));
     d_vir = dalloc(6 * N * sizeof(double));
     d_npair = dalloc(3 * N * sizeof(float));
+    d_atm_pa = dalloc(6 * N * sizeof(double));
     d_out_f = dalloc(3 * N * sizeof(double));
     d_out_e = dalloc(N * sizeof(double));
     d_out_cn = dalloc(N * sizeof(double));
@@ -464,25 +468,29 @@ void Engine::upload_geometry() {
 
 void Engine::ensure_atm_scratch() {
     if (rc3_ <= 0) return;
-    // neighbour capacity: three times the mean count within rc3 plus a margin; a centre above it
-    // raises D3X_ERR_OVERFLOW and the caller retries with a larger capacity
+    // forward-neighbour capacity: 1.5 x the mean number of forward neighbours (half of all
+    // neighbours within rc3) plus a margin; a centre above it raises D3X_ERR_OVERFLOW and the
+    // caller retries with a larger capacity
     double dens = 0;
     for (size_t s = 0; s < sys_.size(); ++s) dens = std::max(dens, (double)sys_[s].n / std::fabs(det3(sys_[s].H)));
-    const double mean = dens * 4.18879020478639 * rc3_ * rc3_ * rc3_;
-    int cap = (int)std::min(262144.0, 3.0 * mean + 1024.0);
-    cap = std::max(cap, atm_cap_);
-    int nb = atm_blocks_req_ > 0 ? atm_blocks_req_ : 1024;
+    const double mean = 0.5 * dens * 4.18879020478639 * rc3_ * rc3_ * rc3_;
+    int cap = (int)std::min(262144.0, 1.5 * mean + 1024.0);
+    cap = (std::max(cap, atm_cap_) + 63) / 64 * 64;
+    int nb = atm_blocks_req_ > 0 ? atm_blocks_req_ : 4096;
     nb = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)nb, n_));
     atm_cap_ = cap;
     atm_blocks_ = nb;
     // the kernel indexes scratch as blockIdx.x * atm_cap_: reallocate whenever either grows
-    if ((size_t)cap * nb > atm_alloc_entries_ || !d_atm_j) {
-        dfree(d_atm_j);
-        dfree(d_atm_buf);
+    if ((size_t)cap * nb > atm_alloc_entries_ || nb > atm_alloc_blocks_ || !d_atm_i32) {
+        dfree(d_atm_i32);
+        dfree(d_atm_T);
+        dfree(d_atm_acc);
         const size_t ts = precision_ == D3X_FP64 ? sizeof(double) : sizeof(float);
         atm_alloc_entries_ = (size_t)cap * nb;
-        d_atm_j = dalloc(atm_alloc_entries_ * sizeof(int));
-        d_atm_buf = dalloc(atm_alloc_entries_ * 6 * ts);
+        atm_alloc_blocks_ = nb;
+        d_atm_i32 = dalloc(atm_alloc_entries_ * 2 * sizeof(int));
+        d_atm_T = dalloc(atm_alloc_entries_ * ATM_NF * ts);
+        d_atm_acc = dalloc((size_t)nb * atm_acc_stride(cap) * sizeof(double));
     }
 }
 
@@ -543,8 +551,14 @@ Params<T> Engine::params() {
     P.rdamp = tab + tab_off_[4];
     P.r0 = tab + tab_off_[5];
     P.atm_cap = atm_cap_;
-    P.atm_j = (int *)d_atm_j;
-    P.atm_buf = (T *)d_atm_buf;
+    P.atm_i32 = (int *)d_atm_i32;
+    P.atm_T = (T *)d_atm_T;
+    P.atm_acc = (double *)d_atm_acc;
+    // packed for the current atom count so that one fill of 6 n doubles clears all four
+    P.fa = (double *)d_atm_pa;
+    P.ga = P.fa + 3 * n_;
+    P.eas = P.ga + n_;
+    P.eat = P.eas + n_;
     P.status = (unsigned *)d_status;
     return P;
 }
@@ -593,11 +607,11 @@ void Engine::run() {
     mark(2);
     Launch<T>::twobody(P, stream_);
     mark(3);
-    if (rc3_ > 0) Launch<T>::atm(P, atm_blocks_, stream_);
+    launch_atm<T>(P, atm_blocks_, stream_);  // zeroes the ATM arrays; kernels only when rc3 > 0
     mark(4);
     Launch<T>::chain(P, stream_);
     mark(5);
-    launch_reduce((const double *)d_e, (const double *)d_vir, (const float *)d_npair, (const int *)d_chunk_b,
+    launch_reduce((const double *)d_e, P.eat, (const double *)d_vir, (const float *)d_npair, (const int *)d_chunk_b,
                   (const int *)d_chunk_e, (const int *)d_chunk_s, (int)chunk_b_.size(), (const int *)d_sys_chunk,
                   (int)sysdev_.size(), (double *)d_partial, (double *)d_out_sys, stream_);
     launch_finalize(P, (double *)d_out_f, (double *)d_out_e, (double *)d_out_cn, (double *)d_out_g, stream_);

@@ -38,8 +38,8 @@ class Engine {
     // forces (eV/A, caller order).  Throws Error on device status.
     void fetch(double *energies, double *forces, double *stresses);
     void fetch_atom_data(double *cn, double *dedcn, double *atom_energy);
-    // Pair counts of the last evaluation per cell: two-body ordered pairs, CN ordered pairs,
-    // ATM centre-pairs (each unique triangle counted three times).
+    // Work counts of the last evaluation per cell: two-body ordered pairs, CN ordered pairs,
+    // ATM triangles (each counted once).
     void fetch_counts(double *counts);
 
     uint64_t num_atoms() const { return n_; }
@@ -124,9 +124,11 @@ class Engine {
     void *d_tab = nullptr;
     size_t tab_off_[6] = {0};
     void *d_status = nullptr;
-    void *d_atm_j = nullptr, *d_atm_buf = nullptr;
+    void *d_atm_i32 = nullptr, *d_atm_T = nullptr, *d_atm_acc = nullptr;
+    void *d_atm_pa = nullptr;  // per-atom ATM results: fa[3N] | ga[N] | eas[N] | eat[N] (double)
     int atm_blocks_ = 0, atm_cap_ = 0;
     size_t atm_alloc_entries_ = 0;  // allocated scratch entries (>= atm_blocks_ * atm_cap_)
+    int atm_alloc_blocks_ = 0;      // blocks the scratch was allocated for (>= atm_blocks_)
 
     bool timing_ = false, timing_pending_ = false;
     hipEvent_t ev_[NPHASE + 1] = {};

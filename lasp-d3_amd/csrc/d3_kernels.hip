@@ -4,7 +4,7 @@
 //   bin sort   k_zero -> k_bin -> k_scan -> k_scatter -> k_canon -> k_gather
 //   k_cn       coordination numbers + normalised CN weights per atom
 //   k_2b       two-body energy, dE/dCN, force, virial
-//   k_atm      Axilrod-Teller-Muto three-body term (adds to energy, dE/dCN, force, virial)
+//   k_atm      Axilrod-Teller-Muto three-body term (d3_atm.hip)
 //   k_chain    dE/dCN chain-rule force and virial
 //   reduce / finalize (d3_reduce.hip)
 // The formulas restate the reference's src/d3_kernel.cu; each kernel cites its counterpart.
@@ -13,126 +13,12 @@
 
 #include <cfloat>
 
-#include "d3_engine.hpp"
+#include "d3_device.hpp"
 
 namespace d3 {
 namespace {
 
-constexpr int WAVE = 64;
-constexpr int BLOCK = 256;
-constexpr int WPB = BLOCK / WAVE;
-constexpr int QCAP = 2 * WAVE;  // per-wave compaction queue (entries)
-
-__device__ __forceinline__ int lane_id() { return threadIdx.x & (WAVE - 1); }
-__device__ __forceinline__ uint64_t ballot64(bool p) { return __ballot(p); }
-__device__ __forceinline__ int mbcnt64(uint64_t m) {
-    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-}
-// Orders LDS traffic between the lanes of one wave (the queue is private to the wave).
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-__device__ __forceinline__ int wrapi(int x, int n) {
-    int r = x % n;
-    return r < 0 ? r + n : r;
-}
-
-__device__ __forceinline__ double xsqrt(double x) { return sqrt(x); }
-__device__ __forceinline__ float xsqrt(float x) { return sqrtf(x); }
-__device__ __forceinline__ double xexp(double x) { return exp(x); }
-__device__ __forceinline__ float xexp(float x) { return expf(x); }
-__device__ __forceinline__ double xcbrt(double x) { return cbrt(x); }
-__device__ __forceinline__ float xcbrt(float x) { return cbrtf(x); }
-__device__ __forceinline__ double xrsqrt(double x) { return rsqrt(x); }
-__device__ __forceinline__ float xrsqrt(float x) { return rsqrtf(x); }
-template <class T>
-__device__ __forceinline__ T finite_or_zero(T x) {
-    return isfinite(x) ? x : T(0);
-}
-
-// Per-lane accumulator: plain sum in FP64; two-sum compensated (hi, lo) in FP32.
-template <class T>
-struct Acc {
-    T s;
-    __device__ void init() { s = T(0); }
-    __device__ void add(T x) { s += x; }
-};
-template <>
-struct Acc<float> {
-    float hi, lo;
-    __device__ void init() { hi = lo = 0.0f; }
-    __device__ void add(float x) {
-        float s = hi + x;
-        float bp = s - hi;
-        lo += (hi - (s - bp)) + (x - bp);
-        hi = s;
-    }
-};
-
-// Wave-wide total of an accumulator, returned in double in every lane (xor butterfly over the
-// 64 lanes; the FP32 variant keeps the exact two-sum error of every step).
-__device__ __forceinline__ double wave_total(const Acc<double> &a) {
-    double v = a.s;
-#pragma unroll
-    for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
-    return v;
-}
-__device__ __forceinline__ double wave_total(const Acc<float> &a) {
-    float hi = a.hi, lo = a.lo;
-#pragma unroll
-    for (int o = WAVE / 2; o > 0; o >>= 1) {
-        float oh = __shfl_xor(hi, o, WAVE), ol = __shfl_xor(lo, o, WAVE);
-        float s = hi + oh;
-        float bp = s - hi;
-        float err = (hi - (s - bp)) + (oh - bp);
-        hi = s;
-        lo = lo + ol + err;
-    }
-    return (double)hi + (double)lo;
-}
-__device__ __forceinline__ float wave_sumf(float v) {
-#pragma unroll
-    for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
-    return v;
-}
-
-// Geometry of one centre atom: its cell, bin coordinates and local position.
-template <class T>
-struct Centre {
-    int i, sid, ti, bx, by, bz, nx, ny, nz, bin_base;
-    T x, y, z;
-    const SysDev *S;
-    __device__ void load(const Params<T> &P, int i_) {
-        i = i_;
-        sid = P.sys_s[i];
-        S = &P.sys[sid];
-        nx = S->nb[0];
-        ny = S->nb[1];
-        nz = S->nb[2];
-        bin_base = S->bin_base;
-        const int b = P.bin_s[i] - bin_base;
-        bx = b % nx;
-        by = (b / nx) % ny;
-        bz = b / (nx * ny);
-        x = P.lx[i];
-        y = P.ly[i];
-        z = P.lz[i];
-        ti = P.type_s[i];
-    }
-    // Atom range of the bin at stencil entry s, and the centre's position plus the image vector.
-    __device__ void bin(const Params<T> &P, int s, int &j0, int &j1, T &ox, T &oy, T &oz) const {
-        const StEntry d = P.st_d[s];
-        const int cx = wrapi(bx + d.dx, nx), cy = wrapi(by + d.dy, ny), cz = wrapi(bz + d.dz, nz);
-        const int nb = bin_base + cx + nx * (cy + ny * cz);
-        j0 = P.bin_start[nb];
-        j1 = P.bin_start[nb + 1];
-        ox = x + P.st_D[4 * s + 0];
-        oy = y + P.st_D[4 * s + 1];
-        oz = z + P.st_D[4 * s + 2];
-    }
-};
+// (device helpers: d3_device.hpp)
 
 // ---------------------------------------------------------------------------------------------
 // Bin sort: counting sort of atoms by bin, then a canonical (ascending caller index) order
@@ -289,6 +175,7 @@ __global__ __launch_bounds__(BLOCK) void k_cn(Params<T> P) {
         const T cn = (T)cnd;
         P.cn[i] = cn;
         P.npair[3 * i + 1] = cnt;
+        P.npair[3 * i + 2] = 0.0f;  // three-body count, set by the ATM kernel when it runs
         T ex[MAX_NREF], la[MAX_NREF];
         T mx = -T(FLT_MAX);
         for (int a = 0; a < P.nref; ++a) {
@@ -570,249 +457,6 @@ __global__ __launch_bounds__(BLOCK) void k_chain(Params<T> P) {
     }
 }
 
-// ---------------------------------------------------------------------------------------------
-// Axilrod-Teller-Muto three-body term, src/d3_kernel.cu:1199-1509.  One block per centre i
-// (persistent blocks loop over centres).  Phase 1 collects the centre's neighbour images with
-// 1e-12 <= r^2 <= rc3^2 (block-wide compaction, deterministic order) into per-block scratch,
-// with C6_ij and dlnC6_ij/dCN_i.  Phase 2 visits every unordered neighbour pair {j, k} with
-// 1e-12 <= r_jk^2 <= rc3^2: each thread owns one k (registers), the j side is staged through
-// LDS tiles.  Every periodic triangle is visited once per vertex, so the centre takes e/3 of
-// its energy, the gradient of e with respect to its own position, its own dE/dCN part and half
-// of the virial of its two incident edges (the totals are exact; forces of index-degenerate
-// triangles are physically consistent, unlike the reference's unscaled forces).
-//   e = C9 A f,  C9 = -sqrt(C6_ij C6_ik C6_jk),  A = (1 + 3 cos cos cos) / (r_ij r_ik r_jk)^3
-//   f = 1 / (1 + 6 x^16),  x = (4/3) (R0_ij R0_ik R0_jk)^(1/3) / (r_ij r_ik r_jk)^(1/3)
-// With a = r_ij^2, b = r_ik^2, c = r_jk^2, Q = (a+b-c)(a-b+c)(-a+b+c):
-//   A = (abc)^(-3/2) + (3/8) Q (abc)^(-5/2)
-//   t_ij = (1/r_ij) de/dr_ij = 2 C9 (dA/da f + 16 A x^16 f^2 / a)
-//   F_i += t_ij d_ij + t_ik d_ik,  dE/dCN_i += (C9/2)(dlnC6_ij + dlnC6_ik) A f
-// ---------------------------------------------------------------------------------------------
-constexpr int ATM_TILE = BLOCK;
-
-template <class T>
-struct AtmTile {
-    T dx[ATM_TILE], dy[ATM_TILE], dz[ATM_TILE], r2[ATM_TILE], c6[ATM_TILE], dl[ATM_TILE];
-    T w[MAX_NREF][ATM_TILE];
-    int t[ATM_TILE];
-};
-
-// Sum over the block's waves of a per-wave total (already equal in every lane of a wave);
-// red holds WPB entries of LDS.  Fixed order, so the result is deterministic.
-__device__ __forceinline__ double block_sum(double wave_tot, double *red) {
-    const int wv = threadIdx.x >> 6, lane = lane_id();
-    __syncthreads();
-    if (lane == 0) red[wv] = wave_tot;
-    __syncthreads();
-    double s = 0.0;
-    for (int k = 0; k < WPB; ++k) s += red[k];
-    return s;
-}
-
-template <class T>
-__global__ __launch_bounds__(BLOCK) void k_atm(Params<T> P) {
-    __shared__ AtmTile<T> J;
-    __shared__ T yc[2 * 64 * MAX_NREF];  // centre's y_i[B][b] (C6 row) -- ne * nref <= 64 * MAX_NREF
-    __shared__ int wcount[WPB];
-    __shared__ double red[WPB];
-    const int tid = threadIdx.x, wv = tid >> 6, lane = lane_id();
-    const int cap = P.atm_cap;
-    int *lj = P.atm_j + (size_t)blockIdx.x * cap;
-    T *lb = P.atm_buf + (size_t)blockIdx.x * cap * 6;
-    T *Ldx = lb, *Ldy = lb + cap, *Ldz = lb + 2 * cap, *Lr2 = lb + 3 * cap, *Lc6 = lb + 4 * cap, *Ldl = lb + 5 * cap;
-    const int nref = P.nref, ne = P.ne;
-
-    for (int i = blockIdx.x; i < P.n; i += gridDim.x) {
-        Centre<T> C;
-        C.load(P, i);
-        // centre C6 rows y_i[B] = C6ref_AB^T w_i and dy_i[B] = C6ref_AB^T dw_i
-        __syncthreads();
-        for (int idx = tid; idx < ne * nref; idx += BLOCK) {
-            const int B = idx / nref, bb = idx % nref;
-            T sy = T(0), sdy = T(0);
-            for (int a = 0; a < nref; ++a) {
-                const T c = P.c6[((C.ti * ne + B) * nref + a) * nref + bb];
-                sy += c * P.w[a * P.n + i];
-                sdy += c * P.dw[a * P.n + i];
-            }
-            yc[idx] = sy;
-            yc[ne * nref + idx] = sdy;
-        }
-        // phase 1: neighbour list
-        int count = 0;
-        const int s0 = C.S->st_off[CUT_ATM], s1 = s0 + C.S->st_cnt[CUT_ATM];
-        for (int s = s0; s < s1; ++s) {
-            int j0, j1;
-            T ox, oy, oz;
-            C.bin(P, s, j0, j1, ox, oy, oz);
-            for (int jb = j0; jb < j1; jb += BLOCK) {
-                const int j = jb + tid;
-                bool in = false;
-                T dx = T(0), dy = T(0), dz = T(0), r2 = T(0);
-                if (j < j1) {
-                    dx = ox - P.lx[j];
-                    dy = oy - P.ly[j];
-                    dz = oz - P.lz[j];
-                    r2 = dx * dx + dy * dy + dz * dz;
-                    in = r2 <= P.rc_32 && r2 >= T(1e-12);
-                }
-                const uint64_t m = ballot64(in);
-                __syncthreads();
-                if (lane == 0) wcount[wv] = __popcll(m);
-                __syncthreads();
-                int off = count;
-                int tot = 0;
-                for (int k = 0; k < WPB; ++k) {
-                    if (k < wv) off += wcount[k];
-                    tot += wcount[k];
-                }
-                if (in) {
-                    const int pos = off + mbcnt64(m);
-                    if (pos < cap) {
-                        lj[pos] = j;
-                        Ldx[pos] = dx;
-                        Ldy[pos] = dy;
-                        Ldz[pos] = dz;
-                        Lr2[pos] = r2;
-                    }
-                }
-                count += tot;
-            }
-        }
-        __syncthreads();
-        if (count > cap) {
-            if (tid == 0) atomicOr(P.status, 2u);
-            continue;  // uniform: the centre is skipped and flagged
-        }
-        const int n = count;
-        // C6_ij and dln C6_ij / dCN_i per neighbour
-        for (int p = tid; p < n; p += BLOCK) {
-            const int j = lj[p], tj = P.type_s[j];
-            T c6 = T(0), dc6 = T(0);
-            for (int b = 0; b < nref; ++b) {
-                const T wj = P.w[b * P.n + j];
-                c6 += yc[tj * nref + b] * wj;
-                dc6 += yc[ne * nref + tj * nref + b] * wj;
-            }
-            Lc6[p] = c6;
-            Ldl[p] = c6 != T(0) ? dc6 / c6 : T(0);
-        }
-        __syncthreads();
-
-        // phase 2: triangles
-        Acc<T> aE, aG, aF[3], aV[6];
-        aE.init();
-        aG.init();
-        for (int k = 0; k < 3; ++k) aF[k].init();
-        for (int k = 0; k < 6; ++k) aV[k].init();
-        float cnt = 0.0f;
-        const T rc32 = P.rc_32;
-        for (int kb = 0; kb * ATM_TILE < n; ++kb) {
-            const int k = kb * ATM_TILE + tid;
-            const bool kval = k < n;
-            T kx = T(0), ky = T(0), kz = T(0), kr2 = T(1), kc6 = T(0), kdl = T(0);
-            T kw[MAX_NREF];
-            int tk = 0;
-            for (int a = 0; a < MAX_NREF; ++a) kw[a] = T(0);
-            if (kval) {
-                const int jk = lj[k];
-                kx = Ldx[k];
-                ky = Ldy[k];
-                kz = Ldz[k];
-                kr2 = Lr2[k];
-                kc6 = Lc6[k];
-                kdl = Ldl[k];
-                tk = P.type_s[jk];
-                for (int a = 0; a < nref; ++a) kw[a] = P.w[a * P.n + jk];
-            }
-            const T r0ik = P.r0[C.ti * ne + tk];
-            for (int jt = 0; jt <= kb; ++jt) {
-                __syncthreads();
-                const int p = jt * ATM_TILE + tid;
-                if (p < n) {
-                    const int jj = lj[p];
-                    J.dx[tid] = Ldx[p];
-                    J.dy[tid] = Ldy[p];
-                    J.dz[tid] = Ldz[p];
-                    J.r2[tid] = Lr2[p];
-                    J.c6[tid] = Lc6[p];
-                    J.dl[tid] = Ldl[p];
-                    J.t[tid] = P.type_s[jj];
-                    for (int a = 0; a < nref; ++a) J.w[a][tid] = P.w[a * P.n + jj];
-                }
-                __syncthreads();
-                if (!kval) continue;
-                int jend = (jt == kb) ? tid : ATM_TILE;
-                const int rem = n - jt * ATM_TILE;
-                if (jend > rem) jend = rem;
-                for (int q = 0; q < jend; ++q) {
-                    const T jx = J.dx[q], jy = J.dy[q], jz = J.dz[q];
-                    const T ex = kx - jx, ey = ky - jy, ez = kz - jz;
-                    const T c = ex * ex + ey * ey + ez * ez;
-                    if (c > rc32 || c < T(1e-12)) continue;
-                    const T a = J.r2[q], b = kr2;
-                    const int tj = J.t[q];
-                    // C6_jk = w_j^T C6ref_{tj tk} w_k
-                    T c6jk = T(0);
-                    const T *ct = P.c6 + (size_t)(tj * ne + tk) * nref * nref;
-                    for (int aa = 0; aa < nref; ++aa) {
-                        T u = T(0);
-                        for (int bb = 0; bb < nref; ++bb) u += ct[aa * nref + bb] * kw[bb];
-                        c6jk += J.w[aa][q] * u;
-                    }
-                    const T C9 = -xsqrt(J.c6[q] * kc6 * c6jk);
-                    const T abc = a * b * c;
-                    const T iR = xrsqrt(abc);
-                    const T iR3 = iR * iR * iR;
-                    const T iabc = T(1) / abc;
-                    const T u1 = a + b - c, u2 = a - b + c, u3 = -a + b + c;
-                    const T Qv = u1 * u2 * u3;
-                    const T iR5 = iR3 * iabc;
-                    const T Aang = iR3 + T(0.375) * Qv * iR5;
-                    const T dQa = u2 * u3 + u1 * u3 - u1 * u2;
-                    const T dQb = u2 * u3 - u1 * u3 + u1 * u2;
-                    const T ia = T(1) / a, ib = T(1) / b;
-                    const T dAa = T(-1.5) * iR3 * ia + T(0.375) * (dQa * iR5 - T(2.5) * Qv * iR5 * ia);
-                    const T dAb = T(-1.5) * iR3 * ib + T(0.375) * (dQb * iR5 - T(2.5) * Qv * iR5 * ib);
-                    const T r0p = P.r0[C.ti * ne + tj] * r0ik * P.r0[tj * ne + tk];
-                    const T x2 = T(16.0 / 9.0) * xcbrt(r0p * r0p * iabc);
-                    const T x4 = x2 * x2, x8 = x4 * x4, x16 = x8 * x8;
-                    const T fd = T(1) / (T(1) + T(6) * x16);
-                    const T e = C9 * Aang * fd;
-                    const T dfa = T(16) * x16 * fd * fd;
-                    const T tij = T(2) * C9 * (dAa * fd + Aang * dfa * ia);
-                    const T tik = T(2) * C9 * (dAb * fd + Aang * dfa * ib);
-                    aE.add(e * T(1.0 / 3.0));
-                    aG.add(T(0.5) * C9 * Aang * fd * (J.dl[q] + kdl));
-                    aF[0].add(tij * jx + tik * kx);
-                    aF[1].add(tij * jy + tik * ky);
-                    aF[2].add(tij * jz + tik * kz);
-                    const T hj = T(-0.5) * tij, hk = T(-0.5) * tik;
-                    aV[0].add(hj * jx * jx + hk * kx * kx);
-                    aV[1].add(hj * jy * jy + hk * ky * ky);
-                    aV[2].add(hj * jz * jz + hk * kz * kz);
-                    aV[3].add(hj * jx * jy + hk * kx * ky);
-                    aV[4].add(hj * jx * jz + hk * kx * kz);
-                    aV[5].add(hj * jy * jz + hk * ky * kz);
-                    cnt += 1.0f;
-                }
-            }
-        }
-        const double E = block_sum(wave_total(aE), red), G = block_sum(wave_total(aG), red);
-        double F[3], V[6];
-        for (int k = 0; k < 3; ++k) F[k] = block_sum(wave_total(aF[k]), red);
-        for (int k = 0; k < 6; ++k) V[k] = block_sum(wave_total(aV[k]), red);
-        const double cn3 = block_sum((double)wave_sumf(cnt), red);
-        if (tid == 0) {
-            const double iv = C.S->inv_vol;
-            P.e[i] += E;
-            P.g[i] += G;
-            for (int k = 0; k < 3; ++k) P.f[3 * i + k] += F[k];
-            for (int k = 0; k < 6; ++k) P.vir[6 * i + k] += V[k] * iv;
-            P.npair[3 * i + 2] = (float)cn3;
-        }
-    }
-}
-
 }  // namespace
 
 template <class T>
@@ -845,12 +489,6 @@ template <class T>
 void Launch<T>::chain(const Params<T> &P, hipStream_t s) {
     if (P.n == 0) return;
     hipLaunchKernelGGL(k_chain<T>, dim3((P.n + WPB - 1) / WPB), dim3(BLOCK), 0, s, P);
-}
-
-template <class T>
-void Launch<T>::atm(const Params<T> &P, int nblocks, hipStream_t s) {
-    if (P.n == 0) return;
-    hipLaunchKernelGGL(k_atm<T>, dim3(nblocks), dim3(BLOCK), 0, s, P);
 }
 
 template struct Launch<double>;

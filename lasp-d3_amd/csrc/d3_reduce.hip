@@ -23,14 +23,15 @@ __device__ double block_sum_d(double v, double *red) {
     return r;
 }
 
-__global__ __launch_bounds__(RB) void k_reduce_chunks(const double *e, const double *vir, const float *npair,
-                                                       const int *cb, const int *ce, double *partial) {
+__global__ __launch_bounds__(RB) void k_reduce_chunks(const double *e, const double *eat, const double *vir,
+                                                       const float *npair, const int *cb, const int *ce,
+                                                       double *partial) {
     __shared__ double red[RB];
     const int c = blockIdx.x;
     double acc[RED_W];
     for (int k = 0; k < RED_W; ++k) acc[k] = 0.0;
     for (int p = cb[c] + threadIdx.x; p < ce[c]; p += RB) {
-        acc[0] += e[p];
+        acc[0] += e[p] + eat[p];  // two-body half-pair energies + anchored ATM triangle energies
         for (int k = 0; k < 6; ++k) acc[1 + k] += vir[6 * p + k];
         for (int k = 0; k < 3; ++k) acc[7 + k] += (double)npair[3 * p + k];
     }
@@ -60,7 +61,7 @@ __global__ void k_finalize(Params<T> P, double *out_f, double *out_e, double *ou
     const int o = P.orig_s[p];
     const double fs = HARTREE_EV / BOHR_A;  // Hartree/bohr -> eV/Angstrom, src/d3.cu:207-212
     for (int k = 0; k < 3; ++k) out_f[3 * o + k] = P.f[3 * p + k] * fs;
-    out_e[o] = -HARTREE_EV * P.e[p];  // E = -Hartree * E+, src/d3.cu:206
+    out_e[o] = -HARTREE_EV * (P.e[p] + P.eas[p]);  // E = -Hartree * E+, src/d3.cu:206
     out_cn[o] = (double)P.cn[p];
     out_g[o] = P.g[p];
 }
@@ -76,12 +77,12 @@ void launch_fill_u32(unsigned *p, int n, unsigned v, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_fill_u32, dim3((n + 255) / 256), dim3(256), 0, s, p, n, v);
 }
 
-void launch_reduce(const double *e, const double *vir, const float *npair, const int *chunk_begin,
+void launch_reduce(const double *e, const double *eat, const double *vir, const float *npair, const int *chunk_begin,
                    const int *chunk_end, const int * /*chunk_sys*/, int nchunks, const int *sys_chunk_off, int nsys,
                    double *partial, double *out_sys, hipStream_t s) {
     if (nchunks > 0)
-        hipLaunchKernelGGL(k_reduce_chunks, dim3(nchunks), dim3(RB), 0, s, e, vir, npair, chunk_begin, chunk_end,
-                           partial);
+        hipLaunchKernelGGL(k_reduce_chunks, dim3(nchunks), dim3(RB), 0, s, e, eat, vir, npair, chunk_begin,
+                           chunk_end, partial);
     hipLaunchKernelGGL(k_reduce_sys, dim3(nsys), dim3(RB), 0, s, partial, sys_chunk_off, out_sys);
 }
 

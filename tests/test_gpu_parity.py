@@ -117,14 +117,22 @@ def test_drop_in_float_api(table):
     assert f.dtype == np.float32 and f.shape == (10, 3) and st.shape == (3, 3)
 
 
-def test_repeat_is_deterministic(table):
-    """ResultStable (test/d3_test.cpp:89-129): repeated evaluations agree bit for bit."""
+@pytest.mark.parametrize("atm", [0.0, 40.0])
+def test_repeat_is_stable(table, atm):
+    """ResultStable (test/d3_test.cpp:89-129, 1e-7 eV / 1e-5 there): without the three-body term
+    repeated evaluations agree bit for bit; with it the energy still does, while forces and stress
+    carry the FP64-atomic scatter of the triangle terms (order-dependent at the 1e-16 level)."""
     s = systems.si64(seed=21)
-    h = d3_lib.Handle(s["Z"], len(s["Z"]), cutoff=s["cutoff"], cn_cutoff=s["cn_cutoff"], atm_cutoff=s["atm"],
+    h = d3_lib.Handle(s["Z"], len(s["Z"]), cutoff=s["cutoff"], cn_cutoff=s["cn_cutoff"], atm_cutoff=atm,
                       damping=1, functional=0)
     h.set_atoms(s["pos"], s["Z"])
     h.set_cell(s["cell"])
     e0, f0, s0 = h.compute()
     for _ in range(8):
         e, f, st = h.compute()
-        assert e == e0 and np.array_equal(f, f0) and np.array_equal(st, s0)
+        assert e == e0
+        if atm == 0.0:
+            assert np.array_equal(f, f0) and np.array_equal(st, s0)
+        else:
+            assert np.max(np.abs(f - f0)) <= 1e-13 * np.max(np.abs(f0))
+            assert np.max(np.abs(st - s0)) <= 1e-13 * np.max(np.abs(s0))
