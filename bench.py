@@ -9,9 +9,10 @@ for the "vs plain FP64" comparison, and the FP32 error against it is reported.
 
 Workloads (seeded generators in lasp-d3_amd/src/d3_structures.py; PBE-D3(BJ), two-body cutoff
 sqrt(9000) bohr, CN and ATM cutoff 40 bohr):
-  water  4096 rigid H2O, 12,288 atoms, 49.7 A cubic cell (default)
+  cu1m   fcc Cu 63^3, 1,000,188 atoms, a = 3.615 A, jitter N(0, 0.05 A) (default: the cell the
+         FP32-vs-FP64 target is quoted on)
+  water  4096 rigid H2O, 12,288 atoms, 49.7 A cubic cell
   si512  diamond Si 4x4x4, 512 atoms
-  cu1m   fcc Cu 63^3, 1,000,188 atoms
   batch  256 triclinic cells of 50-5000 atoms (~300k atoms), one batched evaluation
 
 Prints ONE JSON line.  roofline: the dominant kernel's algorithmic FP32 work per launch
@@ -158,10 +159,10 @@ def cpu_baseline(w, table, cn, g, target_s=15.0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="water", choices=["water", "si512", "cu1m", "batch"])
-    ap.add_argument("--fp64-steps", type=int, default=None)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="cu1m", choices=["water", "si512", "cu1m", "batch"])
+    ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -52,6 +52,9 @@ __device__ __forceinline__ double inv_cbrt(double y) {
 }
 __device__ __forceinline__ float xrcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ double xrcp(double x) { return 1.0 / x; }
+// FP32: the hardware square root (1 ulp) without the correctly-rounded fix-up sequence.
+__device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ double fast_sqrt(double x) { return sqrt(x); }
 
 // A wave-uniform float, moved to a scalar register.
 __device__ __forceinline__ float uni(float v) {
@@ -90,9 +93,8 @@ __global__ __launch_bounds__(AW) void k_atm(Params<T> P) {
     int *Lj = P.atm_i32 + (size_t)blockIdx.x * cap * 2;
     int *Lt = Lj + cap;
     T *LT = P.atm_T + (size_t)blockIdx.x * cap * ATM_NF;
-    // per-neighbour FP64 accumulators La[5][cap] and the centre's per-lane accumulators Cs[11][64]
+    // per-neighbour FP64 accumulators La[5][cap]: F_p (3), H_p, Gc_p (see phase 4)
     double *La = P.atm_acc + (size_t)blockIdx.x * atm_acc_stride(cap);
-    double *Cs = La + 5 * (size_t)cap;
     const T rc32 = P.rc_32;
 
     for (int idx = lane; idx < ne * ne; idx += AW) R0sq[idx] = P.r0[idx] * P.r0[idx];
@@ -184,15 +186,14 @@ __global__ __launch_bounds__(AW) void k_atm(Params<T> P) {
         __syncthreads();
 
         // ---- phase 3: pairs {j, k}, j < k, of forward neighbours.  Per-pair terms are summed in T
-        // over one 64 x 64 tile (at most 64 terms per sum); tile sums go to FP64 accumulators
-        // (La for the neighbours, Cs for the centre), which lane l alone touches for its slots.
-        for (int q = 0; q < 11; ++q) Cs[q * AW + lane] = 0.0;
+        // over one 64 x 64 tile (at most 64 terms per sum); tile sums go to the FP64 per-neighbour
+        // accumulators La, which lane l alone touches for its slots.
         float cnt = 0.0f;
         const int ntile = (nf + AW - 1) / AW;
         for (int kb = 0; kb < ntile; ++kb) {
             const int k = kb * AW + lane;
             const bool kval = k < nf;
-            T kx = T(0), ky = T(0), kz = T(0), kb2 = T(1), kib = T(1), kc6 = T(0), kdli = T(0), kdlk = T(0), kr0 = T(0);
+            T kx = T(0), ky = T(0), kz = T(0), kb2 = T(1), kib = T(1), kc6 = T(0), kr0 = T(0);
             T kw[NR], kdw[NR];
             int tk = 0;
             for (int a = 0; a < NR; ++a) kw[a] = kdw[a] = T(0);
@@ -204,8 +205,6 @@ __global__ __launch_bounds__(AW) void k_atm(Params<T> P) {
                 kb2 = LT[F_R2 * cap + k];
                 kib = LT[F_IR2 * cap + k];
                 kc6 = LT[F_C6 * cap + k];
-                kdli = LT[F_DLI * cap + k];
-                kdlk = LT[F_DLJ * cap + k];
                 kr0 = LT[F_R0SQ * cap + k];
                 tk = Lt[k];
                 for (int a = 0; a < nref; ++a) {
@@ -246,8 +245,6 @@ __global__ __launch_bounds__(AW) void k_atm(Params<T> P) {
                     Jf[F_R2 * AW + lane] = LT[F_R2 * cap + pj];
                     Jf[F_IR2 * AW + lane] = LT[F_IR2 * cap + pj];
                     Jf[F_C6 * AW + lane] = LT[F_C6 * cap + pj];
-                    Jf[F_DLI * AW + lane] = LT[F_DLI * cap + pj];
-                    Jf[F_DLJ * AW + lane] = LT[F_DLJ * cap + pj];
                     Jf[F_R0SQ * AW + lane] = LT[F_R0SQ * cap + pj];
                     const int jj = Lj[pj], tj = Lt[pj];
                     Jt[lane] = tj;
@@ -271,10 +268,8 @@ __global__ __launch_bounds__(AW) void k_atm(Params<T> P) {
                 for (int q = 0; q < 5; ++q) Jacc[q * AW + lane] = T(0);
                 wave_sync();
 
-                // tile-local partial sums (folded into the compensated accumulators per tile)
-                T pE = T(0), pG = T(0), pF0 = T(0), pF1 = T(0), pF2 = T(0);
-                T pW0 = T(0), pW1 = T(0), pW2 = T(0), pW3 = T(0), pW4 = T(0), pW5 = T(0);
-                T kF0 = T(0), kF1 = T(0), kF2 = T(0), kG = T(0), kE = T(0);
+                // tile-local sums of neighbour k (folded into FP64 per tile)
+                T kF0 = T(0), kF1 = T(0), kF2 = T(0), kH = T(0), kG = T(0);
                 const int jlim = (jt == kb) ? lane : AW;  // diagonal tile: j < k
 #pragma unroll 1
                 for (int q = 0; q < AW; ++q) {
@@ -300,7 +295,7 @@ __global__ __launch_bounds__(AW) void k_atm(Params<T> P) {
                         dj6 += dyv * kw[bb];
                     }
                     const T ic6jk = c6jk > T(0) ? xrcp(c6jk) : T(0);
-                    const T C9 = -xsqrt(Jf[F_C6 * AW + jl] * kc6 * c6jk);
+                    const T C9 = -fast_sqrt(Jf[F_C6 * AW + jl] * kc6 * c6jk);
                     const T abc = a * b * c;
                     const T iR = xrsqrt(abc);
                     const T iR2 = iR * iR, iR3 = iR2 * iR, iR5 = iR3 * iR2;
@@ -321,55 +316,30 @@ __global__ __launch_bounds__(AW) void k_atm(Params<T> P) {
                     const T tij = G2 * ia + H2 * dQa;
                     const T tik = G2 * ib + H2 * dQb;
                     const T tjk = G2 * ic + H2 * dQc;
-                    const T e = C9 * A * f;
+                    // hC = e/2; the centre's share, its force, dE/dCN and the virial follow from
+                    // the per-neighbour sums after the tiles (phase 4)
                     const T hC = T(0.5) * C9 * A * f;
-                    const T e3 = e * T(1.0 / 3.0);
-                    // centre i
-                    pE += e;
-                    pG += hC * (Jf[F_DLI * AW + jl] + kdli);
-                    const T fix = tij * jx + tik * kx, fiy = tij * jy + tik * ky, fiz = tij * jz + tik * kz;
-                    pF0 += fix;
-                    pF1 += fiy;
-                    pF2 += fiz;
-                    pW0 -= tij * jx * jx + tik * kx * kx + tjk * vx * vx;
-                    pW1 -= tij * jy * jy + tik * ky * ky + tjk * vy * vy;
-                    pW2 -= tij * jz * jz + tik * kz * kz + tjk * vz * vz;
-                    pW3 -= tij * jx * jy + tik * kx * ky + tjk * vx * vy;
-                    pW4 -= tij * jx * jz + tik * kx * kz + tjk * vx * vz;
-                    pW5 -= tij * jy * jz + tik * ky * kz + tjk * vy * vz;
                     // neighbour j (LDS slot jl: distinct across lanes at each step)
                     const T fjx = -tij * jx + tjk * vx, fjy = -tij * jy + tjk * vy, fjz = -tij * jz + tjk * vz;
                     Jacc[0 * AW + jl] += fjx;
                     Jacc[1 * AW + jl] += fjy;
                     Jacc[2 * AW + jl] += fjz;
-                    Jacc[3 * AW + jl] += hC * (Jf[F_DLJ * AW + jl] + dj6 * ic6jk);
-                    Jacc[4 * AW + jl] += e3;
-                    // neighbour k (registers)
-                    kF0 -= fix + fjx;
-                    kF1 -= fiy + fjy;
-                    kF2 -= fiz + fjz;
-                    kG += hC * (kdlk + dk6 * ic6jk);
-                    kE += e3;
+                    Jacc[3 * AW + jl] += hC;
+                    Jacc[4 * AW + jl] += hC * dj6 * ic6jk;
+                    // neighbour k (registers): grad_k e = -t_ik d_k - t_jk v
+                    kF0 -= tik * kx + tjk * vx;
+                    kF1 -= tik * ky + tjk * vy;
+                    kF2 -= tik * kz + tjk * vz;
+                    kH += hC;
+                    kG += hC * dk6 * ic6jk;
                     cnt += 1.0f;
                 }
-                // fold the tile sums into the FP64 accumulators
-                Cs[0 * AW + lane] += (double)pE;
-                Cs[1 * AW + lane] += (double)pG;
-                Cs[2 * AW + lane] += (double)pF0;
-                Cs[3 * AW + lane] += (double)pF1;
-                Cs[4 * AW + lane] += (double)pF2;
-                Cs[5 * AW + lane] += (double)pW0;
-                Cs[6 * AW + lane] += (double)pW1;
-                Cs[7 * AW + lane] += (double)pW2;
-                Cs[8 * AW + lane] += (double)pW3;
-                Cs[9 * AW + lane] += (double)pW4;
-                Cs[10 * AW + lane] += (double)pW5;
                 if (kval) {
                     La[0 * cap + k] += (double)kF0;
                     La[1 * cap + k] += (double)kF1;
                     La[2 * cap + k] += (double)kF2;
-                    La[3 * cap + k] += (double)kG;
-                    La[4 * cap + k] += (double)kE;
+                    La[3 * cap + k] += (double)kH;
+                    La[4 * cap + k] += (double)kG;
                 }
                 wave_sync();
                 if (jval)
@@ -377,24 +347,49 @@ __global__ __launch_bounds__(AW) void k_atm(Params<T> P) {
             }
         }
 
-        // ---- phase 4: scatter the per-neighbour sums; the centre's own terms
+        // ---- phase 4.  Per neighbour slot p: F_p = sum of grad_p e, H_p = sum of e/2, Gc_p = sum of
+        // (e/2) dlnC6_jk/dCN_p over the triangles it belongs to.  Then
+        //   dE/dCN_p = Gc_p + (dlnC6_ip/dCN_p) H_p,  E/3 share of p = (2/3) H_p,
+        //   centre: E = sum H_p (each triangle feeds two slots), dE/dCN_i = sum (dlnC6_ip/dCN_i) H_p,
+        //   F_i = -sum F_p (translation invariance), virial = sum d_p (x) F_p
+        //   (= -sum over edges t r r^T, since sum_v r_v (x) grad_v e = sum_edges t r r^T).
+        Acc<double> aE, aG, aF[3], aW[9];
+        aE.init();
+        aG.init();
+        for (int q = 0; q < 3; ++q) aF[q].init();
+        for (int q = 0; q < 9; ++q) aW[q].init();
         for (int p = lane; p < nf; p += AW) {
             const int j = Lj[p];
-            atomicAdd(&P.fa[3 * j + 0], La[0 * cap + p]);
-            atomicAdd(&P.fa[3 * j + 1], La[1 * cap + p]);
-            atomicAdd(&P.fa[3 * j + 2], La[2 * cap + p]);
-            atomicAdd(&P.ga[j], La[3 * cap + p]);
-            atomicAdd(&P.eas[j], La[4 * cap + p]);
+            const double fx = La[0 * cap + p], fy = La[1 * cap + p], fz = La[2 * cap + p];
+            const double H = La[3 * cap + p], Gc = La[4 * cap + p];
+            const double dx = (double)LT[F_DX * cap + p], dy = (double)LT[F_DY * cap + p],
+                         dz = (double)LT[F_DZ * cap + p];
+            atomicAdd(&P.fa[3 * j + 0], fx);
+            atomicAdd(&P.fa[3 * j + 1], fy);
+            atomicAdd(&P.fa[3 * j + 2], fz);
+            atomicAdd(&P.ga[j], Gc + (double)LT[F_DLJ * cap + p] * H);
+            atomicAdd(&P.eas[j], H * (2.0 / 3.0));
+            aE.add(H);
+            aG.add((double)LT[F_DLI * cap + p] * H);
+            aF[0].add(-fx);
+            aF[1].add(-fy);
+            aF[2].add(-fz);
+            aW[0].add(dx * fx);
+            aW[1].add(dy * fy);
+            aW[2].add(dz * fz);
+            aW[3].add(dx * fy);
+            aW[4].add(dy * fx);
+            aW[5].add(dx * fz);
+            aW[6].add(dz * fx);
+            aW[7].add(dy * fz);
+            aW[8].add(dz * fy);
         }
-        double cs[11];
-        for (int q = 0; q < 11; ++q) {
-            Acc<double> a;
-            a.s = Cs[q * AW + lane];
-            cs[q] = wave_total(a);
-        }
-        const double E = cs[0], G = cs[1];
-        const double F[3] = {cs[2], cs[3], cs[4]};
-        const double W[6] = {cs[5], cs[6], cs[7], cs[8], cs[9], cs[10]};
+        const double E = wave_total(aE), G = wave_total(aG);
+        const double F[3] = {wave_total(aF[0]), wave_total(aF[1]), wave_total(aF[2])};
+        double wv[9];
+        for (int q = 0; q < 9; ++q) wv[q] = wave_total(aW[q]);
+        const double W[6] = {wv[0], wv[1], wv[2], 0.5 * (wv[3] + wv[4]), 0.5 * (wv[5] + wv[6]),
+                             0.5 * (wv[7] + wv[8])};
         cnt = wave_sumf(cnt);
         if (lane == 0) {
             atomicAdd(&P.fa[3 * i + 0], F[0]);
