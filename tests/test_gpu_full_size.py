@@ -1,0 +1,104 @@
+"""GPU checks at the benchmark workloads' full sizes (BASELINE.json configurations):
+
+* the 512-atom Si diamond cell (the oracle's own CPU case) against the FP64 oracle at the parity bar;
+* the 12,288-atom water box: translation and permutation invariance, sum of forces, stress
+  symmetry, and the FP32 path against the FP64 path (<= 1e-6 relative energy);
+* the 256-cell triclinic batch: per-cell results of the batched evaluation equal single-cell runs
+  for a sample of cells, sum of forces per cell, FP32 against FP64 per cell.
+"""
+
+import numpy as np
+import pytest
+
+import d3_lib
+import d3_oracle
+import d3_structures as S
+
+pytestmark = pytest.mark.gpu
+
+RC = dict(cutoff=float(np.sqrt(9000.0)), cn_cutoff=40.0, atm_cutoff=40.0, damping=1, functional=0)
+
+
+def handle(Z, n, precision=d3_lib.D3X_FP64, **kw):
+    args = dict(RC)
+    args.update(kw)
+    return d3_lib.Handle(np.unique(Z), n, precision=precision, **args)
+
+
+def run(pos, Z, cell, precision=d3_lib.D3X_FP64):
+    h = handle(Z, len(Z), precision)
+    h.set_atoms(pos, Z)
+    h.set_cell(cell)
+    out = h.compute()
+    h.close()
+    return out
+
+
+def test_si512_matches_oracle(table):
+    pos, Z, cell = S.si_diamond(4, jitter=0.02, seed=0)
+    print("case si512 fp64 vs oracle", flush=True)
+    e, f, st = run(pos, Z, cell)
+    o = d3_oracle.compute(pos, Z, cell, table, **RC)
+    assert abs(e - o["energy"]) <= 1e-12 * abs(o["energy"])
+    assert np.max(np.abs(f - o["forces"])) <= 1e-9
+    assert np.max(np.abs(st - o["stress"])) <= 1e-12 * np.max(np.abs(o["stress"]))
+
+
+@pytest.fixture(scope="module")
+def water():
+    return S.water_box(4096, seed=0)
+
+
+def test_water_invariances(water):
+    pos, Z, cell = water
+    print("case water-4096 invariances", flush=True)
+    e, f, st = run(pos, Z, cell)
+    fmax = np.max(np.abs(f))
+    assert np.max(np.abs(f.sum(axis=0))) <= 1e-9 * fmax * len(Z)
+    assert np.max(np.abs(st - st.T)) <= 1e-13 * np.max(np.abs(st))
+    e2, f2, st2 = run(pos + np.array([3.3, -7.1, 12.9]), Z, cell)
+    assert abs(e2 - e) <= 1e-12 * abs(e)
+    assert np.max(np.abs(f2 - f)) <= 1e-9
+    p = np.random.Generator(np.random.PCG64(7)).permutation(len(Z))
+    e3, f3, st3 = run(pos[p], Z[p], cell)
+    assert abs(e3 - e) <= 1e-12 * abs(e)
+    assert np.max(np.abs(f3 - f[p])) <= 1e-9
+    assert np.max(np.abs(st3 - st)) <= 1e-12 * np.max(np.abs(st))
+
+
+def test_water_fp32_error(water):
+    pos, Z, cell = water
+    print("case water-4096 fp32 vs fp64", flush=True)
+    e64, f64, s64 = run(pos, Z, cell)
+    e32, f32, s32 = run(pos, Z, cell, d3_lib.D3X_FP32)
+    rel = abs(e32 - e64) / abs(e64)
+    print(f"  water fp32: rel dE {rel:.2e}, max dF {np.max(np.abs(f32 - f64)):.2e} eV/A, "
+          f"max dS {np.max(np.abs(s32 - s64)):.2e}")
+    assert rel <= 1e-6
+    assert np.max(np.abs(f32 - f64)) <= 1e-5 * np.max(np.abs(f64))
+    assert np.max(np.abs(s32 - s64)) <= 1e-5 * np.max(np.abs(s64))
+
+
+def test_batch256_equals_single_cells():
+    counts, pos, Z, cells = S.triclinic_batch(256, seed=0)
+    print(f"case batch-256 atoms={len(Z)}", flush=True)
+    n = int(counts.sum())
+    h = handle(Z, n)
+    eb, fb, sb = h.compute_batch(counts, pos, Z, cells)
+    h.close()
+    off = np.concatenate([[0], np.cumsum(counts)]).astype(int)
+    for s in range(len(counts)):
+        fs = fb[off[s]:off[s + 1]]
+        assert np.max(np.abs(fs.sum(axis=0))) <= 1e-8 * max(1.0, np.max(np.abs(fs))) * counts[s]
+    for s in (0, 1, 17, 100, int(np.argmax(counts)), int(np.argmin(counts))):
+        sl = slice(off[s], off[s + 1])
+        e, f, st = run(pos[sl], Z[sl], cells[s])
+        assert abs(e - eb[s]) <= 1e-12 * abs(e)
+        assert np.max(np.abs(f - fb[sl])) <= 1e-10
+        assert np.max(np.abs(st - sb[s])) <= 1e-12 * np.max(np.abs(st))
+    h32 = handle(Z, n, d3_lib.D3X_FP32)
+    e32, _, _ = h32.compute_batch(counts, pos, Z, cells)
+    h32.close()
+    rel = np.max(np.abs(e32 - eb) / np.abs(eb))
+    print(f"  batch fp32: max per-cell rel dE {rel:.2e}")
+    assert rel <= 1e-6
