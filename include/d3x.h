@@ -68,6 +68,13 @@ int d3x_get_counts(d3_handle_t *handle, double *counts);
 int d3x_set_timing(d3_handle_t *handle, int enable);
 int d3x_phase_times(d3_handle_t *handle, double ms[6]);
 
+/* hipGraph mode (enable != 0): the first evaluation captures the whole launch sequence (bin
+ * sort, CN, two-body, three-body, chain rule, reductions) into a graph on the handle's stream and
+ * later evaluations replay it; any change of atom count, cell geometry, parameters or scratch
+ * sizes re-captures.  D3_GRAPH=1 in the environment enables it for handles made by the
+ * reference-compatible init functions. */
+int d3x_set_graph(d3_handle_t *handle, int enable);
+
 /* The HIP stream (hipStream_t) the handle launches on; d3x_set_stream replaces it (NULL =
  * the handle's own stream). */
 void *d3x_get_stream(d3_handle_t *handle);

@@ -87,6 +87,13 @@ d3_handle_t *make_handle(const uint16_t *elements, uint64_t ne, uint64_t max_len
     return st ? nullptr : h;
 }
 
+// D3_GRAPH=1: reference-compatible handles evaluate through a captured hipGraph.
+d3_handle_t *with_env_graph(d3_handle_t *h) {
+    const char *g = std::getenv("D3_GRAPH");
+    if (h && g && std::strcmp(g, "1") == 0) H(h)->eng->set_graph(true);
+    return h;
+}
+
 }  // namespace
 
 extern "C" {
@@ -94,8 +101,8 @@ extern "C" {
 d3_handle_t *init_d3_handle(uint16_t *elements, uint64_t num_elements, uint64_t max_length, real_t cutoff_radius,
                             real_t coordination_number_cutoff, damping_type_t damping_type,
                             functional_t functional_type) {
-    return make_handle(elements, num_elements, max_length, cutoff_radius, coordination_number_cutoff, 0.0,
-                       (int)damping_type, (int)functional_type, default_precision());
+    return with_env_graph(make_handle(elements, num_elements, max_length, cutoff_radius, coordination_number_cutoff,
+                                      0.0, (int)damping_type, (int)functional_type, default_precision()));
 }
 
 d3_handle_t *init_d3_handle_with_atm(uint16_t *elements, uint64_t length_elements, uint64_t max_length,
@@ -105,8 +112,9 @@ d3_handle_t *init_d3_handle_with_atm(uint16_t *elements, uint64_t length_element
         report("init_d3_handle_with_atm", "atm_cutoff must be positive");
         return nullptr;
     }
-    return make_handle(elements, length_elements, max_length, cutoff_radius, coordination_number_cutoff, atm_cutoff,
-                       (int)damping_type, (int)functional_type, default_precision());
+    return with_env_graph(make_handle(elements, length_elements, max_length, cutoff_radius,
+                                      coordination_number_cutoff, atm_cutoff, (int)damping_type, (int)functional_type,
+                                      default_precision()));
 }
 
 void set_atoms(d3_handle_t *handle, real_t *coords, uint16_t *elements, uint64_t length) {
@@ -253,6 +261,11 @@ int d3x_phase_times(d3_handle_t *handle, double ms[6]) {
     int n = -1;
     uint32_t st = guarded(H(handle), "d3x_phase_times", [&] { n = H(handle)->eng->phase_times(ms); });
     return st ? -1 : n;
+}
+
+int d3x_set_graph(d3_handle_t *handle, int enable) {
+    if (!handle) return (int)(D3X_ERR_GENERIC | D3X_ERR_INPUT);
+    return (int)guarded(H(handle), "d3x_set_graph", [&] { H(handle)->eng->set_graph(enable != 0); });
 }
 
 void *d3x_get_stream(d3_handle_t *handle) { return handle ? (void *)H(handle)->eng->stream() : nullptr; }

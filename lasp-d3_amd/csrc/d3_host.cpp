@@ -201,6 +201,7 @@ Engine::Engine(const uint16_t *elements, uint64_t num_elements, uint64_t max_len
 
 Engine::~Engine() {
     if (own_stream_) (void)hipStreamSynchronize(own_stream_);
+    if (gexec_) (void)hipGraphExecDestroy(gexec_);
     void **all[] = {&d_pos_in, &d_type_in, &d_sys_in, &d_key, &d_perm, &d_perm2, &d_wrapped, &d_lx, &d_ly, &d_lz,
                     &d_type_s, &d_orig_s, &d_bin_s, &d_sys_s, &d_cn, &d_w, &d_dw, &d_e, &d_g, &d_f, &d_vir,
                     &d_npair, &d_out_f, &d_out_e, &d_out_cn, &d_out_g, &d_bin_count, &d_bin_start, &d_cursor,
@@ -496,7 +497,8 @@ void Engine::ensure_atm_scratch() {
 
 template <class T>
 Params<T> Engine::params() {
-    Params<T> P{};
+    Params<T> P;
+    std::memset(&P, 0, sizeof(P));  // padding too: the graph signature compares the raw bytes
     P.n = (int)n_;
     P.ne = ne_;
     P.nref = nref_;
@@ -598,7 +600,36 @@ void Engine::run() {
     }
     ensure_atm_scratch();
     Params<T> P = params<T>();
+    if (graph_mode_ && !timing_) {
+        // The graph holds every kernel argument and launch shape; re-capture when any changes.
+        std::vector<unsigned char> sig(sizeof(P) + 6 * sizeof(int64_t));
+        std::memcpy(sig.data(), &P, sizeof(P));
+        const int64_t extra[6] = {nbins_, atm_blocks_, (int64_t)chunk_b_.size(), (int64_t)sysdev_.size(),
+                                  (int64_t)(intptr_t)stream_, (int64_t)(intptr_t)d_tab};
+        std::memcpy(sig.data() + sizeof(P), extra, sizeof(extra));
+        if (!gexec_ || sig != gsig_) {
+            if (gexec_) (void)hipGraphExecDestroy(gexec_);
+            gexec_ = nullptr;
+            hipGraph_t g = nullptr;
+            HIPCHK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+            enqueue<T>(P);
+            HIPCHK(hipStreamEndCapture(stream_, &g));
+            const hipError_t ie = hipGraphInstantiate(&gexec_, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            HIPCHK(ie);
+            gsig_ = std::move(sig);
+            ++graph_captures_;
+        }
+        HIPCHK(hipGraphLaunch(gexec_, stream_));
+        return;
+    }
     if (timing_) harvest();
+    enqueue<T>(P);
+    if (timing_) timing_pending_ = true;
+}
+
+template <class T>
+void Engine::enqueue(const Params<T> &P) {
     launch_fill_u32((unsigned *)d_status, 1, 0u, stream_);
     mark(0);
     Launch<T>::bin_sort(P, nbins_, (int *)d_bin_start, stream_);
@@ -616,8 +647,17 @@ void Engine::run() {
                   (int)sysdev_.size(), (double *)d_partial, (double *)d_out_sys, stream_);
     launch_finalize(P, (double *)d_out_f, (double *)d_out_e, (double *)d_out_cn, (double *)d_out_g, stream_);
     mark(6);
-    if (timing_) timing_pending_ = true;
     HIPCHK(hipGetLastError());
+}
+
+void Engine::set_graph(bool on) {
+    graph_mode_ = on;
+    if (!on && gexec_) {
+        (void)hipStreamSynchronize(stream_);
+        (void)hipGraphExecDestroy(gexec_);
+        gexec_ = nullptr;
+        gsig_.clear();
+    }
 }
 
 void Engine::mark(int k) {

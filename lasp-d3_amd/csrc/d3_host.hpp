@@ -66,6 +66,11 @@ class Engine {
     void set_timing(bool on);
     int phase_times(double ms[NPHASE]);
 
+    // hipGraph mode: the first evaluation captures the whole launch sequence into a graph, later
+    // ones replay it; a change of any kernel argument or launch shape re-captures.
+    void set_graph(bool on);
+    int graph_captures() const { return graph_captures_; }
+
    private:
     struct Cell {
         double H[9] = {0};  // bohr
@@ -76,6 +81,8 @@ class Engine {
     Params<T> params();
     template <class T>
     void run();
+    template <class T>
+    void enqueue(const Params<T> &P);
     void build_geometry();
     void upload_geometry();
     void alloc_atoms();
@@ -129,6 +136,11 @@ class Engine {
     int atm_blocks_ = 0, atm_cap_ = 0;
     size_t atm_alloc_entries_ = 0;  // allocated scratch entries (>= atm_blocks_ * atm_cap_)
     int atm_alloc_blocks_ = 0;      // blocks the scratch was allocated for (>= atm_blocks_)
+
+    bool graph_mode_ = false;
+    hipGraphExec_t gexec_ = nullptr;
+    std::vector<unsigned char> gsig_;
+    int graph_captures_ = 0;
 
     bool timing_ = false, timing_pending_ = false;
     hipEvent_t ev_[NPHASE + 1] = {};

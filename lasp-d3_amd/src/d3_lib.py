@@ -63,6 +63,7 @@ SIGNATURES = {
     "d3x_get_counts": (C.c_int, [_vp, _dp]),
     "d3x_set_timing": (C.c_int, [_vp, C.c_int]),
     "d3x_phase_times": (C.c_int, [_vp, _dp]),
+    "d3x_set_graph": (C.c_int, [_vp, C.c_int]),
     "d3x_get_stream": (_vp, [_vp]),
     "d3x_set_stream": (C.c_int, [_vp, _vp]),
     "d3x_compute_batch_f64": (C.c_int, [_vp, _u64, _u64p, _dp, _u16p, _dp, _dp, _dp, _dp]),
@@ -209,6 +210,9 @@ class Handle:
         c = np.zeros((nsys, 3))
         self._check(self.lib.d3x_get_counts(self.h, ptr(c, C.c_double)), "d3x_get_counts")
         return c
+
+    def set_graph(self, on: bool):
+        self._check(self.lib.d3x_set_graph(self.h, int(on)), "d3x_set_graph")
 
     def set_timing(self, on: bool):
         self._check(self.lib.d3x_set_timing(self.h, int(on)), "d3x_set_timing")
