@@ -75,6 +75,18 @@ int d3x_phase_times(d3_handle_t *handle, double ms[6]);
  * reference-compatible init functions. */
 int d3x_set_graph(d3_handle_t *handle, int enable);
 
+/* Tuning parameters: bin width of the cell lists (bohr; default half the CN cutoff) and the
+ * number of persistent blocks of the three-body kernel (default 4096).  Values <= 0 keep the
+ * current setting.  d3x_autotune times full evaluations of the handle's current input over a
+ * small candidate set (atoms and cell must be set), keeps the fastest, stores it in the JSON cache
+ * at cache_path (NULL: $D3_TUNE_CACHE, if set) keyed by precision, cutoffs, density bucket and
+ * element count, and returns the best time in ms (negative on error).  Handles consult
+ * $D3_TUNE_CACHE for their input's key unless tuning was set explicitly.  Results do not depend
+ * on these parameters beyond summation order. */
+int d3x_set_tuning(d3_handle_t *handle, double bin_width_bohr, int atm_blocks);
+int d3x_get_tuning(d3_handle_t *handle, double *bin_width_bohr, int *atm_blocks);
+double d3x_autotune(d3_handle_t *handle, const char *cache_path);
+
 /* The HIP stream (hipStream_t) the handle launches on; d3x_set_stream replaces it (NULL =
  * the handle's own stream). */
 void *d3x_get_stream(d3_handle_t *handle);

@@ -263,6 +263,27 @@ int d3x_phase_times(d3_handle_t *handle, double ms[6]) {
     return st ? -1 : n;
 }
 
+int d3x_set_tuning(d3_handle_t *handle, double bin_width_bohr, int atm_blocks) {
+    if (!handle) return (int)(D3X_ERR_GENERIC | D3X_ERR_INPUT);
+    return (int)guarded(H(handle), "d3x_set_tuning",
+                        [&] { H(handle)->eng->set_tuning(bin_width_bohr, atm_blocks); });
+}
+
+int d3x_get_tuning(d3_handle_t *handle, double *bin_width_bohr, int *atm_blocks) {
+    if (!handle) return (int)(D3X_ERR_GENERIC | D3X_ERR_INPUT);
+    if (bin_width_bohr) *bin_width_bohr = H(handle)->eng->bin_width();
+    if (atm_blocks) *atm_blocks = H(handle)->eng->atm_blocks_request() > 0 ? H(handle)->eng->atm_blocks_request() : 4096;
+    return 0;
+}
+
+double d3x_autotune(d3_handle_t *handle, const char *cache_path) {
+    if (!handle) return -1.0;
+    double best = -1.0;
+    uint32_t st = guarded(H(handle), "d3x_autotune",
+                          [&] { best = H(handle)->eng->autotune(cache_path ? cache_path : ""); });
+    return st ? -1.0 : best;
+}
+
 int d3x_set_graph(d3_handle_t *handle, int enable) {
     if (!handle) return (int)(D3X_ERR_GENERIC | D3X_ERR_INPUT);
     return (int)guarded(H(handle), "d3x_set_graph", [&] { H(handle)->eng->set_graph(enable != 0); });

@@ -191,6 +191,7 @@ Engine::Engine(const uint16_t *elements, uint64_t num_elements, uint64_t max_len
     // Bins about half the CN cutoff wide: the CN / chain-rule stencils then skip most corner
     // bins, and the two-body stencil's bins stay small against its cutoff.
     bin_width_ = std::max(4.0, 0.5 * std::min(rccn_, rc2_));
+    if (const char *tc = std::getenv("D3_TUNE_CACHE")) tune_cache_ = tc;
     sys_.resize(1);
     HIPCHK(hipStreamCreate(&own_stream_));
     stream_ = own_stream_;
@@ -650,6 +651,148 @@ void Engine::enqueue(const Params<T> &P) {
     HIPCHK(hipGetLastError());
 }
 
+// ------------------------------------------------------------------------------------ tuning
+
+std::string Engine::tune_key() const {
+    // precision | cutoffs (0.01 bohr) | density bucket (quarter octaves of atoms/bohr^3) | elements
+    double dens = 0;
+    for (const Cell &c : sys_)
+        if (c.set) dens = std::max(dens, (double)c.n / std::fabs(det3(c.H)));
+    const int db = dens > 0 ? (int)std::lround(4.0 * std::log2(dens)) : -999;
+    char buf[160];
+    std::snprintf(buf, sizeof(buf), "%s|rc%.2f|cn%.2f|atm%.2f|d%d|ne%d|cells%s",
+                  precision_ == D3X_FP64 ? "fp64" : "fp32", rc2_, rccn_, rc3_, db, ne_, sys_.size() > 1 ? "N" : "1");
+    return buf;
+}
+
+namespace {
+// Cache file: one JSON object, one entry per line:  "key": {"bin_width": w, "atm_blocks": b, "ms": t}
+bool cache_lookup(const std::string &path, const std::string &key, double &w, int &b) {
+    FILE *f = std::fopen(path.c_str(), "r");
+    if (!f) return false;
+    char line[512];
+    bool found = false;
+    const std::string pat = "\"" + key + "\"";
+    while (std::fgets(line, sizeof(line), f)) {
+        std::string s(line);
+        if (s.find(pat) == std::string::npos) continue;
+        const size_t pw = s.find("\"bin_width\":"), pb = s.find("\"atm_blocks\":");
+        if (pw == std::string::npos || pb == std::string::npos) continue;
+        w = std::atof(s.c_str() + pw + 12);
+        b = std::atoi(s.c_str() + pb + 13);
+        found = w > 0 && b > 0;
+    }
+    std::fclose(f);
+    return found;
+}
+void cache_store(const std::string &path, const std::string &key, double w, int b, double ms) {
+    std::vector<std::string> keep;
+    if (FILE *f = std::fopen(path.c_str(), "r")) {
+        char line[512];
+        while (std::fgets(line, sizeof(line), f)) {
+            std::string s(line);
+            if (s.find("\"bin_width\"") == std::string::npos) continue;
+            if (s.find("\"" + key + "\"") != std::string::npos) continue;
+            while (!s.empty() && (s.back() == '\n' || s.back() == ',' || s.back() == ' ')) s.pop_back();
+            keep.push_back(s);
+        }
+        std::fclose(f);
+    }
+    char buf[512];
+    std::snprintf(buf, sizeof(buf), "  \"%s\": {\"bin_width\": %.6f, \"atm_blocks\": %d, \"ms\": %.4f}", key.c_str(), w,
+                  b, ms);
+    keep.push_back(buf);
+    const std::string tmp = path + ".tmp";
+    FILE *f = std::fopen(tmp.c_str(), "w");
+    if (!f) return;
+    std::fprintf(f, "{\n");
+    for (size_t k = 0; k < keep.size(); ++k) std::fprintf(f, "%s%s\n", keep[k].c_str(), k + 1 < keep.size() ? "," : "");
+    std::fprintf(f, "}\n");
+    std::fclose(f);
+    std::rename(tmp.c_str(), path.c_str());
+}
+}  // namespace
+
+void Engine::set_tuning(double bin_width, int atm_blocks) {
+    if (bin_width > 0) {
+        if (!std::isfinite(bin_width)) fail(D3X_ERR_INPUT, "bin width must be finite");
+        bin_width_ = bin_width;
+        geom_dirty_ = true;
+    }
+    if (atm_blocks > 0) atm_blocks_req_ = atm_blocks;
+    tuning_explicit_ = true;
+}
+
+void Engine::apply_tune_cache() {
+    if (tune_cache_.empty()) return;
+    double w;
+    int b;
+    if (cache_lookup(tune_cache_, tune_key(), w, b)) {
+        bin_width_ = w;
+        atm_blocks_req_ = b;
+    }
+}
+
+double Engine::timed_evaluation() {
+    compute_device();  // builds geometry, sizes scratch
+    set_timing(true);
+    compute_device();
+    double ms[NPHASE];
+    phase_times(ms);
+    set_timing(false);
+    unsigned status = 0;
+    HIPCHK(hipMemcpy(&status, d_status, sizeof(unsigned), hipMemcpyDeviceToHost));
+    if (status & 2u) {  // three-body capacity too small for this input: grow and time again
+        grow_atm_cap();
+        return timed_evaluation();
+    }
+    double t = 0;
+    for (double m : ms) t += m;
+    return t;
+}
+
+double Engine::autotune(const std::string &cache_path) {
+    if (!atoms_set_) fail(D3X_ERR_STATE, "set_atoms has not been called");
+    const bool graph = graph_mode_;
+    set_graph(false);
+    const double base = 0.5 * std::min(rccn_, rc2_);
+    std::vector<double> widths;
+    for (double fct : {1.0, 0.75, 0.6, 1.5, 0.5})
+        if (base * fct >= 4.0) widths.push_back(base * fct);
+    double best_w = bin_width_, best = 1e300;
+    for (double w : widths) {
+        bin_width_ = w;
+        geom_dirty_ = true;
+        const double t = timed_evaluation();
+        if (t < best) {
+            best = t;
+            best_w = w;
+        }
+    }
+    bin_width_ = best_w;
+    geom_dirty_ = true;
+    int best_b = atm_blocks_req_ > 0 ? atm_blocks_req_ : 4096;
+    if (rc3_ > 0) {
+        best = 1e300;
+        for (int b : {2048, 4096, 8192, 16384}) {
+            atm_blocks_req_ = b;
+            const double t = timed_evaluation();
+            if (t < best) {
+                best = t;
+                best_b = b;
+            }
+        }
+    } else {
+        best = timed_evaluation();
+    }
+    atm_blocks_req_ = best_b;
+    tuning_explicit_ = true;
+    const std::string path = cache_path.empty() ? tune_cache_ : cache_path;
+    if (!path.empty()) cache_store(path, tune_key(), best_w, best_b, best);
+    set_graph(graph);
+    return best;
+}
+
 void Engine::set_graph(bool on) {
     graph_mode_ = on;
     if (!on && gexec_) {
@@ -696,6 +839,7 @@ void Engine::compute_device() {
     for (const Cell &c : sys_)
         if (!c.set) fail(D3X_ERR_STATE, "set_cell has not been called");
     if (geom_dirty_) {
+        if (!tuning_explicit_) apply_tune_cache();
         build_geometry();
         upload_geometry();
         geom_dirty_ = false;

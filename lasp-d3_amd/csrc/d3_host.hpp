@@ -71,6 +71,15 @@ class Engine {
     void set_graph(bool on);
     int graph_captures() const { return graph_captures_; }
 
+    // Tuning parameters: bin width (bohr) and the three-body kernel's persistent grid size.
+    // autotune() times full evaluations of the current input over a small candidate set, keeps
+    // the fastest, and stores it in a JSON cache keyed by tune_key(); the cache is consulted
+    // when geometry is built for an input whose key has an entry (unless set explicitly).
+    void set_tuning(double bin_width, int atm_blocks);
+    double autotune(const std::string &cache_path);
+    std::string tune_key() const;
+    int atm_blocks_request() const { return atm_blocks_req_; }
+
    private:
     struct Cell {
         double H[9] = {0};  // bohr
@@ -136,6 +145,11 @@ class Engine {
     int atm_blocks_ = 0, atm_cap_ = 0;
     size_t atm_alloc_entries_ = 0;  // allocated scratch entries (>= atm_blocks_ * atm_cap_)
     int atm_alloc_blocks_ = 0;      // blocks the scratch was allocated for (>= atm_blocks_)
+
+    bool tuning_explicit_ = false;  // set_tuning / autotune fixed the parameters
+    std::string tune_cache_;        // cache consulted at geometry build (empty: none)
+    void apply_tune_cache();
+    double timed_evaluation();
 
     bool graph_mode_ = false;
     hipGraphExec_t gexec_ = nullptr;

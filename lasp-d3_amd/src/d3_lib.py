@@ -64,6 +64,9 @@ SIGNATURES = {
     "d3x_set_timing": (C.c_int, [_vp, C.c_int]),
     "d3x_phase_times": (C.c_int, [_vp, _dp]),
     "d3x_set_graph": (C.c_int, [_vp, C.c_int]),
+    "d3x_set_tuning": (C.c_int, [_vp, C.c_double, C.c_int]),
+    "d3x_get_tuning": (C.c_int, [_vp, _dp, _ip]),
+    "d3x_autotune": (C.c_double, [_vp, C.c_char_p]),
     "d3x_get_stream": (_vp, [_vp]),
     "d3x_set_stream": (C.c_int, [_vp, _vp]),
     "d3x_compute_batch_f64": (C.c_int, [_vp, _u64, _u64p, _dp, _u16p, _dp, _dp, _dp, _dp]),
@@ -210,6 +213,20 @@ class Handle:
         c = np.zeros((nsys, 3))
         self._check(self.lib.d3x_get_counts(self.h, ptr(c, C.c_double)), "d3x_get_counts")
         return c
+
+    def set_tuning(self, bin_width: float = 0.0, atm_blocks: int = 0):
+        self._check(self.lib.d3x_set_tuning(self.h, float(bin_width), int(atm_blocks)), "d3x_set_tuning")
+
+    def tuning(self):
+        w, b = C.c_double(), C.c_int()
+        self._check(self.lib.d3x_get_tuning(self.h, C.byref(w), C.byref(b)), "d3x_get_tuning")
+        return w.value, b.value
+
+    def autotune(self, cache_path: str | None = None) -> float:
+        t = self.lib.d3x_autotune(self.h, cache_path.encode() if cache_path else None)
+        if t < 0:
+            raise D3Error("d3x_autotune", -1)
+        return t
 
     def set_graph(self, on: bool):
         self._check(self.lib.d3x_set_graph(self.h, int(on)), "d3x_set_graph")

@@ -1,0 +1,71 @@
+"""Tuning parameters change only the summation order: results for several bin widths and
+three-body grid sizes agree to the parity bar; the autotuner runs and its cache round-trips."""
+
+import json
+import os
+
+import numpy as np
+import pytest
+
+import d3_lib
+import systems
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("precision", [d3_lib.D3X_FP64, d3_lib.D3X_FP32])
+def test_results_invariant_under_tuning(precision):
+    s = systems.si64(seed=31)
+    ref = None
+    for w, b in ((None, None), (5.0, 1), (9.5, 7), (14.0, 64), (41.0, 4096), (60.0, 300)):
+        h = d3_lib.Handle(s["Z"], len(s["Z"]), cutoff=s["cutoff"], cn_cutoff=s["cn_cutoff"], atm_cutoff=s["atm"],
+                          damping=1, functional=0, precision=precision)
+        if w is not None:
+            h.set_tuning(w, b)
+        h.set_atoms(s["pos"], s["Z"])
+        h.set_cell(s["cell"])
+        out = h.compute()
+        h.close()
+        if ref is None:
+            ref = out
+            continue
+        # FP64: the parity bar; FP32: its own error budget (forces in this crystal nearly cancel,
+        # so they get an absolute floor of 1e-6 eV/A, the size of FP32 force errors)
+        fp64 = precision == d3_lib.D3X_FP64
+        tol = 1e-12 if fp64 else 2e-6
+        assert abs(out[0] - ref[0]) <= tol * abs(ref[0]), (w, b)
+        assert np.max(np.abs(out[1] - ref[1])) <= (1e-9 if fp64 else 1e-6), (w, b)
+        assert np.max(np.abs(out[2] - ref[2])) <= tol * 10 * np.max(np.abs(ref[2])) + (0 if fp64 else 1e-9), (w, b)
+
+
+def test_autotune_and_cache(tmp_path):
+    s = systems.triclinic_mixed(n=24, seed=8)
+    cache = str(tmp_path / "tune.json")
+    h = d3_lib.Handle(s["Z"], len(s["Z"]), cutoff=s["cutoff"], cn_cutoff=s["cn_cutoff"], atm_cutoff=s["atm"],
+                      damping=1, functional=2)
+    h.set_atoms(s["pos"], s["Z"])
+    h.set_cell(s["cell"])
+    e0, f0, _ = h.compute()
+    t = h.autotune(cache)
+    w, b = h.tuning()
+    assert t > 0 and w > 0 and b > 0
+    e1, f1, _ = h.compute()
+    assert abs(e1 - e0) <= 1e-12 * abs(e0)
+    h.close()
+    data = json.load(open(cache))
+    assert len(data) == 1
+    (key, entry), = data.items()
+    assert entry["bin_width"] == pytest.approx(w) and entry["atm_blocks"] == b
+    # a new handle with the cache in the environment picks the tuned values for the same input
+    os.environ["D3_TUNE_CACHE"] = cache
+    try:
+        h2 = d3_lib.Handle(s["Z"], len(s["Z"]), cutoff=s["cutoff"], cn_cutoff=s["cn_cutoff"],
+                           atm_cutoff=s["atm"], damping=1, functional=2)
+        h2.set_atoms(s["pos"], s["Z"])
+        h2.set_cell(s["cell"])
+        e2, _, _ = h2.compute()
+        assert h2.tuning() == (pytest.approx(w), b)
+        assert abs(e2 - e0) <= 1e-12 * abs(e0)
+        h2.close()
+    finally:
+        del os.environ["D3_TUNE_CACHE"]
