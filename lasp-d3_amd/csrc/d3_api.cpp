@@ -272,7 +272,7 @@ int d3x_set_tuning(d3_handle_t *handle, double bin_width_bohr, int atm_blocks) {
 int d3x_get_tuning(d3_handle_t *handle, double *bin_width_bohr, int *atm_blocks) {
     if (!handle) return (int)(D3X_ERR_GENERIC | D3X_ERR_INPUT);
     if (bin_width_bohr) *bin_width_bohr = H(handle)->eng->bin_width();
-    if (atm_blocks) *atm_blocks = H(handle)->eng->atm_blocks_request() > 0 ? H(handle)->eng->atm_blocks_request() : 4096;
+    if (atm_blocks) *atm_blocks = H(handle)->eng->atm_blocks_effective();
     return 0;
 }
 

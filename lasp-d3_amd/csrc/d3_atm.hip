@@ -8,26 +8,28 @@
 // forward neighbours of its anchor with r_jk <= rc3.  (The reference enumerates i >= j >= k over
 // every image and rescales index-degenerate triplets by 1/2 or 1/6.)
 //
-// Work layout.  One wave per centre (persistent blocks of 64 lanes).  Phase 1 compacts the
-// forward neighbours (ballot/mbcnt) into per-block scratch; phase 2 adds C6_ij and its two
-// logarithmic CN derivatives; phase 3 visits the pairs in 64 x 64 tiles: lane l owns neighbour
-// k = 64 kb + l in registers, the j tile sits in LDS together with its C6 rows
-// y_j[B] = C6ref_{A_j B}^T w_j and dy_j[B] = C6ref^T dw_j, and at step q lane l takes
-// j = (l + q) mod 64, so no two lanes touch the same j accumulator in the same step.  Tile pairs
-// whose bounding boxes are farther apart than rc3 are skipped.  Phase 4 adds the per-neighbour
-// sums to the per-atom arrays with FP64 atomics.
+// Work layout.  One wave per centre; persistent one-wave blocks take centres from an atomic
+// counter.  Phase 1 compacts the forward neighbours (ballot/mbcnt) into per-block scratch;
+// phase 2 turns them into records (C6_ij and its logarithmic CN derivatives, 1/a, a^-1/2,
+// sqrt C6_ij, damping term); phase 3 visits the pairs in 64 x 64 tiles: lane l owns neighbour
+// k = 64 kb + l in registers, the j tile sits in LDS with its C6 rows y_j[B] = C6ref_{A_j B}^T w_j
+// and dy_j[B] = C6ref^T dw_j, and at step q lane l takes j = (l + q) mod 64.  Tile pairs whose
+// bounding boxes are farther apart than rc3 are skipped.  Phase 4 turns the per-neighbour sums
+// into forces, dE/dCN, energies and the virial and adds them to the per-atom arrays with FP64
+// atomics.
 //
-// Per triangle (a = r_ij^2, b = r_ik^2, c = r_jk^2, d_j = r_i - r_j, d_k = r_i - r_k, v = r_j - r_k):
+// Per triangle (a = r_ij^2, b = r_ik^2, c = r_jk^2, d_j = r_i - r_j, d_k = r_i - r_k):
 //   e = C9 A f,  C9 = -sqrt(C6_ij C6_ik C6_jk)
 //   A = (abc)^(-3/2) + (3/8) Q (abc)^(-5/2),  Q = (a+b-c)(a-b+c)(-a+b+c)   [= (1 + 3 cos cos cos)/(r r r)^3]
-//   f = 1 / (1 + 6 x^16),  x^6 = y = (4/3)^6 R0_ij^2 R0_ik^2 R0_jk^2 / (abc),  x^16 = y^3 y^(-1/3)
-//   t_xy = (1/r_xy) de/dr_xy = 2 de/d(r_xy^2)
-//        = 2 C9 [(P1 f + A fx) / (r_xy^2) + (3/8) iR5 f dQ/d(r_xy^2)],
-//     P1 = -(3/2)(abc)^(-3/2) - (15/16) Q (abc)^(-5/2),  fx = 16 x^16 f^2
-//   grad_i e = t_ij d_j + t_ik d_k,  grad_j e = -t_ij d_j + t_jk v,  grad_k e = -(grad_i + grad_j)
+//   f = 1 / (1 + 6 x^16),  x^6 = y = (4/3)^6 R0_ij^2 R0_ik^2 R0_jk^2 / (abc),  x^16 = y^(8/3)
+//   t_xy = 2 de/d(r_xy^2) = 2 C9 f [(P1 + (8/3) A (1 - f)) / r_xy^2 + (3/8) iR5 dQ/d(r_xy^2)],
+//     P1 = -(3/2)(abc)^(-3/2) - (15/16) Q (abc)^(-5/2),  (6 x^16 f = 1 - f)
+//   grad_j e = -(t_ij + t_jk) d_j + t_jk d_k,  grad_k e = -(t_ik + t_jk) d_k + t_jk d_j,
+//   grad_i e = -(grad_j e + grad_k e)
 //   de/dCN_x = (C9/2)(dlnC6_xy/dCN_x + dlnC6_xz/dCN_x) A f
-//   virial += -(t_ij d_j d_j^T + t_ik d_k d_k^T + t_jk v v^T)
 //   each vertex's atom takes e/3 of the energy (the reference's split, src/d3_kernel.cu:1394-1399)
+// FP32 evaluates x^16 as exp2((8/3) log2 y) with log2 y split into per-neighbour, per-type and
+// per-pair parts; FP64 as y^3 y^(-1/3).
 #include <hip/hip_runtime.h>
 
 #include "d3_device.hpp"
@@ -37,24 +39,33 @@ namespace {
 
 constexpr int AW = 64;                          // one wave per block
 constexpr double K6 = 4096.0 / 729.0;           // (4/3)^6
-enum { F_DX, F_DY, F_DZ, F_R2, F_IR2, F_C6, F_DLI, F_DLJ, F_R0SQ };  // record / LDS fields
+constexpr double L2K6_83 = 6.6405999884615;     // (8/3) log2((4/3)^6)
+constexpr float FAR = 1e18f;                    // coordinate of padding lanes (fails every cutoff)
+// record fields (global scratch, one array of cap entries per field)
+enum { F_DX, F_DY, F_DZ, F_R2, F_IR2, F_RA, F_SC6, F_LAM, F_DLI, F_DLJ };
+// per-neighbour FP64 accumulators: A_p = sum (t_ip + t_pq), B_p = sum t_pq d_q, H_p = sum e/2,
+// Gc_p = sum (e/2) dlnC6_pq/dCN_p over the triangles {i, p, q} of the centre
+enum { A_A, A_BX, A_BY, A_BZ, A_H, A_G, A_NF };
+static_assert(A_NF == 6, "atm_acc_stride holds 6 doubles per neighbour slot");
 
-// y^(-1/3): FP32 through the hardware exp2/log2; FP64 from the FP32 value and two Newton steps
-// z <- z (4 - y z^3) / 3 (quadratic convergence, no division).
-__device__ __forceinline__ float inv_cbrt(float y) {
-    return __builtin_amdgcn_exp2f(-(1.0f / 3.0f) * __builtin_amdgcn_logf(y));
-}
+template <class T>
+struct alignas(2 * sizeof(T)) T2 {
+    T x, y;
+};
+
+// y^(-1/3) in FP64: float guess and two Newton steps z <- z (4 - y z^3) / 3.
 __device__ __forceinline__ double inv_cbrt(double y) {
-    double z = (double)inv_cbrt((float)y);
+    double z = (double)__builtin_amdgcn_exp2f(-(1.0f / 3.0f) * __builtin_amdgcn_logf((float)y));
     z = z * (4.0 - y * z * z * z) * (1.0 / 3.0);
     z = z * (4.0 - y * z * z * z) * (1.0 / 3.0);
     return z;
 }
-__device__ __forceinline__ float xrcp(float x) { return __builtin_amdgcn_rcpf(x); }
-__device__ __forceinline__ double xrcp(double x) { return 1.0 / x; }
-// FP32: the hardware square root (1 ulp) without the correctly-rounded fix-up sequence.
-__device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
-__device__ __forceinline__ double fast_sqrt(double x) { return sqrt(x); }
+// Per-pair reciprocal and reciprocal square root: the FP32 hardware instructions (1 ulp),
+// correctly rounded FP64.
+__device__ __forceinline__ float prcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double prcp(double x) { return 1.0 / x; }
+__device__ __forceinline__ float prsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ double prsqrt(double x) { return 1.0 / sqrt(x); }
 
 // A wave-uniform float, moved to a scalar register.
 __device__ __forceinline__ float uni(float v) {
@@ -72,48 +83,75 @@ __device__ __forceinline__ float wmax(float v) {
     return v;
 }
 
+// LDS carve-up, shared by the kernel and the host-side size query.
+template <class T>
+struct AtmLds {
+    T2<T> *J0, *J1, *J2, *J3, *Jyd;  // j tile: (dx, dy) (dz, a) (1/a, a^-1/2) (sqrt C6_ij, lam); rows (y, dy)
+    T *Jacc;                         // [A_NF][64] j-side tile sums
+    T *Cy, *Cdy;                     // centre rows [ne * nref]
+    T *Rt;                           // per type pair: FP32 (8/3) log2 R0^2, FP64 R0^2
+    int *Jt;                         // j type x ne [64]
+    __host__ __device__ static size_t bytes(int ne, int nref) {
+        const size_t ynr = (size_t)ne * nref;
+        return (4 + ynr) * AW * sizeof(T2<T>) + (size_t)A_NF * AW * sizeof(T) + 2 * ynr * sizeof(T) +
+               (size_t)ne * ne * sizeof(T) + AW * sizeof(int);
+    }
+    __device__ AtmLds(unsigned char *s, int ne, int nref) {
+        const size_t ynr = (size_t)ne * nref;
+        J0 = reinterpret_cast<T2<T> *>(s);
+        J1 = J0 + AW;
+        J2 = J1 + AW;
+        J3 = J2 + AW;
+        Jyd = J3 + AW;
+        Jacc = reinterpret_cast<T *>(Jyd + ynr * AW);
+        Cy = Jacc + A_NF * AW;
+        Cdy = Cy + ynr;
+        Rt = Cdy + ynr;
+        Jt = reinterpret_cast<int *>(Rt + (size_t)ne * ne);
+    }
+};
+
 template <class T, int NR>
 __global__ __launch_bounds__(AW) void k_atm(Params<T> P) {
     extern __shared__ __align__(16) unsigned char smem[];
+    constexpr bool F32 = sizeof(T) == 4;
     const int lane = threadIdx.x;
-    constexpr int nref = NR;
-    const int ne = P.ne, ynr = ne * nref;
-    // LDS: J tile fields [9][64] T | type [64] int | y [ynr][64] | dy [ynr][64] | acc [5][64] T |
-    //      centre rows Cy [ynr], Cdy [ynr] | R0^2 table [ne*ne]
-    T *Jf = reinterpret_cast<T *>(smem);
-    int *Jt = reinterpret_cast<int *>(Jf + 9 * AW);
-    T *Jy = reinterpret_cast<T *>(Jt + AW);
-    T *Jdy = Jy + (size_t)ynr * AW;
-    T *Jacc = Jdy + (size_t)ynr * AW;
-    T *Cy = Jacc + 5 * AW;
-    T *Cdy = Cy + ynr;
-    T *R0sq = Cdy + ynr;
+    const int ne = P.ne, ynr = ne * NR;
+    AtmLds<T> S(smem, ne, NR);
 
     const int cap = P.atm_cap;
     int *Lj = P.atm_i32 + (size_t)blockIdx.x * cap * 2;
     int *Lt = Lj + cap;
     T *LT = P.atm_T + (size_t)blockIdx.x * cap * ATM_NF;
-    // per-neighbour FP64 accumulators La[5][cap]: F_p (3), H_p, Gc_p (see phase 4)
     double *La = P.atm_acc + (size_t)blockIdx.x * atm_acc_stride(cap);
     const T rc32 = P.rc_32;
 
-    for (int idx = lane; idx < ne * ne; idx += AW) R0sq[idx] = P.r0[idx] * P.r0[idx];
+    for (int idx = lane; idx < ne * ne; idx += AW) {
+        const T r0sq = P.r0[idx] * P.r0[idx];
+        S.Rt[idx] = F32 ? T((8.0f / 3.0f) * log2f((float)r0sq)) : r0sq;
+    }
 
-    for (int i = blockIdx.x; i < P.n; i += gridDim.x) {
+    for (;;) {
+        // Ticket: every lane takes part in the atomic (lane 0 adds 1, the others 0), and the value
+        // is read back as a scalar.  No lane-dependent branch may sit in this loop: the compiler
+        // may split a loop on such a branch into an inner loop for lanes 1..63 and an outer one
+        // for lane 0, and lanes 1..63 would then run on without ever taking a ticket.
+        const int i = __builtin_amdgcn_readfirstlane(atomicAdd(P.atm_next, lane == 0 ? 1 : 0));
+        if (i >= P.n) break;
         Centre<T> C;
         C.load(P, i);
         const int ti = C.ti;
         __syncthreads();  // previous centre done with LDS and scratch
         for (int idx = lane; idx < ynr; idx += AW) {
-            const int B = idx / nref, bb = idx % nref;
+            const int B = idx / NR, bb = idx % NR;
             T sy = T(0), sdy = T(0);
-            for (int a = 0; a < nref; ++a) {
-                const T c = P.c6[((ti * ne + B) * nref + a) * nref + bb];
+            for (int a = 0; a < NR; ++a) {
+                const T c = P.c6[((ti * ne + B) * NR + a) * NR + bb];
                 sy += c * P.w[a * P.n + i];
                 sdy += c * P.dw[a * P.n + i];
             }
-            Cy[idx] = sy;
-            Cdy[idx] = sdy;
+            S.Cy[idx] = sy;
+            S.Cdy[idx] = sdy;
         }
         __syncthreads();
 
@@ -153,47 +191,49 @@ __global__ __launch_bounds__(AW) void k_atm(Params<T> P) {
                 count += __popcll(m);
             }
         }
-        if (count > cap) {  // uniform: flag, skip the centre, the host retries with a larger capacity
-            if (lane == 0) {
-                atomicOr(P.status, 2u);
-                P.eat[i] = 0.0;
-            }
-            continue;
-        }
-        const int nf = count;
+        // Over capacity: flag it and run the centre with no neighbours (its results are zeros);
+        // the host retries with a larger capacity.  (No `continue`: one back-edge keeps the
+        // loop in the simple shape described at the ticket.)
+        if (count > cap && lane == 0) atomicOr(P.status, 2u);
+        const int nf = count > cap ? 0 : count;
         __syncthreads();  // phase-1 slots were written by arbitrary lanes
 
-        // ---- phase 2: C6_ij, dlnC6_ij/dCN_i, dlnC6_ij/dCN_j, 1/r^2, R0_ij^2; zero accumulators.
-        // Slot p is always touched by lane p % 64, here and below.
+        // ---- phase 2: records; zero the accumulators.  Slot p is always touched by lane p % 64,
+        // here and below.
         for (int p = lane; p < nf; p += AW) {
             const int j = Lj[p], tj = P.type_s[j];
             T c6 = T(0), dci = T(0), dcj = T(0);
-            for (int b = 0; b < nref; ++b) {
+            for (int b = 0; b < NR; ++b) {
                 const T wj = P.w[b * P.n + j], dwj = P.dw[b * P.n + j];
-                c6 += Cy[tj * nref + b] * wj;
-                dci += Cdy[tj * nref + b] * wj;
-                dcj += Cy[tj * nref + b] * dwj;
+                c6 += S.Cy[tj * NR + b] * wj;
+                dci += S.Cdy[tj * NR + b] * wj;
+                dcj += S.Cy[tj * NR + b] * dwj;
             }
             const T ic6 = c6 > T(0) ? T(1) / c6 : T(0);
+            const T a = LT[F_R2 * cap + p];
+            const T ia = T(1) / a;
+            const T r0 = P.r0[ti * ne + tj];
             Lt[p] = tj;
-            LT[F_IR2 * cap + p] = T(1) / LT[F_R2 * cap + p];
-            LT[F_C6 * cap + p] = c6;
+            LT[F_IR2 * cap + p] = ia;
+            LT[F_RA * cap + p] = T(1) / xsqrt(a);
+            LT[F_SC6 * cap + p] = xsqrt(c6);
+            LT[F_LAM * cap + p] = F32 ? T((8.0f / 3.0f) * log2f((float)(r0 * r0 * ia))) : r0 * r0 * ia;
             LT[F_DLI * cap + p] = dci * ic6;
             LT[F_DLJ * cap + p] = dcj * ic6;
-            LT[F_R0SQ * cap + p] = R0sq[ti * ne + tj];
-            for (int q = 0; q < 5; ++q) La[q * cap + p] = 0.0;
+            for (int q = 0; q < A_NF; ++q) La[q * cap + p] = 0.0;
         }
         __syncthreads();
 
         // ---- phase 3: pairs {j, k}, j < k, of forward neighbours.  Per-pair terms are summed in T
         // over one 64 x 64 tile (at most 64 terms per sum); tile sums go to the FP64 per-neighbour
         // accumulators La, which lane l alone touches for its slots.
-        float cnt = 0.0f;
+        uint64_t cnt = 0;
         const int ntile = (nf + AW - 1) / AW;
         for (int kb = 0; kb < ntile; ++kb) {
             const int k = kb * AW + lane;
             const bool kval = k < nf;
-            T kx = T(0), ky = T(0), kz = T(0), kb2 = T(1), kib = T(1), kc6 = T(0), kr0 = T(0);
+            // padding lanes sit far away, so every pair with them fails the cutoff test
+            T kx = T(-FAR), ky = T(-FAR), kz = T(-FAR), kb2 = T(1), kib = T(1), krb = T(1), ksc6 = T(0), klam = T(0);
             T kw[NR], kdw[NR];
             int tk = 0;
             for (int a = 0; a < NR; ++a) kw[a] = kdw[a] = T(0);
@@ -204,10 +244,11 @@ __global__ __launch_bounds__(AW) void k_atm(Params<T> P) {
                 kz = LT[F_DZ * cap + k];
                 kb2 = LT[F_R2 * cap + k];
                 kib = LT[F_IR2 * cap + k];
-                kc6 = LT[F_C6 * cap + k];
-                kr0 = LT[F_R0SQ * cap + k];
+                krb = LT[F_RA * cap + k];
+                ksc6 = LT[F_SC6 * cap + k];
+                klam = F32 ? LT[F_LAM * cap + k] + T(L2K6_83) : T(K6) * LT[F_LAM * cap + k];
                 tk = Lt[k];
-                for (int a = 0; a < nref; ++a) {
+                for (int a = 0; a < NR; ++a) {
                     kw[a] = P.w[a * P.n + jk];
                     kdw[a] = P.dw[a * P.n + jk];
                 }
@@ -216,12 +257,13 @@ __global__ __launch_bounds__(AW) void k_atm(Params<T> P) {
             const float kminx = uni(wmin(kval ? (float)kx : 3e38f)), kmaxx = uni(wmax(kval ? (float)kx : -3e38f));
             const float kminy = uni(wmin(kval ? (float)ky : 3e38f)), kmaxy = uni(wmax(kval ? (float)ky : -3e38f));
             const float kminz = uni(wmin(kval ? (float)kz : 3e38f)), kmaxz = uni(wmax(kval ? (float)kz : -3e38f));
+            const int ykb = tk * NR * AW;
 
             for (int jt = 0; jt <= kb; ++jt) {
                 const int pj = jt * AW + lane;
                 const bool jval = pj < nf;
                 // bounding box of the j tile from the records (global scratch, same-lane slots)
-                T jxs = T(0), jys = T(0), jzs = T(0);
+                T jxs = T(FAR), jys = T(FAR), jzs = T(FAR);
                 if (jval) {
                     jxs = LT[F_DX * cap + pj];
                     jys = LT[F_DY * cap + pj];
@@ -236,119 +278,148 @@ __global__ __launch_bounds__(AW) void k_atm(Params<T> P) {
                 // skip only when clearly apart (1e-3 relative margin covers float rounding)
                 if (gx * gx + gy * gy + gz * gz > (float)rc32 * 1.001f) continue;
 
-                // stage the j tile
+                // stage the j tile (padding lanes: far coordinates, zero rows)
                 wave_sync();
-                if (jval) {
-                    Jf[F_DX * AW + lane] = jxs;
-                    Jf[F_DY * AW + lane] = jys;
-                    Jf[F_DZ * AW + lane] = jzs;
-                    Jf[F_R2 * AW + lane] = LT[F_R2 * cap + pj];
-                    Jf[F_IR2 * AW + lane] = LT[F_IR2 * cap + pj];
-                    Jf[F_C6 * AW + lane] = LT[F_C6 * cap + pj];
-                    Jf[F_R0SQ * AW + lane] = LT[F_R0SQ * cap + pj];
-                    const int jj = Lj[pj], tj = Lt[pj];
-                    Jt[lane] = tj;
+                {
+                    T a = T(1), ia = T(1), ra = T(1), sc6 = T(0), lam = T(0);
+                    int tj = 0;
                     T wj[NR], dwj[NR];
-                    for (int a = 0; a < nref; ++a) {
-                        wj[a] = P.w[a * P.n + jj];
-                        dwj[a] = P.dw[a * P.n + jj];
-                    }
-                    for (int B = 0; B < ne; ++B)
-                        for (int bb = 0; bb < nref; ++bb) {
-                            T sy = T(0), sdy = T(0);
-                            for (int a = 0; a < nref; ++a) {
-                                const T c = P.c6[((tj * ne + B) * nref + a) * nref + bb];
-                                sy += c * wj[a];
-                                sdy += c * dwj[a];
-                            }
-                            Jy[(B * nref + bb) * AW + lane] = sy;
-                            Jdy[(B * nref + bb) * AW + lane] = sdy;
+                    for (int q = 0; q < NR; ++q) wj[q] = dwj[q] = T(0);
+                    if (jval) {
+                        a = LT[F_R2 * cap + pj];
+                        ia = LT[F_IR2 * cap + pj];
+                        ra = LT[F_RA * cap + pj];
+                        sc6 = LT[F_SC6 * cap + pj];
+                        lam = LT[F_LAM * cap + pj];
+                        const int jj = Lj[pj];
+                        tj = Lt[pj];
+                        for (int q = 0; q < NR; ++q) {
+                            wj[q] = P.w[q * P.n + jj];
+                            dwj[q] = P.dw[q * P.n + jj];
                         }
+                    }
+                    S.J0[lane] = T2<T>{jxs, jys};
+                    S.J1[lane] = T2<T>{jzs, a};
+                    S.J2[lane] = T2<T>{ia, ra};
+                    S.J3[lane] = T2<T>{sc6, lam};
+                    S.Jt[lane] = tj * ne;  // row of the per-type-pair table
+                    for (int B = 0; B < ne; ++B)
+                        for (int bb = 0; bb < NR; ++bb) {
+                            T sy = T(0), sdy = T(0);
+                            for (int q = 0; q < NR; ++q) {
+                                const T c = P.c6[((tj * ne + B) * NR + q) * NR + bb];
+                                sy += c * wj[q];
+                                sdy += c * dwj[q];
+                            }
+                            S.Jyd[(B * NR + bb) * AW + lane] = T2<T>{sy, sdy};
+                        }
+                    for (int q = 0; q < A_NF; ++q) S.Jacc[q * AW + lane] = T(0);
                 }
-                for (int q = 0; q < 5; ++q) Jacc[q * AW + lane] = T(0);
                 wave_sync();
 
                 // tile-local sums of neighbour k (folded into FP64 per tile)
-                T kF0 = T(0), kF1 = T(0), kF2 = T(0), kH = T(0), kG = T(0);
+                T kA = T(0), kBx = T(0), kBy = T(0), kBz = T(0), kH = T(0), kG = T(0);
                 const int jlim = (jt == kb) ? lane : AW;  // diagonal tile: j < k
 #pragma unroll 1
                 for (int q = 0; q < AW; ++q) {
-                    // lane l updates the j accumulator that lane l+1 updated one step earlier
-                    wave_sync();
+                    // FP32: lane l updates the j slot that lane l+1 updated one step earlier
+                    if constexpr (F32) wave_sync();
                     const int jl = (lane + q) & (AW - 1);
-                    const int jg = jt * AW + jl;
-                    if (!(kval && jl < jlim && jg < nf)) continue;
-                    const T jx = Jf[F_DX * AW + jl], jy = Jf[F_DY * AW + jl], jz = Jf[F_DZ * AW + jl];
-                    const T vx = kx - jx, vy = ky - jy, vz = kz - jz;  // v = r_j - r_k
+                    const T2<T> j0 = S.J0[jl], j1 = S.J1[jl];
+                    const T jx = j0.x, jy = j0.y, jz = j1.x;
+                    const T vx = kx - jx, vy = ky - jy, vz = kz - jz;  // r_j - r_k
                     const T c = vx * vx + vy * vy + vz * vz;
-                    if (c > rc32 || c < T(1e-12)) continue;
-                    const T a = Jf[F_R2 * AW + jl], ia = Jf[F_IR2 * AW + jl];
-                    const T b = kb2, ib = kib;
-                    const int tj = Jt[jl];
+                    const bool ok = jl < jlim && c <= rc32 && c >= T(1e-12);
+                    const uint64_t m = __builtin_amdgcn_ballot_w64(ok);
+                    if (m == 0) continue;  // uniform
+                    cnt += __popcll(m);
+                    if (!ok) continue;
+                    const T a = j1.y, b = kb2;
+                    const T2<T> j2 = S.J2[jl], j3 = S.J3[jl];
+                    const int tjr = S.Jt[jl];
                     // C6_jk and its CN derivatives from the j row for type(k)
                     T c6jk = T(0), dk6 = T(0), dj6 = T(0);
-                    const int yb = (tk * nref) * AW + jl;
-                    for (int bb = 0; bb < nref; ++bb) {
-                        const T yv = Jy[yb + bb * AW], dyv = Jdy[yb + bb * AW];
-                        c6jk += yv * kw[bb];
-                        dk6 += yv * kdw[bb];
-                        dj6 += dyv * kw[bb];
+#pragma unroll
+                    for (int bb = 0; bb < NR; ++bb) {
+                        const T2<T> yd = S.Jyd[ykb + bb * AW + jl];
+                        c6jk += yd.x * kw[bb];
+                        dk6 += yd.x * kdw[bb];
+                        dj6 += yd.y * kw[bb];
                     }
-                    const T ic6jk = c6jk > T(0) ? xrcp(c6jk) : T(0);
-                    const T C9 = -fast_sqrt(Jf[F_C6 * AW + jl] * kc6 * c6jk);
-                    const T abc = a * b * c;
-                    const T iR = xrsqrt(abc);
+                    const T rs = prsqrt(c6jk);
+                    const T C9 = -(j3.x * ksc6) * (c6jk * rs);
+                    const T ic6 = rs * rs;
+                    const T rcq = prsqrt(c);
+                    const T ic = rcq * rcq;
+                    const T iR = j2.y * krb * rcq;
                     const T iR2 = iR * iR, iR3 = iR2 * iR, iR5 = iR3 * iR2;
-                    const T ic = xrcp(c);
-                    const T u1 = a + b - c, u2 = a - b + c, u3 = -a + b + c;
-                    const T Q = u1 * u2 * u3;
-                    const T dQa = u2 * u3 + u1 * (u3 - u2);
-                    const T dQb = u2 * u3 + u1 * (u2 - u3);
-                    const T dQc = u1 * (u2 + u3) - u2 * u3;
-                    const T A = iR3 + T(0.375) * Q * iR5;
-                    const T yv6 = T(K6) * Jf[F_R0SQ * AW + jl] * kr0 * R0sq[tj * ne + tk] * ia * ib * ic;
-                    const T x16 = yv6 * yv6 * yv6 * inv_cbrt(yv6);
-                    const T f = xrcp(T(1) + T(6) * x16);
-                    const T fx = T(16) * x16 * f * f;
-                    const T P1 = T(-1.5) * iR3 - T(0.9375) * Q * iR5;
-                    const T G2 = T(2) * C9 * (P1 * f + A * fx);
-                    const T H2 = T(0.75) * C9 * iR5 * f;
-                    const T tij = G2 * ia + H2 * dQa;
-                    const T tik = G2 * ib + H2 * dQb;
+                    const T tsum = a + b + c;
+                    const T u1 = tsum - T(2) * c, u2 = tsum - T(2) * b, u3 = tsum - T(2) * a;
+                    const T p23 = u2 * u3, d32 = u3 - u2;
+                    const T Q = u1 * p23;
+                    const T dQa = p23 + u1 * d32, dQb = p23 - u1 * d32, dQc = u1 * (u2 + u3) - p23;
+                    const T QiR5 = Q * iR5;
+                    const T A = iR3 + T(0.375) * QiR5;
+                    const T P1 = T(-1.5) * iR3 - T(0.9375) * QiR5;
+                    T x16;
+                    if constexpr (F32) {
+                        x16 = __builtin_amdgcn_exp2f(j3.y + klam + S.Rt[tjr + tk] -
+                                                     (8.0f / 3.0f) * __builtin_amdgcn_logf(c));
+                    } else {
+                        const T y = j3.y * klam * S.Rt[tjr + tk] * ic;
+                        x16 = y * y * y * inv_cbrt(y);
+                    }
+                    const T f = prcp(T(1) + T(6) * x16);
+                    const T cf = C9 * f;
+                    const T hC = T(0.5) * cf * A;  // e / 2
+                    const T G2 = T(2) * cf * (P1 + T(8.0 / 3.0) * A * (T(1) - f));
+                    const T H2 = T(0.75) * cf * iR5;
+                    const T tij = G2 * j2.x + H2 * dQa;
+                    const T tik = G2 * kib + H2 * dQb;
                     const T tjk = G2 * ic + H2 * dQc;
-                    // hC = e/2; the centre's share, its force, dE/dCN and the virial follow from
-                    // the per-neighbour sums after the tiles (phase 4)
-                    const T hC = T(0.5) * C9 * A * f;
-                    // neighbour j (LDS slot jl: distinct across lanes at each step)
-                    const T fjx = -tij * jx + tjk * vx, fjy = -tij * jy + tjk * vy, fjz = -tij * jz + tjk * vz;
-                    Jacc[0 * AW + jl] += fjx;
-                    Jacc[1 * AW + jl] += fjy;
-                    Jacc[2 * AW + jl] += fjz;
-                    Jacc[3 * AW + jl] += hC;
-                    Jacc[4 * AW + jl] += hC * dj6 * ic6jk;
-                    // neighbour k (registers): grad_k e = -t_ik d_k - t_jk v
-                    kF0 -= tik * kx + tjk * vx;
-                    kF1 -= tik * ky + tjk * vy;
-                    kF2 -= tik * kz + tjk * vz;
+                    const T hc6 = hC * ic6;
+                    // neighbour j: LDS slot jl, distinct across lanes at each step.  FP32: plain
+                    // read-modify-write (LDS float atomics measured ~6x slower on gfx950); FP64:
+                    // LDS atomics (no per-step barrier, measured ~4% faster than the barrier form).
+                    if constexpr (F32) {
+                        S.Jacc[A_A * AW + jl] += tij + tjk;
+                        S.Jacc[A_BX * AW + jl] += tjk * kx;
+                        S.Jacc[A_BY * AW + jl] += tjk * ky;
+                        S.Jacc[A_BZ * AW + jl] += tjk * kz;
+                        S.Jacc[A_H * AW + jl] += hC;
+                        S.Jacc[A_G * AW + jl] += hc6 * dj6;
+                    } else {
+                        atomicAdd(&S.Jacc[A_A * AW + jl], tij + tjk);
+                        atomicAdd(&S.Jacc[A_BX * AW + jl], tjk * kx);
+                        atomicAdd(&S.Jacc[A_BY * AW + jl], tjk * ky);
+                        atomicAdd(&S.Jacc[A_BZ * AW + jl], tjk * kz);
+                        atomicAdd(&S.Jacc[A_H * AW + jl], hC);
+                        atomicAdd(&S.Jacc[A_G * AW + jl], hc6 * dj6);
+                    }
+                    // neighbour k: registers
+                    kA += tik + tjk;
+                    kBx += tjk * jx;
+                    kBy += tjk * jy;
+                    kBz += tjk * jz;
                     kH += hC;
-                    kG += hC * dk6 * ic6jk;
-                    cnt += 1.0f;
+                    kG += hc6 * dk6;
                 }
                 if (kval) {
-                    La[0 * cap + k] += (double)kF0;
-                    La[1 * cap + k] += (double)kF1;
-                    La[2 * cap + k] += (double)kF2;
-                    La[3 * cap + k] += (double)kH;
-                    La[4 * cap + k] += (double)kG;
+                    La[A_A * cap + k] += (double)kA;
+                    La[A_BX * cap + k] += (double)kBx;
+                    La[A_BY * cap + k] += (double)kBy;
+                    La[A_BZ * cap + k] += (double)kBz;
+                    La[A_H * cap + k] += (double)kH;
+                    La[A_G * cap + k] += (double)kG;
                 }
                 wave_sync();
                 if (jval)
-                    for (int q = 0; q < 5; ++q) La[q * cap + pj] += (double)Jacc[q * AW + lane];
+                    for (int q = 0; q < A_NF; ++q) La[q * cap + pj] += (double)S.Jacc[q * AW + lane];
             }
         }
 
-        // ---- phase 4.  Per neighbour slot p: F_p = sum of grad_p e, H_p = sum of e/2, Gc_p = sum of
-        // (e/2) dlnC6_jk/dCN_p over the triangles it belongs to.  Then
+        // ---- phase 4.  Per neighbour slot p: F_p = grad_p E = -d_p A_p + B_p, H_p = sum of e/2,
+        // Gc_p = sum of (e/2) dlnC6_pq/dCN_p over the triangles it belongs to.  Then
         //   dE/dCN_p = Gc_p + (dlnC6_ip/dCN_p) H_p,  E/3 share of p = (2/3) H_p,
         //   centre: E = sum H_p (each triangle feeds two slots), dE/dCN_i = sum (dlnC6_ip/dCN_i) H_p,
         //   F_i = -sum F_p (translation invariance), virial = sum d_p (x) F_p
@@ -360,10 +431,12 @@ __global__ __launch_bounds__(AW) void k_atm(Params<T> P) {
         for (int q = 0; q < 9; ++q) aW[q].init();
         for (int p = lane; p < nf; p += AW) {
             const int j = Lj[p];
-            const double fx = La[0 * cap + p], fy = La[1 * cap + p], fz = La[2 * cap + p];
-            const double H = La[3 * cap + p], Gc = La[4 * cap + p];
             const double dx = (double)LT[F_DX * cap + p], dy = (double)LT[F_DY * cap + p],
                          dz = (double)LT[F_DZ * cap + p];
+            const double Ap = La[A_A * cap + p];
+            const double fx = La[A_BX * cap + p] - dx * Ap, fy = La[A_BY * cap + p] - dy * Ap,
+                         fz = La[A_BZ * cap + p] - dz * Ap;
+            const double H = La[A_H * cap + p], Gc = La[A_G * cap + p];
             atomicAdd(&P.fa[3 * j + 0], fx);
             atomicAdd(&P.fa[3 * j + 1], fy);
             atomicAdd(&P.fa[3 * j + 2], fz);
@@ -390,7 +463,6 @@ __global__ __launch_bounds__(AW) void k_atm(Params<T> P) {
         for (int q = 0; q < 9; ++q) wv[q] = wave_total(aW[q]);
         const double W[6] = {wv[0], wv[1], wv[2], 0.5 * (wv[3] + wv[4]), 0.5 * (wv[5] + wv[6]),
                              0.5 * (wv[7] + wv[8])};
-        cnt = wave_sumf(cnt);
         if (lane == 0) {
             atomicAdd(&P.fa[3 * i + 0], F[0]);
             atomicAdd(&P.fa[3 * i + 1], F[1]);
@@ -400,14 +472,16 @@ __global__ __launch_bounds__(AW) void k_atm(Params<T> P) {
             P.eat[i] = E;
             const double iv = C.S->inv_vol;
             for (int q = 0; q < 6; ++q) P.vir[6 * i + q] += W[q] * iv;
-            P.npair[3 * i + 2] = cnt;
+            P.npair[3 * i + 2] = (float)cnt;
         }
     }
 }
 
-__global__ void k_fill_d(double *p, size_t n) {
+// Zeroes fa | ga | eas | eat (6 N doubles) and the centre counter.
+__global__ void k_fill_d(double *p, size_t n, int *counter) {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) p[t] = 0.0;
+    if (t == 0) *counter = 0;
 }
 
 // P.g += ga, P.f += fa (after the ATM kernel, before the chain rule).
@@ -419,18 +493,33 @@ __global__ void k_merge(Params<T> P) {
     for (int q = 0; q < 3; ++q) P.f[3 * p + q] += P.fa[3 * p + q];
 }
 
+template <class T>
+const void *atm_kernel(int nref) {
+    return nref == 5 ? reinterpret_cast<const void *>(&k_atm<T, 5>) : reinterpret_cast<const void *>(&k_atm<T, 7>);
+}
+
 }  // namespace
 
 size_t atm_lds_bytes(int ne, int nref, size_t ts) {
-    const size_t ynr = (size_t)ne * nref;
-    return 9 * AW * ts + AW * sizeof(int) + 2 * ynr * AW * ts + 5 * AW * ts + 2 * ynr * ts + (size_t)ne * ne * ts;
+    return ts == sizeof(double) ? AtmLds<double>::bytes(ne, nref) : AtmLds<float>::bytes(ne, nref);
+}
+
+int atm_resident_blocks(int ne, int nref, size_t ts) {
+    const size_t lds = atm_lds_bytes(ne, nref, ts);
+    const void *fn = ts == sizeof(double) ? atm_kernel<double>(nref) : atm_kernel<float>(nref);
+    int per_cu = 0, dev = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, AW, lds) != hipSuccess) per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        cus = 0;
+    return per_cu > 0 && cus > 0 ? per_cu * cus : 4096;
 }
 
 template <class T>
 void launch_atm(const Params<T> &P, int nblocks, hipStream_t s) {
-    // fa (3N), ga, eas, eat (N each) are contiguous: zero them in one pass
+    // fa (3N), ga, eas, eat (N each) are contiguous: zero them (and the centre counter) in one pass
     const size_t nz = (size_t)6 * P.n;
-    if (nz) hipLaunchKernelGGL(k_fill_d, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, s, P.fa, nz);
+    if (nz) hipLaunchKernelGGL(k_fill_d, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, s, P.fa, nz, P.atm_next);
     if (P.n == 0 || !(P.rc_32 > T(0))) return;
     const size_t lds = atm_lds_bytes(P.ne, P.nref, sizeof(T));
     if (P.nref == 5)

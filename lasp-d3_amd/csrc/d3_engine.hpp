@@ -79,21 +79,23 @@ struct Params {
     const T *c6, *cnref, *rcov, *q4, *rdamp, *r0;
     // ATM: per-block scratch for one centre's forward neighbours (capacity atm_cap):
     //   atm_i32: j, type (2 x cap); atm_T: record fields (ATM_NF x cap);
-    //   atm_acc: per-neighbour accumulators F[3], dE/dCN, E/3 (5 x cap, double)
+    //   atm_acc: per-neighbour accumulators (6 x cap, double)
     // and per-atom results (sorted order): fa[3N], ga[N], eas[N] filled with atomics (every
     // triangle's gradient, dE/dCN and e/3 for each vertex), eat[N] = sum of e over the
-    // triangles anchored at each centre (deterministic total).
+    // triangles anchored at each centre (deterministic total).  atm_next: the next centre to
+    // take (zeroed before every launch).
     int atm_cap;
     int *atm_i32;
     T *atm_T;
     double *atm_acc;
     double *fa, *ga, *eas, *eat;
+    int *atm_next;
     unsigned *status;
 };
 
 constexpr int ATM_NF = 10;  // record fields per ATM neighbour
-// doubles of ATM accumulator scratch per block: 5 per neighbour slot + 11 x 64 for the centre
-__host__ __device__ constexpr size_t atm_acc_stride(int cap) { return 5 * (size_t)cap + 11 * 64; }
+// doubles of ATM accumulator scratch per block: 6 per neighbour slot
+__host__ __device__ constexpr size_t atm_acc_stride(int cap) { return 6 * (size_t)cap; }
 
 // Host-side launchers (d3_kernels.hip).
 template <class T>
@@ -106,10 +108,11 @@ struct Launch {
 
 // Three-body term (d3_atm.hip): zeroes the ATM per-atom arrays, runs the triangle kernel on
 // nblocks persistent one-wave blocks (when P.rc_32 > 0) and merges forces and dE/dCN into
-// P.f / P.g.
+// P.f / P.g.  atm_resident_blocks: how many of those blocks the current device holds at once.
 template <class T>
 void launch_atm(const Params<T> &P, int nblocks, hipStream_t s);
 size_t atm_lds_bytes(int ne, int nref, size_t tsize);
+int atm_resident_blocks(int ne, int nref, size_t tsize);
 
 // Reduction of per-atom energy / virial / pair counts into per-cell totals (deterministic),
 // and scatter of per-atom results to caller order with unit conversion.

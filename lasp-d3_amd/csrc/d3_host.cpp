@@ -196,8 +196,8 @@ Engine::Engine(const uint16_t *elements, uint64_t num_elements, uint64_t max_len
     HIPCHK(hipStreamCreate(&own_stream_));
     stream_ = own_stream_;
     alloc_atoms();
-    d_status = dalloc(sizeof(unsigned));
-    HIPCHK(hipMemset(d_status, 0, sizeof(unsigned)));
+    d_status = dalloc(2 * sizeof(unsigned));  // status word | three-body centre counter
+    HIPCHK(hipMemset(d_status, 0, 2 * sizeof(unsigned)));
 }
 
 Engine::~Engine() {
@@ -478,7 +478,7 @@ void Engine::ensure_atm_scratch() {
     const double mean = 0.5 * dens * 4.18879020478639 * rc3_ * rc3_ * rc3_;
     int cap = (int)std::min(262144.0, 1.5 * mean + 1024.0);
     cap = (std::max(cap, atm_cap_) + 63) / 64 * 64;
-    int nb = atm_blocks_req_ > 0 ? atm_blocks_req_ : 4096;
+    int nb = atm_blocks_req_ > 0 ? atm_blocks_req_ : atm_resident();
     nb = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)nb, n_));
     atm_cap_ = cap;
     atm_blocks_ = nb;
@@ -563,6 +563,7 @@ Params<T> Engine::params() {
     P.eas = P.ga + n_;
     P.eat = P.eas + n_;
     P.status = (unsigned *)d_status;
+    P.atm_next = (int *)d_status + 1;
     return P;
 }
 
@@ -771,10 +772,13 @@ double Engine::autotune(const std::string &cache_path) {
     }
     bin_width_ = best_w;
     geom_dirty_ = true;
-    int best_b = atm_blocks_req_ > 0 ? atm_blocks_req_ : 4096;
+    const int res = atm_resident();
+    int best_b = atm_blocks_req_ > 0 ? atm_blocks_req_ : res;
     if (rc3_ > 0) {
         best = 1e300;
-        for (int b : {2048, 4096, 8192, 16384}) {
+        // blocks take centres from a shared counter: beyond the resident count more blocks only
+        // add scratch, fewer leave compute units idle
+        for (int b : {res / 2, (3 * res) / 4, res, 2 * res}) {
             atm_blocks_req_ = b;
             const double t = timed_evaluation();
             if (t < best) {

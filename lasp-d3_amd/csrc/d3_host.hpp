@@ -79,6 +79,8 @@ class Engine {
     double autotune(const std::string &cache_path);
     std::string tune_key() const;
     int atm_blocks_request() const { return atm_blocks_req_; }
+    // Three-body grid of the next evaluation: the request, else the device's resident count.
+    int atm_blocks_effective() { return atm_blocks_req_ > 0 ? atm_blocks_req_ : atm_resident(); }
 
    private:
     struct Cell {
@@ -143,6 +145,12 @@ class Engine {
     void *d_atm_i32 = nullptr, *d_atm_T = nullptr, *d_atm_acc = nullptr;
     void *d_atm_pa = nullptr;  // per-atom ATM results: fa[3N] | ga[N] | eas[N] | eat[N] (double)
     int atm_blocks_ = 0, atm_cap_ = 0;
+    int atm_resident_ = 0;  // three-body blocks resident on the device at once (0: not queried)
+    int atm_resident() {
+        if (atm_resident_ <= 0)
+            atm_resident_ = atm_resident_blocks(ne_, nref_, precision_ == 0 ? sizeof(double) : sizeof(float));
+        return atm_resident_;
+    }
     size_t atm_alloc_entries_ = 0;  // allocated scratch entries (>= atm_blocks_ * atm_cap_)
     int atm_alloc_blocks_ = 0;      // blocks the scratch was allocated for (>= atm_blocks_)
 
