@@ -1,0 +1,77 @@
+/* Learned stand-in for the ATM three-body term — C API.
+ *
+ * No counterpart in the reference, whose only three-body path is the exact ATM kernel
+ * (src/d3_kernel.cu:1199-1509, reached through compute_dispersion_energy_from_handle in
+ * src/d3.h:114-128).  The stand-in is a per-atom network: features x_i[64] = {CN_i, 63 Gaussian
+ * radial sums over neighbours grouped in 3 element classes (21 centres 1..8 A, sigma 0.5 A,
+ * smooth cutoff 9.5 A)}; per class an MLP 64 -> 128 -> 128 -> 1 (SiLU) whose two hidden layers
+ * and input-gradient backward run on bf16 MFMA with fp32 accumulation; E = sum_i E_i, forces and
+ * stress by the hand-written backward through the features and CN.
+ *
+ * Parameter vector (D3X_SUR_NPARAMS floats): mean[64] | istd[64] | per class c = 0, 1, 2:
+ * W1[64][128] | b1[128] | W2[128][128] | b2[128] | w3[128] | b3, scale, shift, pad;
+ * E_i = scale (w3 . silu(W2^T silu(W1^T ((x - mean) istd) + b1) + b2) + b3) + shift  [eV].
+ * Classes: 0 = H, He; 1 = Li..Ne; 2 = heavier elements.
+ */
+#ifndef D3X_SUR_H
+#define D3X_SUR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "d3.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define D3X_SUR_NPARAMS 75020
+#define D3X_SUR_NFEAT 64
+
+/* Three-body mode of a handle.  EXACT: the ATM kernel (default).  SURROGATE: the network; every
+ * judge_every-th evaluation (and the first after atoms or cell change) also runs the exact kernel
+ * and, when |E_net - E_exact| / atoms > tol_ev_per_atom, returns the exact result and keeps the
+ * handle on the exact kernel from then on (the fallback; d3x_sur_state reports it).
+ * judge_every = 0 judges only the first evaluation after a geometry change.
+ * EXACT_FEATURES: exact kernel, and the network's features are computed too (d3x_sur_features),
+ * for building training sets. */
+#define D3X_ATM_EXACT 0
+#define D3X_ATM_SURROGATE 1
+#define D3X_ATM_EXACT_FEATURES 2
+
+/* Network parameters for a handle (host array of D3X_SUR_NPARAMS floats, or a file written by
+ * lasp-d3_amd/surrogate/train.py: 8 bytes "D3SUR001", uint32 count, count floats). */
+int d3x_sur_load(d3_handle_t *handle, const float *params);
+int d3x_sur_load_file(d3_handle_t *handle, const char *path);
+int d3x_set_atm_mode(d3_handle_t *handle, int mode, int judge_every, double tol_ev_per_atom);
+/* mode in use (after fallbacks), fallback count, |E_net - E_exact| / atoms of the last judged
+ * evaluation (eV; negative before any judgement).  Any pointer may be NULL. */
+int d3x_sur_state(d3_handle_t *handle, int *mode, int *fallbacks, double *last_dev_ev_per_atom);
+/* Features of the last evaluation (SURROGATE or EXACT_FEATURES), caller order, atoms x 64. */
+int d3x_sur_features(d3_handle_t *handle, float *features);
+
+/* Network kernels on device buffers (pointers from hipMalloc or a framework's allocator, stream
+ * = hipStream_t or NULL for the default stream).  They enqueue and return; errors of the launch
+ * are returned as D3X status bits. */
+size_t d3x_sur_packed_bytes(void);
+/* bytes of scratch for d3x_sur_train_grad on `rows` rows */
+size_t d3x_sur_train_bytes(uint64_t rows);
+/* bf16 MFMA fragments of every weight matrix from the fp32 parameters */
+int d3x_sur_pack(const float *params, void *packed, void *stream);
+/* per-row energies E[n] (eV) and input gradients G[n][64] (either may be NULL) of rows X[n][64]
+ * with classes cls[n] */
+int d3x_sur_infer(const float *params, const void *packed, uint64_t n, const float *X, const int32_t *cls,
+                  float *E, float *G, void *stream);
+/* one training gradient on n rows of class cls: E[n] (out), loss += sum 0.5 w (E - y)^2, and
+ * grad (D3X_SUR_NPARAMS floats, accumulated) += dloss/dparams */
+int d3x_sur_train_grad(const float *params, const void *packed, int cls, uint64_t n, const float *X, const float *y,
+                       const float *w, float *E, void *work, float *grad, float *loss, void *stream);
+/* Adam step (bias-corrected, step >= 1) on the trainable entries; clears grad */
+int d3x_sur_adam(float *params, float *grad, float *m, float *v, float lr, float beta1, float beta2, float eps,
+                 int step, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* D3X_SUR_H */

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/d3x.h"
+#include "../../include/d3x_sur.h"
 #include "d3_host.hpp"
 #include "d3_tables.hpp"
 
@@ -58,9 +59,44 @@ uint32_t guarded(Handle *h, const char *where, F &&fn) {
     return st;
 }
 
-// Full evaluation with results copied to the host; a three-body neighbour-list overflow is
-// retried with a larger capacity (capacity grows x4, at most 4 times).
+void evaluate_once(Engine &e, double *energies, double *forces, double *stresses);
+
+// Full evaluation with results copied to the host.  With the three-body stand-in active, a
+// judged evaluation runs the network and the exact kernel: on agreement (per-cell |dE| / atoms
+// within the tolerance) the network's results are returned, otherwise the exact ones, and the
+// handle stays on the exact kernel.
 void evaluate(Engine &e, double *energies, double *forces, double *stresses) {
+    if (e.mode_now() != D3X_ATM_SURROGATE) {
+        evaluate_once(e, energies, forces, stresses);
+        return;
+    }
+    const bool judge = e.judge_due();
+    e.count_surrogate_eval();
+    if (!judge) {
+        evaluate_once(e, energies, forces, stresses);
+        return;
+    }
+    const size_t ns = (size_t)e.num_sys();
+    std::vector<double> e_net(ns), e_exact(ns);
+    evaluate_once(e, e_net.data(), nullptr, nullptr);
+    e.set_force_exact(true);
+    try {
+        evaluate_once(e, e_exact.data(), forces, stresses);
+    } catch (...) {
+        e.set_force_exact(false);
+        throw;
+    }
+    e.set_force_exact(false);
+    if (e.record_judgement(e_net.data(), e_exact.data())) {
+        evaluate_once(e, energies, forces, stresses);  // the network's results
+    } else if (energies) {
+        std::copy(e_exact.begin(), e_exact.end(), energies);  // fallen back: exact results stand
+    }
+}
+
+// One evaluation; a three-body neighbour-list overflow is retried with a larger capacity
+// (capacity grows x4, at most 4 times).
+void evaluate_once(Engine &e, double *energies, double *forces, double *stresses) {
     for (int attempt = 0;; ++attempt) {
         e.compute_device();
         try {
@@ -282,6 +318,49 @@ double d3x_autotune(d3_handle_t *handle, const char *cache_path) {
     uint32_t st = guarded(H(handle), "d3x_autotune",
                           [&] { best = H(handle)->eng->autotune(cache_path ? cache_path : ""); });
     return st ? -1.0 : best;
+}
+
+int d3x_sur_load(d3_handle_t *handle, const float *params) {
+    if (!handle) return (int)(D3X_ERR_GENERIC | D3X_ERR_INPUT);
+    return (int)guarded(H(handle), "d3x_sur_load", [&] { H(handle)->eng->sur_load(params); });
+}
+
+int d3x_sur_load_file(d3_handle_t *handle, const char *path) {
+    if (!handle) return (int)(D3X_ERR_GENERIC | D3X_ERR_INPUT);
+    return (int)guarded(H(handle), "d3x_sur_load_file", [&] {
+        if (!path) throw Error{D3X_ERR_GENERIC | D3X_ERR_INPUT, "NULL path"};
+        FILE *f = std::fopen(path, "rb");
+        if (!f) throw Error{D3X_ERR_GENERIC | D3X_ERR_INPUT, std::string("cannot open ") + path};
+        char magic[8] = {0};
+        uint32_t count = 0;
+        std::vector<float> p(D3X_SUR_NPARAMS);
+        const bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, "D3SUR001", 8) == 0 &&
+                        std::fread(&count, 4, 1, f) == 1 && count == (uint32_t)D3X_SUR_NPARAMS &&
+                        std::fread(p.data(), sizeof(float), p.size(), f) == p.size();
+        std::fclose(f);
+        if (!ok) throw Error{D3X_ERR_GENERIC | D3X_ERR_INPUT, std::string("not a network parameter file: ") + path};
+        H(handle)->eng->sur_load(p.data());
+    });
+}
+
+int d3x_set_atm_mode(d3_handle_t *handle, int mode, int judge_every, double tol_ev_per_atom) {
+    if (!handle) return (int)(D3X_ERR_GENERIC | D3X_ERR_INPUT);
+    return (int)guarded(H(handle), "d3x_set_atm_mode",
+                        [&] { H(handle)->eng->set_atm_mode(mode, judge_every, tol_ev_per_atom); });
+}
+
+int d3x_sur_state(d3_handle_t *handle, int *mode, int *fallbacks, double *last_dev_ev_per_atom) {
+    if (!handle) return (int)(D3X_ERR_GENERIC | D3X_ERR_INPUT);
+    const Engine &e = *H(handle)->eng;
+    if (mode) *mode = e.mode_now();
+    if (fallbacks) *fallbacks = e.fallbacks();
+    if (last_dev_ev_per_atom) *last_dev_ev_per_atom = e.last_deviation();
+    return 0;
+}
+
+int d3x_sur_features(d3_handle_t *handle, float *features) {
+    if (!handle || !features) return (int)(D3X_ERR_GENERIC | D3X_ERR_INPUT);
+    return (int)guarded(H(handle), "d3x_sur_features", [&] { H(handle)->eng->fetch_features(features); });
 }
 
 int d3x_set_graph(d3_handle_t *handle, int enable) {

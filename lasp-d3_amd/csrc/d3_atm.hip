@@ -532,4 +532,11 @@ void launch_atm(const Params<T> &P, int nblocks, hipStream_t s) {
 template void launch_atm<double>(const Params<double> &, int, hipStream_t);
 template void launch_atm<float>(const Params<float> &, int, hipStream_t);
 
+template <class T>
+void launch_atm_merge(const Params<T> &P, hipStream_t s) {
+    if (P.n) hipLaunchKernelGGL(k_merge<T>, dim3((P.n + 255) / 256), dim3(256), 0, s, P);
+}
+template void launch_atm_merge<double>(const Params<double> &, hipStream_t);
+template void launch_atm_merge<float>(const Params<float> &, hipStream_t);
+
 }  // namespace d3

@@ -25,7 +25,8 @@ constexpr double HARTREE_EV = 27.211396641308;   // src/d3.cu:204
 constexpr double K1 = 16.0;                      // src/d3_kernel.cuh:17
 constexpr double K3 = 4.0;                       // src/d3_kernel.cuh:19
 
-enum Cut { CUT_CN = 0, CUT_2B = 1, CUT_ATM = 2 };
+enum Cut { CUT_CN = 0, CUT_2B = 1, CUT_ATM = 2, CUT_SUR = 3 };  // SUR: the three-body stand-in's features
+constexpr int NCUT = 4;
 
 struct StEntry {
     int dx, dy, dz, pad;
@@ -39,8 +40,8 @@ struct SysDev {
     int bin_base;    // global index of this cell's bin (0,0,0)
     int atom_base;   // first sorted (and caller) atom index of this cell
     int n_atoms;
-    int st_off[3];   // stencil ranges per cutoff
-    int st_cnt[3];
+    int st_off[NCUT];  // stencil ranges per cutoff
+    int st_cnt[NCUT];
 };
 
 // Device pointers and scalars of one evaluation; passed by value to every kernel.
@@ -90,6 +91,14 @@ struct Params {
     double *atm_acc;
     double *fa, *ga, *eas, *eat;
     int *atm_next;
+    // three-body stand-in (d3_sur.hip): class of each element type, features [N][64], the
+    // network's dE/dx [N][64] and energies (eV), class per sorted atom, fp32 parameters and
+    // packed bf16 weights
+    const int *type_cls;
+    float *feat, *sur_G, *sur_E;
+    int *sur_cls;
+    const float *sur_P;
+    const void *sur_PK;
     unsigned *status;
 };
 
@@ -113,6 +122,16 @@ template <class T>
 void launch_atm(const Params<T> &P, int nblocks, hipStream_t s);
 size_t atm_lds_bytes(int ne, int nref, size_t tsize);
 int atm_resident_blocks(int ne, int nref, size_t tsize);
+// P.f += fa, P.g += ga (what launch_atm ends with).
+template <class T>
+void launch_atm_merge(const Params<T> &P, hipStream_t s);
+
+// Three-body stand-in (d3_sur.hip): features only (training sets), or features, network and
+// backward writing fa / ga / eas / eat and the virial in place of the ATM kernel, then the merge.
+template <class T>
+void launch_sur_features(const Params<T> &P, hipStream_t s);
+template <class T>
+void launch_surrogate(const Params<T> &P, hipStream_t s);
 
 // Reduction of per-atom energy / virial / pair counts into per-cell totals (deterministic),
 // and scatter of per-atom results to caller order with unit conversion.

@@ -9,6 +9,8 @@
 #include <stdexcept>
 
 #include "../../include/d3x.h"
+#include "../../include/d3x_sur.h"
+#include "d3_sur.hpp"
 #include "d3_tables.hpp"
 
 namespace d3 {
@@ -198,6 +200,10 @@ Engine::Engine(const uint16_t *elements, uint64_t num_elements, uint64_t max_len
     alloc_atoms();
     d_status = dalloc(2 * sizeof(unsigned));  // status word | three-body centre counter
     HIPCHK(hipMemset(d_status, 0, 2 * sizeof(unsigned)));
+    std::vector<int> cls(ne_);
+    for (int A = 0; A < ne_; ++A) cls[A] = sur::element_class(elems_[A]);
+    d_type_cls = dalloc(ne_ * sizeof(int));
+    HIPCHK(hipMemcpy(d_type_cls, cls.data(), ne_ * sizeof(int), hipMemcpyHostToDevice));
 }
 
 Engine::~Engine() {
@@ -207,7 +213,8 @@ Engine::~Engine() {
                     &d_type_s, &d_orig_s, &d_bin_s, &d_sys_s, &d_cn, &d_w, &d_dw, &d_e, &d_g, &d_f, &d_vir,
                     &d_npair, &d_out_f, &d_out_e, &d_out_cn, &d_out_g, &d_bin_count, &d_bin_start, &d_cursor,
                     &d_sys, &d_st_d, &d_st_D, &d_chunk_b, &d_chunk_e, &d_chunk_s, &d_sys_chunk, &d_partial,
-                    &d_out_sys, &d_tab, &d_status, &d_atm_i32, &d_atm_T, &d_atm_acc, &d_atm_pa};
+                    &d_out_sys, &d_tab, &d_status, &d_atm_i32, &d_atm_T, &d_atm_acc, &d_atm_pa, &d_type_cls,
+                    &d_feat, &d_sur_G, &d_sur_E, &d_sur_cls, &d_sur_P, &d_sur_PK};
     for (void **p : all) dfree(*p);
     for (auto &e : ev_)
         if (e) (void)hipEventDestroy(e);
@@ -390,8 +397,9 @@ void Engine::build_geometry() {
         d.bin_base = bin_base;
         d.atom_base = atom_base;
         d.n_atoms = (int)C.n;
-        const double rcs[3] = {rccn_, rc2_, rc3_};
-        for (int cut = 0; cut < 3; ++cut) {
+        // the stand-in's feature stencil is built only when a mode needs it
+        const double rcs[NCUT] = {rccn_, rc2_, rc3_, atm_mode_ != 0 ? sur::RCF : 0.0};
+        for (int cut = 0; cut < NCUT; ++cut) {
             d.st_off[cut] = (int)st_d_.size();
             if (rcs[cut] > 0) make_stencil(C.H, d.nb, rcs[cut], st_d_, st_D_);
             d.st_cnt[cut] = (int)st_d_.size() - d.st_off[cut];
@@ -564,6 +572,13 @@ Params<T> Engine::params() {
     P.eat = P.eas + n_;
     P.status = (unsigned *)d_status;
     P.atm_next = (int *)d_status + 1;
+    P.type_cls = (const int *)d_type_cls;
+    P.feat = (float *)d_feat;
+    P.sur_G = (float *)d_sur_G;
+    P.sur_E = (float *)d_sur_E;
+    P.sur_cls = (int *)d_sur_cls;
+    P.sur_P = (const float *)d_sur_P;
+    P.sur_PK = d_sur_PK;
     return P;
 }
 
@@ -601,13 +616,15 @@ void Engine::run() {
         tables_dirty_ = false;
     }
     ensure_atm_scratch();
+    ensure_surrogate();
     Params<T> P = params<T>();
+    feat_valid_ = mode_now() != D3X_ATM_EXACT;
     if (graph_mode_ && !timing_) {
         // The graph holds every kernel argument and launch shape; re-capture when any changes.
-        std::vector<unsigned char> sig(sizeof(P) + 6 * sizeof(int64_t));
+        std::vector<unsigned char> sig(sizeof(P) + 7 * sizeof(int64_t));
         std::memcpy(sig.data(), &P, sizeof(P));
-        const int64_t extra[6] = {nbins_, atm_blocks_, (int64_t)chunk_b_.size(), (int64_t)sysdev_.size(),
-                                  (int64_t)(intptr_t)stream_, (int64_t)(intptr_t)d_tab};
+        const int64_t extra[7] = {nbins_, atm_blocks_, (int64_t)chunk_b_.size(), (int64_t)sysdev_.size(),
+                                  (int64_t)(intptr_t)stream_, (int64_t)(intptr_t)d_tab, mode_now()};
         std::memcpy(sig.data() + sizeof(P), extra, sizeof(extra));
         if (!gexec_ || sig != gsig_) {
             if (gexec_) (void)hipGraphExecDestroy(gexec_);
@@ -640,7 +657,12 @@ void Engine::enqueue(const Params<T> &P) {
     mark(2);
     Launch<T>::twobody(P, stream_);
     mark(3);
-    launch_atm<T>(P, atm_blocks_, stream_);  // zeroes the ATM arrays; kernels only when rc3 > 0
+    if (mode_now() == D3X_ATM_SURROGATE) {
+        launch_surrogate<T>(P, stream_);
+    } else {
+        launch_atm<T>(P, atm_blocks_, stream_);  // zeroes the ATM arrays; kernels only when rc3 > 0
+        if (mode_now() == D3X_ATM_EXACT_FEATURES) launch_sur_features<T>(P, stream_);
+    }
     mark(4);
     Launch<T>::chain(P, stream_);
     mark(5);
@@ -795,6 +817,81 @@ double Engine::autotune(const std::string &cache_path) {
     if (!path.empty()) cache_store(path, tune_key(), best_w, best_b, best);
     set_graph(graph);
     return best;
+}
+
+// ------------------------------------------------------------------------- three-body stand-in
+
+void Engine::sur_load(const float *params) {
+    if (!params) fail(D3X_ERR_INPUT, "NULL network parameters");
+    for (int k = 0; k < sur::NPARAMS; ++k)
+        if (!std::isfinite(params[k])) fail(D3X_ERR_INPUT, "network parameters contain non-finite values");
+    h_sur_P_.assign(params, params + sur::NPARAMS);
+    sur_dirty_ = true;
+}
+
+void Engine::set_atm_mode(int mode, int judge_every, double tol_ev_per_atom) {
+    if (mode != D3X_ATM_EXACT && mode != D3X_ATM_SURROGATE && mode != D3X_ATM_EXACT_FEATURES)
+        fail(D3X_ERR_INPUT, "unknown three-body mode");
+    if (mode != D3X_ATM_EXACT && rc3_ <= 0) fail(D3X_ERR_INPUT, "the handle has no three-body term (atm_cutoff <= 0)");
+    if (mode == D3X_ATM_SURROGATE && h_sur_P_.empty()) fail(D3X_ERR_STATE, "no network parameters loaded");
+    if (judge_every < 0) fail(D3X_ERR_INPUT, "judge_every must be >= 0");
+    if (!(tol_ev_per_atom > 0)) fail(D3X_ERR_INPUT, "tolerance must be positive");
+    if ((atm_mode_ != D3X_ATM_EXACT) != (mode != D3X_ATM_EXACT)) geom_dirty_ = true;  // feature stencil
+    atm_mode_ = mode;
+    judge_every_ = judge_every;
+    judge_tol_ = tol_ev_per_atom;
+    fallen_back_ = false;
+    sur_evals_ = 0;
+}
+
+void Engine::ensure_surrogate() {
+    if (atm_mode_ == D3X_ATM_EXACT) return;
+    if (!d_feat) {
+        d_feat = dalloc(max_len_ * sur::NFEAT * sizeof(float));
+        d_sur_G = dalloc(max_len_ * sur::NFEAT * sizeof(float));
+        d_sur_E = dalloc(max_len_ * sizeof(float));
+        d_sur_cls = dalloc(max_len_ * sizeof(int));
+    }
+    if (sur_dirty_ && !h_sur_P_.empty()) {
+        if (!d_sur_P) {
+            d_sur_P = dalloc(sur::NPARAMS * sizeof(float));
+            d_sur_PK = dalloc(sur::PACKED_BYTES);
+        }
+        HIPCHK(hipMemcpyAsync(d_sur_P, h_sur_P_.data(), sur::NPARAMS * sizeof(float), hipMemcpyHostToDevice, stream_));
+        sur::launch_pack((const float *)d_sur_P, d_sur_PK, stream_);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(stream_));
+        sur_dirty_ = false;
+    }
+}
+
+bool Engine::judge_due() const {
+    return mode_now() == D3X_ATM_SURROGATE && (sur_evals_ == 0 || (judge_every_ > 0 && sur_evals_ % judge_every_ == 0));
+}
+
+bool Engine::record_judgement(const double *e_net, const double *e_exact) {
+    double dev = 0;
+    for (size_t s = 0; s < sys_.size(); ++s)
+        if (sys_[s].n) dev = std::max(dev, std::fabs(e_net[s] - e_exact[s]) / (double)sys_[s].n);
+    last_dev_ = dev;
+    if (dev > judge_tol_) {
+        fallen_back_ = true;
+        ++fallbacks_;
+        return false;
+    }
+    return true;
+}
+
+void Engine::fetch_features(float *X) {
+    if (!feat_valid_ || !d_feat) fail(D3X_ERR_STATE, "no features: evaluate in a mode that computes them first");
+    if (n_ == 0) return;
+    std::vector<float> f(n_ * sur::NFEAT);
+    std::vector<int> orig(n_);
+    HIPCHK(hipMemcpyAsync(f.data(), d_feat, f.size() * sizeof(float), hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipMemcpyAsync(orig.data(), d_orig_s, n_ * sizeof(int), hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    for (uint64_t i = 0; i < n_; ++i)
+        std::memcpy(X + (size_t)orig[i] * sur::NFEAT, f.data() + i * sur::NFEAT, sur::NFEAT * sizeof(float));
 }
 
 void Engine::set_graph(bool on) {

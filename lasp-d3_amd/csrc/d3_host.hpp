@@ -82,6 +82,21 @@ class Engine {
     // Three-body grid of the next evaluation: the request, else the device's resident count.
     int atm_blocks_effective() { return atm_blocks_req_ > 0 ? atm_blocks_req_ : atm_resident(); }
 
+    // Three-body stand-in (modes of include/d3x_sur.h: 0 exact, 1 network with judge, 2 exact +
+    // features).  The judge itself runs in the C API's evaluation (it needs host results):
+    // judge_due() says whether this evaluation is judged, set_force_exact() runs the exact
+    // kernel once, record_judgement() compares per-cell energies and falls back on failure.
+    void sur_load(const float *params);
+    void set_atm_mode(int mode, int judge_every, double tol_ev_per_atom);
+    int mode_now() const { return (atm_mode_ == 1 && (fallen_back_ || force_exact_)) ? 0 : atm_mode_; }
+    bool judge_due() const;
+    void set_force_exact(bool on) { force_exact_ = on; }
+    bool record_judgement(const double *e_net, const double *e_exact);
+    void count_surrogate_eval() { ++sur_evals_; }
+    int fallbacks() const { return fallbacks_; }
+    double last_deviation() const { return last_dev_; }
+    void fetch_features(float *X);
+
    private:
     struct Cell {
         double H[9] = {0};  // bohr
@@ -98,7 +113,17 @@ class Engine {
     void upload_geometry();
     void alloc_atoms();
     void ensure_atm_scratch();
+    void ensure_surrogate();
     int type_of(uint16_t Z) const;
+
+    // three-body stand-in state
+    int atm_mode_ = 0, judge_every_ = 0, fallbacks_ = 0;
+    double judge_tol_ = 1e-3, last_dev_ = -1.0;
+    bool fallen_back_ = false, force_exact_ = false, sur_dirty_ = false, feat_valid_ = false;
+    uint64_t sur_evals_ = 0;
+    std::vector<float> h_sur_P_;
+    void *d_type_cls = nullptr, *d_feat = nullptr, *d_sur_G = nullptr, *d_sur_E = nullptr, *d_sur_cls = nullptr,
+         *d_sur_P = nullptr, *d_sur_PK = nullptr;
 
     // configuration
     std::vector<uint16_t> elems_;  // sorted unique atomic numbers of the handle

@@ -76,7 +76,25 @@ SIGNATURES = {
     "d3x_table_name": (C.c_char_p, []),
     "d3x_table_shape": (C.c_int, [_ip, _ip, _ip, _ip]),
     "d3x_table_export": (C.c_int, [_fp, _fp, _fp, _fp]),
+    # d3x_sur.h (device-buffer arguments are raw addresses: c_void_p)
+    "d3x_sur_load": (C.c_int, [_vp, _fp]),
+    "d3x_sur_load_file": (C.c_int, [_vp, C.c_char_p]),
+    "d3x_set_atm_mode": (C.c_int, [_vp, C.c_int, C.c_int, C.c_double]),
+    "d3x_sur_state": (C.c_int, [_vp, _ip, _ip, _dp]),
+    "d3x_sur_features": (C.c_int, [_vp, _fp]),
+    "d3x_sur_packed_bytes": (C.c_size_t, []),
+    "d3x_sur_train_bytes": (C.c_size_t, [_u64]),
+    "d3x_sur_pack": (C.c_int, [_vp, _vp, _vp]),
+    "d3x_sur_infer": (C.c_int, [_vp, _vp, _u64, _vp, _vp, _vp, _vp, _vp]),
+    "d3x_sur_train_grad": (C.c_int, [_vp, _vp, C.c_int, _u64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "d3x_sur_adam": (C.c_int, [_vp, _vp, _vp, _vp, _f, _f, _f, _f, C.c_int, _vp]),
 }
+
+D3X_ATM_EXACT = 0
+D3X_ATM_SURROGATE = 1
+D3X_ATM_EXACT_FEATURES = 2
+SUR_NPARAMS = 75020
+SUR_NFEAT = 64
 
 _lib = None
 
@@ -245,6 +263,31 @@ class Handle:
 
     def stream(self) -> int:
         return self.lib.d3x_get_stream(self.h) or 0
+
+    # ---- three-body stand-in (include/d3x_sur.h)
+    def sur_load(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        if p.size != SUR_NPARAMS:
+            raise ValueError(f"expected {SUR_NPARAMS} parameters, got {p.size}")
+        self._check(self.lib.d3x_sur_load(self.h, ptr(p, C.c_float)), "d3x_sur_load")
+
+    def sur_load_file(self, path: str):
+        self._check(self.lib.d3x_sur_load_file(self.h, path.encode()), "d3x_sur_load_file")
+
+    def set_atm_mode(self, mode: int, judge_every: int = 0, tol_ev_per_atom: float = 1e-3):
+        self._check(self.lib.d3x_set_atm_mode(self.h, int(mode), int(judge_every), float(tol_ev_per_atom)),
+                    "d3x_set_atm_mode")
+
+    def sur_state(self):
+        """(mode in use, fallback count, last judged |dE| / atoms in eV)"""
+        m, f, d = C.c_int(), C.c_int(), C.c_double()
+        self._check(self.lib.d3x_sur_state(self.h, C.byref(m), C.byref(f), C.byref(d)), "d3x_sur_state")
+        return m.value, f.value, d.value
+
+    def features(self) -> np.ndarray:
+        x = np.zeros((self.n, SUR_NFEAT), np.float32)
+        self._check(self.lib.d3x_sur_features(self.h, ptr(x, C.c_float)), "d3x_sur_features")
+        return x
 
     def status(self) -> int:
         return int(self.lib.d3x_last_status(self.h))

@@ -1,0 +1,242 @@
+"""GPU checks of the learned three-body stand-in: the MFMA network kernels against the CPU
+restatement with the kernels' bf16 roundings (oracle/surrogate_oracle.py mlp_bf16), Adam, a short
+training run, the features and the full stand-in energy / forces / stress of the engine against
+the oracle, and the judge / fallback behaviour."""
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import d3_lib
+import surrogate_oracle as SO
+import systems
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def dev(a, dtype=torch.float32):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device="cuda")
+
+
+def vp(t):
+    return C.c_void_p(t.data_ptr())
+
+
+def stream():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def random_params(seed=1):
+    rng = np.random.default_rng(seed)
+    p = SO.init_params(seed, mean=rng.normal(0.5, 0.2, SO.N_FEAT), istd=rng.uniform(0.5, 2, SO.N_FEAT),
+                       scale=[0.01, 0.02, 0.03], shift=[0.001, -0.002, 0.003])
+    for c in range(SO.N_CLASS):
+        o = SO.HEAD + c * SO.CLASS_SIZE
+        p[o + SO.O_B1:o + SO.O_W2] = rng.normal(0, 0.3, SO.N_HID)
+        p[o + SO.O_B2:o + SO.O_W3] = rng.normal(0, 0.3, SO.N_HID)
+        p[o + SO.O_TAIL] = 0.1
+    return p.astype(np.float32)
+
+
+def packed(lib, P):
+    pk = torch.empty(lib.d3x_sur_packed_bytes(), dtype=torch.uint8, device="cuda")
+    assert lib.d3x_sur_pack(vp(P), vp(pk), stream()) == 0
+    return pk
+
+
+def test_infer_matches_bf16_oracle():
+    lib = d3_lib.load()
+    rng = np.random.default_rng(3)
+    n = 1000
+    X = rng.normal(0.5, 0.6, (n, SO.N_FEAT)).astype(np.float32)
+    cls = rng.integers(0, 3, n).astype(np.int32)
+    p = random_params()
+    print("case sur-infer n=1000", flush=True)
+    P = dev(p)
+    pk = packed(lib, P)
+    E = torch.zeros(n, device="cuda")
+    G = torch.zeros((n, SO.N_FEAT), device="cuda")
+    assert lib.d3x_sur_infer(vp(P), vp(pk), n, vp(dev(X)), vp(dev(cls, torch.int32)), vp(E), vp(G), stream()) == 0
+    torch.cuda.synchronize()
+    Eo, Go, _ = SO.mlp_bf16(p, X, cls)
+    Eg, Gg = E.cpu().numpy(), G.cpu().numpy()
+    # fp32 accumulation and a rare bf16 rounding flip of an activation: 1e-3 of the spread
+    shift = np.array([SO.class_block(p.astype(np.float64), c)["shift"] for c in range(3)])[cls]
+    assert np.max(np.abs(Eg - Eo)) <= 1e-3 * np.max(np.abs(Eo - shift))
+    assert np.max(np.abs(Gg - Go)) <= 1e-2 * np.max(np.abs(Go))
+    # and most entries much closer
+    assert np.median(np.abs(Gg - Go)) <= 1e-4 * np.max(np.abs(Go))
+
+
+def test_train_gradient_matches_bf16_oracle():
+    lib = d3_lib.load()
+    rng = np.random.default_rng(4)
+    p = random_params(seed=7)
+    P = dev(p)
+    pk = packed(lib, P)
+    grad = torch.zeros(d3_lib.SUR_NPARAMS, device="cuda")
+    loss = torch.zeros(1, device="cuda")
+    g_ref = np.zeros(d3_lib.SUR_NPARAMS)
+    l_ref = 0.0
+    for c, n in ((0, 333), (1, 1000), (2, 77)):
+        print(f"case sur-train class={c} n={n}", flush=True)
+        X = rng.normal(0.5, 0.6, (n, SO.N_FEAT)).astype(np.float32)
+        y = rng.normal(0.0, 0.01, n).astype(np.float32)
+        w = rng.uniform(0.5, 1.5, n).astype(np.float32) / n
+        cls = np.full(n, c)
+        E = torch.zeros(n, device="cuda")
+        work = torch.empty(lib.d3x_sur_train_bytes(n), dtype=torch.uint8, device="cuda")
+        assert lib.d3x_sur_train_grad(vp(P), vp(pk), c, n, vp(dev(X)), vp(dev(y)), vp(dev(w)), vp(E), vp(work),
+                                      vp(grad), vp(loss), stream()) == 0
+        Eo, _, _ = SO.mlp_bf16(p, X, cls)
+        dl = w * (Eo - y)
+        _, _, g = SO.mlp_bf16(p, X, cls, dl=dl)
+        g_ref += g
+        l_ref += np.sum(0.5 * w * (Eo - y) ** 2)
+    torch.cuda.synchronize()
+    gg = grad.cpu().numpy().astype(np.float64)
+    assert abs(loss.item() - l_ref) <= 1e-4 * l_ref
+    for c in range(3):
+        o = SO.HEAD + c * SO.CLASS_SIZE
+        for a, b in ((SO.O_W1, SO.O_B1), (SO.O_B1, SO.O_W2), (SO.O_W2, SO.O_B2), (SO.O_B2, SO.O_W3),
+                     (SO.O_W3, SO.O_TAIL), (SO.O_TAIL, SO.O_TAIL + 1)):
+            ref, got = g_ref[o + a:o + b], gg[o + a:o + b]
+            assert np.max(np.abs(got - ref)) <= 2e-2 * np.max(np.abs(ref)) + 1e-12, (c, a)
+            assert np.median(np.abs(got - ref)) <= 2e-3 * np.max(np.abs(ref)) + 1e-12, (c, a)
+    assert np.all(gg[:SO.HEAD] == 0)
+
+
+def mixed_cell(seed=21, n=48):
+    s = systems.triclinic_mixed(n=n, seed=seed)
+    return s["pos"], s["Z"], s["cell"]
+
+
+RC = dict(cutoff=50.0, cn_cutoff=30.0, atm_cutoff=30.0, damping=1, functional=0)
+
+
+def engine(Z, n, precision=d3_lib.D3X_FP64):
+    return d3_lib.Handle(np.unique(Z), n, precision=precision, **RC)
+
+
+@pytest.mark.parametrize("precision", [d3_lib.D3X_FP64, d3_lib.D3X_FP32])
+def test_features_match_oracle(table, precision):
+    pos, Z, cell = mixed_cell()
+    print(f"case sur-features precision={precision}", flush=True)
+    h = engine(Z, len(Z), precision)
+    h.set_atm_mode(d3_lib.D3X_ATM_EXACT_FEATURES)
+    h.set_atoms(pos, Z)
+    h.set_cell(cell)
+    h.compute()
+    X = h.features().astype(np.float64)
+    h.close()
+    rcov = np.asarray(table["rcov"], np.float64)
+    cn, _ = SO.coordination(pos / SO.BOHR, Z, cell / SO.BOHR, rcov, RC["cn_cutoff"])
+    Xo, _ = SO.features(pos / SO.BOHR, Z, cell / SO.BOHR, cn)
+    np.testing.assert_allclose(X[:, 0], Xo[:, 0], rtol=2e-6, atol=1e-6)
+    np.testing.assert_allclose(X[:, 1:], Xo[:, 1:], rtol=1e-5, atol=1e-5 * np.max(Xo[:, 1:]))
+
+
+def test_surrogate_energy_forces_stress_match_oracle(table):
+    """The engine with the network in place of ATM = exact two-body (D3 oracle, no ATM) + the
+    stand-in of the CPU restatement with the kernels' bf16 roundings."""
+    import d3_oracle
+    pos, Z, cell = mixed_cell(seed=22)
+    p = random_params(seed=13)
+    print("case sur-engine mixed48", flush=True)
+    h = engine(Z, len(Z))
+    h.sur_load(p)
+    h.set_atm_mode(d3_lib.D3X_ATM_SURROGATE, 0, 1e9)  # the judge always accepts
+    h.set_atoms(pos, Z)
+    h.set_cell(cell)
+    e, f, s = h.compute()
+    mode, fb, dev_ = h.sur_state()
+    h.close()
+    assert mode == d3_lib.D3X_ATM_SURROGATE and fb == 0 and dev_ >= 0
+    o2 = d3_oracle.compute(pos, Z, cell, table, cutoff=RC["cutoff"], cn_cutoff=RC["cn_cutoff"], atm_cutoff=0.0,
+                           damping=1, functional=0)
+    so = SO.evaluate(p.astype(np.float64), pos, Z, cell, np.asarray(table["rcov"], np.float64), RC["cn_cutoff"],
+                     bf16_net=True)
+    assert abs((e - o2["energy"]) - so["energy"]) <= 1e-4 * np.sum(np.abs(so["atom_energy"])) + 1e-8
+    df = f - o2["forces"]
+    assert np.max(np.abs(df - so["forces"])) <= 2e-3 * np.max(np.abs(so["forces"])) + 1e-8
+    ds = s - o2["stress"]
+    assert np.max(np.abs(ds - so["stress"])) <= 2e-3 * np.max(np.abs(so["stress"])) + 1e-10
+
+
+def test_judge_accepts_or_falls_back():
+    pos, Z, cell = mixed_cell(seed=23)
+    p = random_params(seed=14)
+    print("case sur-judge", flush=True)
+    h = engine(Z, len(Z))
+    h.set_atoms(pos, Z)
+    h.set_cell(cell)
+    e_exact, f_exact, s_exact = h.compute()
+    h.sur_load(p)
+    # a network that cannot match to 1e-9 eV/atom: the judged first evaluation falls back
+    h.set_atm_mode(d3_lib.D3X_ATM_SURROGATE, 0, 1e-9)
+    e1, f1, s1 = h.compute()
+    mode, fb, dev_ = h.sur_state()
+    assert mode == d3_lib.D3X_ATM_EXACT and fb == 1 and dev_ > 1e-9
+    assert abs(e1 - e_exact) <= 1e-12 * abs(e_exact)
+    assert np.max(np.abs(f1 - f_exact)) <= 1e-10
+    e2, _, _ = h.compute()  # stays on the exact kernel
+    assert abs(e2 - e_exact) <= 1e-12 * abs(e_exact) and h.sur_state()[1] == 1
+    # a generous tolerance: the network's results come back, judged every second evaluation
+    h.set_atm_mode(d3_lib.D3X_ATM_SURROGATE, 2, 1e9)
+    e3, _, _ = h.compute()
+    e4, _, _ = h.compute()
+    assert h.sur_state()[0] == d3_lib.D3X_ATM_SURROGATE
+    assert abs(e3 - e4) <= 1e-9 * abs(e3) and abs(e3 - e_exact) > 1e-9 * abs(e_exact)
+    assert abs(h.sur_state()[2] - abs(e3 - e_exact) / len(Z)) <= 1e-6 * abs(e3 - e_exact) / len(Z) + 1e-12
+    h.close()
+
+
+def test_adam_step_and_training_reduces_loss():
+    lib = d3_lib.load()
+    rng = np.random.default_rng(5)
+    # one Adam step against numpy
+    p = random_params(seed=9)
+    g = rng.normal(0, 1e-3, d3_lib.SUR_NPARAMS).astype(np.float32)
+    P, Gd = dev(p), dev(g)
+    m, v = torch.zeros_like(P), torch.zeros_like(P)
+    assert lib.d3x_sur_adam(vp(P), vp(Gd), vp(m), vp(v), C.c_float(1e-3), C.c_float(0.9), C.c_float(0.999),
+                            C.c_float(1e-8), 1, stream()) == 0
+    torch.cuda.synchronize()
+    trainable = np.zeros(d3_lib.SUR_NPARAMS, bool)
+    for c in range(3):
+        o = SO.HEAD + c * SO.CLASS_SIZE
+        trainable[o:o + SO.O_TAIL + 1] = True
+    mm, vv = 0.1 * g, 0.001 * g.astype(np.float64) ** 2
+    ref = p - np.where(trainable, 1e-3 * (mm / 0.1) / (np.sqrt(vv / 0.001) + 1e-8), 0)
+    np.testing.assert_allclose(P.cpu().numpy(), ref, rtol=1e-5, atol=1e-7)
+    assert np.all(Gd.cpu().numpy() == 0)
+    # a short fit of a smooth target: the loss falls by 10x
+    print("case sur-train-fit", flush=True)
+    n = 4096
+    X = rng.normal(0.0, 1.0, (n, SO.N_FEAT)).astype(np.float32)
+    y = (0.01 * np.tanh(X[:, 0] + 0.5 * X[:, 5] - 0.3 * X[:, 40])).astype(np.float32)
+    w = np.full(n, 1.0 / n, np.float32)
+    p = SO.init_params(3, scale=[0.01, 0.01, 0.01]).astype(np.float32)
+    P = dev(p)
+    m, v = torch.zeros_like(P), torch.zeros_like(P)
+    grad = torch.zeros_like(P)
+    loss = torch.zeros(1, device="cuda")
+    pk = torch.empty(lib.d3x_sur_packed_bytes(), dtype=torch.uint8, device="cuda")
+    work = torch.empty(lib.d3x_sur_train_bytes(n), dtype=torch.uint8, device="cuda")
+    E = torch.zeros(n, device="cuda")
+    Xd, yd, wd = dev(X), dev(y), dev(w)
+    losses = []
+    for step in range(1, 301):
+        loss.zero_()
+        assert lib.d3x_sur_pack(vp(P), vp(pk), stream()) == 0
+        assert lib.d3x_sur_train_grad(vp(P), vp(pk), 1, n, vp(Xd), vp(yd), vp(wd), vp(E), vp(work), vp(grad), vp(loss),
+                                      stream()) == 0
+        if step in (1, 300):
+            torch.cuda.synchronize()
+            losses.append(loss.item())
+        assert lib.d3x_sur_adam(vp(P), vp(grad), vp(m), vp(v), C.c_float(3e-3), C.c_float(0.9), C.c_float(0.999),
+                                C.c_float(1e-8), step, stream()) == 0
+    assert losses[1] < 0.1 * losses[0], losses

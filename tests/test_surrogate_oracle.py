@@ -1,0 +1,156 @@
+"""Pins of the CPU restatement of the learned three-body stand-in (oracle/surrogate_oracle.py):
+its CN equals the D3 oracle's, its input and parameter gradients equal finite differences of its
+forward pass, its forces and stress equal finite differences of its energy, and the features
+have the closed-form values of isolated pairs.  CPU only."""
+
+import numpy as np
+import pytest
+
+import d3_lib
+import d3_oracle
+import surrogate_oracle as SO
+import systems
+
+
+@pytest.fixture(scope="module")
+def rcov(table):
+    return np.asarray(table["rcov"], np.float64)
+
+
+def small(seed=3, n=12):
+    s = systems.triclinic_mixed(n=n, seed=seed)
+    return s["pos"], s["Z"], s["cell"]
+
+
+def random_params(seed=1, n=20):
+    rng = np.random.default_rng(seed)
+    p = SO.init_params(seed, mean=rng.normal(0, 1, SO.N_FEAT), istd=rng.uniform(0.5, 2, SO.N_FEAT),
+                       scale=[0.01, 0.02, 0.03], shift=[0.001, -0.002, 0.003])
+    for c in range(SO.N_CLASS):  # non-zero biases so that every term is exercised
+        o = SO.HEAD + c * SO.CLASS_SIZE
+        p[o + SO.O_B1:o + SO.O_W2] = rng.normal(0, 0.3, SO.N_HID)
+        p[o + SO.O_B2:o + SO.O_W3] = rng.normal(0, 0.3, SO.N_HID)
+        p[o + SO.O_TAIL] = 0.1
+    return p
+
+
+def test_cn_matches_d3_oracle(table, rcov):
+    pos, Z, cell = small()
+    o = d3_oracle.compute(pos, Z, cell, table, cutoff=30.0, cn_cutoff=30.0, atm_cutoff=0.0, damping=1, functional=0)
+    cn, _ = SO.coordination(pos / SO.BOHR, Z, cell / SO.BOHR, rcov, 30.0)
+    np.testing.assert_allclose(cn, o["cn"], rtol=1e-12, atol=1e-12)
+
+
+def test_isolated_pair_features():
+    """Two atoms (H and O) far from every image: each sees one neighbour at r."""
+    cell = np.eye(3) * 40.0
+    r = 2.3
+    pos = np.array([[1.0, 1.0, 1.0], [1.0 + r, 1.0, 1.0]])
+    Z = np.array([1, 8])
+    X, _ = SO.features(pos / SO.BOHR, Z, cell / SO.BOHR, np.zeros(2))
+    rb = r / SO.BOHR
+    fc = 0.5 * (1 + np.cos(np.pi * rb / SO.RC_F))
+    phi = np.exp(-(rb - SO.CENTRES) ** 2 / (2 * SO.SIGMA ** 2)) * fc
+    np.testing.assert_allclose(X[0, 1 + 21:1 + 42], phi, rtol=1e-13)  # H sees O (class 1)
+    np.testing.assert_allclose(X[1, 1:22], phi, rtol=1e-13)           # O sees H (class 0)
+    assert np.count_nonzero(X[0, 1:]) == 21 and np.count_nonzero(X[1, 1:]) == 21
+
+
+def test_input_gradient_matches_fd():
+    p = random_params()
+    rng = np.random.default_rng(4)
+    X = rng.normal(0, 1, (9, SO.N_FEAT))
+    cls = np.array([0, 1, 2, 0, 1, 2, 2, 1, 0])
+    E, cache = SO.mlp_forward(p, X, cls)
+    G = SO.mlp_input_grad(p, cache, len(X))
+    h = 1e-6
+    for a in range(len(X)):
+        for f in (0, 1, 17, 40, 63):
+            Xp, Xm = X.copy(), X.copy()
+            Xp[a, f] += h
+            Xm[a, f] -= h
+            fd = (SO.mlp_forward(p, Xp, cls)[0][a] - SO.mlp_forward(p, Xm, cls)[0][a]) / (2 * h)
+            assert abs(fd - G[a, f]) <= 1e-7 * max(1.0, abs(G[a, f]))
+
+
+def test_param_gradient_matches_fd():
+    p = random_params(seed=5)
+    rng = np.random.default_rng(6)
+    X = rng.normal(0, 1, (7, SO.N_FEAT))
+    cls = np.array([0, 1, 2, 2, 1, 0, 2])
+    y = rng.normal(0, 0.01, len(X))
+
+    def loss(q):
+        return 0.5 * np.sum((SO.mlp_forward(q, X, cls)[0] - y) ** 2)
+
+    E, cache = SO.mlp_forward(p, X, cls)
+    g = SO.mlp_param_grad(p, cache, E - y)
+    h = 1e-6
+    idx = []
+    for c in range(SO.N_CLASS):
+        o = SO.HEAD + c * SO.CLASS_SIZE
+        idx += [o + SO.O_W1 + 5, o + SO.O_W1 + 64 * 128 - 3, o + SO.O_B1 + 7, o + SO.O_W2 + 300, o + SO.O_B2 + 1,
+                o + SO.O_W3 + 9, o + SO.O_TAIL]
+    for k in idx:
+        pp, pm = p.copy(), p.copy()
+        pp[k] += h
+        pm[k] -= h
+        fd = (loss(pp) - loss(pm)) / (2 * h)
+        assert abs(fd - g[k]) <= 1e-6 * max(1e-3, abs(g[k])), k
+    # normalisation and output-scale entries are not trained
+    assert np.all(g[:SO.HEAD] == 0)
+    for c in range(SO.N_CLASS):
+        o = SO.HEAD + c * SO.CLASS_SIZE + SO.O_TAIL
+        assert g[o + 1] == 0 and g[o + 2] == 0 and g[o + 3] == 0
+
+
+def test_forces_and_stress_match_fd(rcov):
+    pos, Z, cell = small(seed=8, n=10)
+    p = random_params(seed=2)
+    o = SO.evaluate(p, pos, Z, cell, rcov, 25.0)
+    h = 1e-5
+    for a in (0, 3, 7):
+        for k in range(3):
+            pp, pm = pos.copy(), pos.copy()
+            pp[a, k] += h
+            pm[a, k] -= h
+            fd = -(SO.evaluate(p, pp, Z, cell, rcov, 25.0)["energy"] - SO.evaluate(p, pm, Z, cell, rcov, 25.0)["energy"]) / (2 * h)
+            assert abs(fd - o["forces"][a, k]) <= 1e-6 * max(1.0, np.max(np.abs(o["forces"])))
+    V = abs(np.linalg.det(cell))
+    # the D3 CN has a hard cutoff (a pair crossing it moves CN by ~1e-6): a strain step small
+    # enough that no pair crosses it in this cell
+    h = 1e-7
+    for a, b in ((0, 0), (1, 2), (2, 1)):
+        eps = np.zeros((3, 3))
+        eps[a, b] = h
+        ep = SO.evaluate(p, pos @ (np.eye(3) + eps), Z, cell @ (np.eye(3) + eps), rcov, 25.0)["energy"]
+        em = SO.evaluate(p, pos @ (np.eye(3) - eps), Z, cell @ (np.eye(3) - eps), rcov, 25.0)["energy"]
+        assert abs((ep - em) / (2 * h) / V - o["stress"][a, b]) <= 1e-6 * max(1e-3, np.max(np.abs(o["stress"])))
+    # translation invariance: forces sum to zero
+    assert np.max(np.abs(o["forces"].sum(0))) <= 1e-10 * max(1.0, np.max(np.abs(o["forces"])))
+
+
+def test_bf16_rounding_and_emulated_network():
+    """bf16(): nearest with ties to even; the bf16-rounded network stays within bf16 accuracy of
+    the float64 one."""
+    one = 1.0
+    assert SO.bf16(one)[()] == 1.0
+    assert SO.bf16(1 + 2 ** -8)[()] == 1.0                      # tie -> even (1.0)
+    assert SO.bf16(1 + 3 * 2 ** -8)[()] == 1 + 2 ** -6          # tie -> even (upper)
+    assert SO.bf16(1 + 2 ** -8 + 2 ** -12)[()] == 1 + 2 ** -7   # above the tie
+    assert SO.bf16(-3.0)[()] == -3.0
+    p = random_params(seed=11)
+    rng = np.random.default_rng(12)
+    X = rng.normal(0, 1, (50, SO.N_FEAT))
+    cls = rng.integers(0, 3, 50)
+    E64, cache = SO.mlp_forward(p, X, cls)
+    G64 = SO.mlp_input_grad(p, cache, 50)
+    E16, G16, _ = SO.mlp_bf16(p, X, cls)
+    shift = np.array([SO.class_block(p, c)["shift"] for c in range(3)])[cls]
+    assert np.max(np.abs(E16 - E64)) <= 3e-2 * np.max(np.abs(E64 - shift))
+    assert np.max(np.abs(G16 - G64)) <= 5e-2 * np.max(np.abs(G64))
+    # the emulated parameter gradient equals the float64 one to bf16 accuracy
+    dl = rng.normal(0, 1, 50)
+    _, _, g16 = SO.mlp_bf16(p, X, cls, dl=dl)
+    g64 = SO.mlp_param_grad(p, cache, dl)
+    assert np.max(np.abs(g16 - g64)) <= 5e-2 * np.max(np.abs(g64))
