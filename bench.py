@@ -136,6 +136,47 @@ def phase_profile(h, runs=2):
     return {k: v / max(n, 1) for k, v in ph.items()}
 
 
+def hbm_used_gb(ev):
+    """Device memory in use by this process and the runtime, from hipMemGetInfo (GB)."""
+    free, total = C.c_size_t(), C.c_size_t()
+    if ev.hip.hipMemGetInfo(C.byref(free), C.byref(total)) != 0:
+        return None
+    return (total.value - free.value) / 1e9
+
+
+SUR_WEIGHTS = os.path.join(ROOT, "lasp-d3_amd", "surrogate", "atm_surrogate.d3sur")
+
+
+def surrogate_leg():
+    """The three-body network against the exact ATM kernel (FP32 handles, same cutoffs), on the
+    water box (its training distribution; the box itself is not in the training set) and on the
+    100,000-atom random H/C/N/O/Si/Cu cell at 0.08 atoms/A^3: three-body phase time (HIP events,
+    average of 3 evaluations), whole-step time, the judge's |dE| / atoms against the exact kernel,
+    and whether it fell back."""
+    cases = []
+    for name, (pos, Z, cell) in (("water-4096H2O", S.water_box(4096, seed=0)),
+                                  ("random-100000", S.random_cell(100000, seed=4242))):
+        n = len(Z)
+        print(f"[bench] surrogate leg {name}", file=sys.stderr, flush=True)
+        h = d3_lib.Handle(np.unique(Z), n, cutoff=RC2, cn_cutoff=RCCN, atm_cutoff=RC3, damping=1, functional=0,
+                          precision=d3_lib.D3X_FP32)
+        h.set_atoms(pos, Z)
+        h.set_cell(cell)
+        e_x = h.compute()[0]
+        px = phase_profile(h, runs=3)
+        h.sur_load_file(SUR_WEIGHTS)
+        h.set_atm_mode(d3_lib.D3X_ATM_SURROGATE, 0, 1e-3)
+        e_n = h.compute()[0]  # judged against the exact kernel
+        mode, fb, dev = h.sur_state()
+        pn = phase_profile(h, runs=3)
+        h.close()
+        cases.append(dict(workload=name, atoms=n, atm_ms_exact=px["atm"], atm_ms_network=pn["atm"],
+                          atm_speedup=px["atm"] / max(pn["atm"], 1e-9), step_ms_exact=sum(px.values()),
+                          step_ms_network=sum(pn.values()), judge_dev_mev_per_atom=1e3 * dev,
+                          fell_back=bool(fb), mode_after=mode, e_exact_ev=e_x, e_returned_ev=e_n))
+    return dict(weights=os.path.relpath(SUR_WEIGHTS, ROOT), tolerance_mev_per_atom=1.0, cases=cases)
+
+
 def cpu_baseline(w, table, cn, g, target_s=15.0):
     """FP64 oracle on a bounded random sample of centre atoms, all host threads of this job."""
     import d3_oracle
@@ -164,6 +205,7 @@ def main():
     ap.add_argument("--workload", default="cu1m", choices=["water", "si512", "cu1m", "batch"])
     ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the three-body network leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
@@ -204,10 +246,16 @@ def main():
     achieved = flops[dom] / (prof32[dom] * 1e-3) / 1e12
     total_flops = sum(flops.values())
 
+    hbm = hbm_used_gb(ev)
+    h32.close()
+    h64.close()
+
     cpu = None
     if not args.no_cpu and w["kind"] == "single":
         import d3_oracle  # noqa: F401
         cpu = cpu_baseline(w, d3_lib.table(), cn64, g64)
+
+    sur = None if args.no_surrogate else surrogate_leg()
 
     out = {
         "metric": METRIC,
@@ -238,6 +286,8 @@ def main():
                      "achieved": achieved, "peak": PEAK_FP32, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32,
                      "traffic": None},
         "cpu_baseline": cpu,
+        "hbm_used_gb": hbm,
+        "surrogate": sur,
     }
     print(json.dumps(out))
 

@@ -1,0 +1,79 @@
+"""What the trained three-body network (lasp-d3_amd/surrogate/atm_surrogate.d3sur, written by
+lasp-d3_amd/surrogate/train.py) gains over the method without it, on cells it was not trained on:
+
+* energy: with the network the error against full D3 (two-body + exact ATM) is far inside
+  1 meV/atom, while dropping the three-body term (the method without the network) misses by the
+  whole ATM energy (~8-15 meV/atom here);
+* stress: the network removes most of the stress error of dropping the term;
+* forces (water): smaller error than dropping the term.  (The Si cell's three-body forces are
+  ~4e-5 eV/A; the network's force error there is larger than that, and the test does not claim a
+  gain for them.)
+* time: the network's three-body phase is >= 5x faster than the exact kernel's;
+* out of distribution (a random H/C/N/O/Si/Cu cell; C, N, Cu are not in the training set) the
+  judge sees more than 1 meV/atom, falls back and the exact results come back.
+"""
+
+import os
+
+import numpy as np
+import pytest
+
+import d3_lib
+import d3_structures as S
+
+pytestmark = pytest.mark.gpu
+
+WEIGHTS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "lasp-d3_amd", "surrogate",
+                       "atm_surrogate.d3sur")
+RC = dict(cutoff=float(np.sqrt(9000.0)), cn_cutoff=40.0, damping=1, functional=0, precision=d3_lib.D3X_FP32)
+
+
+def run(pos, Z, cell, atm, mode=None, tol=1e-3):
+    h = d3_lib.Handle(np.unique(Z), len(Z), atm_cutoff=atm, **RC)
+    h.set_atoms(pos, Z)
+    h.set_cell(cell)
+    if mode is not None:
+        h.sur_load_file(WEIGHTS)
+        h.set_atm_mode(mode, 0, tol)
+        h.compute()  # judged
+    h.set_timing(True)
+    e, f, s = h.compute()
+    t = h.phase_times()[0]["atm"]
+    state = h.sur_state()
+    h.close()
+    return e, f, s, t, state
+
+
+@pytest.mark.parametrize("kind", ["water", "si"])
+def test_network_gain_on_unseen_cells(kind):
+    if kind == "water":
+        pos, Z, cell = S.water_box(4096, seed=2999)  # the training boxes use seeds 2000..2024
+    else:
+        pos, Z, cell = S.si_diamond(4, jitter=0.02, seed=9999)  # training: 1000..1999
+    n = len(Z)
+    print(f"case gain {kind} atoms={n}", flush=True)
+    e_full, f_full, s_full, t_exact, _ = run(pos, Z, cell, 40.0)
+    e_2b, f_2b, s_2b, _, _ = run(pos, Z, cell, 0.0)
+    e_net, f_net, s_net, t_net, (mode, fb, dev) = run(pos, Z, cell, 40.0, d3_lib.D3X_ATM_SURROGATE)
+    assert mode == d3_lib.D3X_ATM_SURROGATE and fb == 0 and dev <= 1e-3
+    err_net, err_2b = abs(e_net - e_full) / n, abs(e_2b - e_full) / n
+    print(f"  energy error per atom: network {1e3 * err_net:.4f} meV, without {1e3 * err_2b:.3f} meV; "
+          f"ATM phase {t_exact:.2f} ms exact vs {t_net:.3f} ms network", flush=True)
+    assert err_net <= 1e-3 and err_2b >= 5e-3 and err_net <= 0.05 * err_2b
+    sn, s2 = np.max(np.abs(s_net - s_full)), np.max(np.abs(s_2b - s_full))
+    assert sn <= 0.2 * s2
+    if kind == "water":
+        fn, f2 = np.sqrt(np.mean((f_net - f_full) ** 2)), np.sqrt(np.mean((f_2b - f_full) ** 2))
+        assert fn < f2
+    assert t_exact >= 5 * t_net
+
+
+def test_out_of_distribution_cell_falls_back():
+    pos, Z, cell = S.random_cell(20000, seed=77)
+    print("case gain random20000 (fallback)", flush=True)
+    e_full, f_full, s_full, _, _ = run(pos, Z, cell, 40.0)
+    e_net, f_net, s_net, _, (mode, fb, dev) = run(pos, Z, cell, 40.0, d3_lib.D3X_ATM_SURROGATE)
+    assert fb == 1 and mode == d3_lib.D3X_ATM_EXACT and dev > 1e-3
+    # after the fallback the handle returns the exact kernel's results
+    assert abs(e_net - e_full) <= 1e-6 * abs(e_full)
+    assert np.max(np.abs(f_net - f_full)) <= 1e-5
