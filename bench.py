@@ -147,6 +147,19 @@ def hbm_used_gb(ev):
 SUR_WEIGHTS = os.path.join(ROOT, "lasp-d3_amd", "surrogate", "atm_surrogate.d3sur")
 
 
+def profiled_traffic(kernel: str, workload_desc: str):
+    """HBM bytes per launch of `kernel` on the workload, from the counter summary committed under
+    profiles/ (rocprofv3 FETCH_SIZE and WRITE_SIZE runs scaled by a plain copy of known size):
+    the line 'traffic_bytes_per_launch: X' of profiles/<kernel>_<workload>_traffic.md, or None."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload_desc}_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
 def surrogate_leg():
     """The three-body network against the exact ATM kernel (FP32 handles, same cutoffs), on the
     water box (its training distribution; the box itself is not in the training set) and on the
@@ -206,18 +219,37 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the three-body network leg")
+    ap.add_argument("--only-surrogate", action="store_true",
+                    help="run only the three-body network leg and print its JSON (for counter runs)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only copy BYTES device-to-device three times with a plain copy kernel "
+                         "(reference for scaling FETCH_SIZE / WRITE_SIZE counter readings)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
               file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        import torch  # a plain device copy kernel of known size
+        x = torch.ones(args.copy_calibration // 4, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": args.copy_calibration, "copies": 3}))
+        return
+    if args.only_surrogate:
+        print(json.dumps({"surrogate": surrogate_leg()}))
+        return
 
     w = workload(args.workload)
     n = len(w["Z"])
     ev = HipEvents()
     print(f"[bench] workload {w['desc']} atoms={n}", file=sys.stderr, flush=True)
 
+    hbm0 = hbm_used_gb(ev)
     h32 = make_handle(w, d3_lib.D3X_FP32)
+    hbm32 = hbm_used_gb(ev)
     ms32, wall32 = time_steps(h32, ev, args.steps, args.warmup)
     e32, f32, s32 = fetch(h32, w)
     print(f"[bench] fp32 {ms32:.2f} ms/step", file=sys.stderr, flush=True)
@@ -246,7 +278,9 @@ def main():
     achieved = flops[dom] / (prof32[dom] * 1e-3) / 1e12
     total_flops = sum(flops.values())
 
-    hbm = hbm_used_gb(ev)
+    hbm = dict(fp32_handle_gb=None if hbm0 is None or hbm32 is None else hbm32 - hbm0,
+               both_handles_gb=None if hbm0 is None else hbm_used_gb(ev) - hbm0,
+               note="hipMemGetInfo deltas: the FP32 handle after its first evaluation, then with the FP64 handle too")
     h32.close()
     h64.close()
 
@@ -284,9 +318,10 @@ def main():
         "roofline": {"bound": "valu", "kernel": {"atm": "k_atm<float>", "two_body": "k_2b<float>",
                                                   "cn": "k_cn<float>", "chain": "k_chain<float>"}[dom],
                      "achieved": achieved, "peak": PEAK_FP32, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32,
-                     "traffic": None},
+                     "traffic": profiled_traffic({"atm": "k_atm_f32", "two_body": "k_2b_f32", "cn": "k_cn_f32",
+                                                  "chain": "k_chain_f32"}[dom], w["desc"])},
         "cpu_baseline": cpu,
-        "hbm_used_gb": hbm,
+        "hbm_footprint": hbm,
         "surrogate": sur,
     }
     print(json.dumps(out))
