@@ -88,12 +88,13 @@ template <class T>
 struct AtmLds {
     T2<T> *J0, *J1, *J2, *J3, *Jyd;  // j tile: (dx, dy) (dz, a) (1/a, a^-1/2) (sqrt C6_ij, lam); rows (y, dy)
     T *Jacc;                         // [A_NF][64] j-side tile sums
+    T *Khi, *Klo;                    // [A_NF][64] k-side sums over the j tiles (two-sum pair in FP32)
     T *Cy, *Cdy;                     // centre rows [ne * nref]
     T *Rt;                           // per type pair: FP32 (8/3) log2 R0^2, FP64 R0^2
     int *Jt;                         // j type x ne [64]
     __host__ __device__ static size_t bytes(int ne, int nref) {
         const size_t ynr = (size_t)ne * nref;
-        return (4 + ynr) * AW * sizeof(T2<T>) + (size_t)A_NF * AW * sizeof(T) + 2 * ynr * sizeof(T) +
+        return (4 + ynr) * AW * sizeof(T2<T>) + (size_t)3 * A_NF * AW * sizeof(T) + 2 * ynr * sizeof(T) +
                (size_t)ne * ne * sizeof(T) + AW * sizeof(int);
     }
     __device__ AtmLds(unsigned char *s, int ne, int nref) {
@@ -104,7 +105,9 @@ struct AtmLds {
         J3 = J2 + AW;
         Jyd = J3 + AW;
         Jacc = reinterpret_cast<T *>(Jyd + ynr * AW);
-        Cy = Jacc + A_NF * AW;
+        Khi = Jacc + A_NF * AW;
+        Klo = Khi + A_NF * AW;
+        Cy = Klo + A_NF * AW;
         Cdy = Cy + ynr;
         Rt = Cdy + ynr;
         Jt = reinterpret_cast<int *>(Rt + (size_t)ne * ne);
@@ -258,6 +261,8 @@ __global__ __launch_bounds__(AW) void k_atm(Params<T> P) {
             const float kminy = uni(wmin(kval ? (float)ky : 3e38f)), kmaxy = uni(wmax(kval ? (float)ky : -3e38f));
             const float kminz = uni(wmin(kval ? (float)kz : 3e38f)), kmaxz = uni(wmax(kval ? (float)kz : -3e38f));
             const int ykb = tk * NR * AW;
+            // k-side sums over all j tiles: lane-private LDS columns, folded to FP64 once per kb
+            for (int q = 0; q < A_NF; ++q) S.Khi[q * AW + lane] = S.Klo[q * AW + lane] = T(0);
 
             for (int jt = 0; jt <= kb; ++jt) {
                 const int pj = jt * AW + lane;
@@ -404,18 +409,25 @@ __global__ __launch_bounds__(AW) void k_atm(Params<T> P) {
                     kH += hC;
                     kG += hc6 * dk6;
                 }
-                if (kval) {
-                    La[A_A * cap + k] += (double)kA;
-                    La[A_BX * cap + k] += (double)kBx;
-                    La[A_BY * cap + k] += (double)kBy;
-                    La[A_BZ * cap + k] += (double)kBz;
-                    La[A_H * cap + k] += (double)kH;
-                    La[A_G * cap + k] += (double)kG;
+                {
+                    // fold the tile sums of neighbour k (exact two-sum in FP32, plain in FP64)
+                    const T v[A_NF] = {kA, kBx, kBy, kBz, kH, kG};
+                    for (int q = 0; q < A_NF; ++q) {
+                        const T hi = S.Khi[q * AW + lane], s = hi + v[q];
+                        if constexpr (F32) {
+                            const T bp = s - hi;
+                            S.Klo[q * AW + lane] += (hi - (s - bp)) + (v[q] - bp);
+                        }
+                        S.Khi[q * AW + lane] = s;
+                    }
                 }
                 wave_sync();
                 if (jval)
                     for (int q = 0; q < A_NF; ++q) La[q * cap + pj] += (double)S.Jacc[q * AW + lane];
             }
+            if (kval)
+                for (int q = 0; q < A_NF; ++q)
+                    La[q * cap + k] += (double)S.Khi[q * AW + lane] + (double)S.Klo[q * AW + lane];
         }
 
         // ---- phase 4.  Per neighbour slot p: F_p = grad_p E = -d_p A_p + B_p, H_p = sum of e/2,
