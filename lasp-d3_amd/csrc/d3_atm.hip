@@ -88,16 +88,16 @@ template <class T>
 struct AtmLds {
     T2<T> *J0, *J1, *J2, *J3, *Jyd;  // j tile: (dx, dy) (dz, a) (1/a, a^-1/2) (sqrt C6_ij, lam); rows (y, dy)
     T *Jacc;                         // [A_NF][64] j-side tile sums
-    T *Khi, *Klo;                    // [A_NF][64] k-side sums over the j tiles (two-sum pair in FP32)
+    T *Khi, *Klo;                    // [A_NF][64] k-side sums over the j tiles (two-sum pair in FP32; KL only)
     T *Cy, *Cdy;                     // centre rows [ne * nref]
     T *Rt;                           // per type pair: FP32 (8/3) log2 R0^2, FP64 R0^2
     int *Jt;                         // j type x ne [64]
-    __host__ __device__ static size_t bytes(int ne, int nref) {
+    __host__ __device__ static size_t bytes(int ne, int nref, bool kl) {
         const size_t ynr = (size_t)ne * nref;
-        return (4 + ynr) * AW * sizeof(T2<T>) + (size_t)3 * A_NF * AW * sizeof(T) + 2 * ynr * sizeof(T) +
-               (size_t)ne * ne * sizeof(T) + AW * sizeof(int);
+        return (4 + ynr) * AW * sizeof(T2<T>) + (size_t)(kl ? 3 : 1) * A_NF * AW * sizeof(T) +
+               2 * ynr * sizeof(T) + (size_t)ne * ne * sizeof(T) + AW * sizeof(int);
     }
-    __device__ AtmLds(unsigned char *s, int ne, int nref) {
+    __device__ AtmLds(unsigned char *s, int ne, int nref, bool kl) {
         const size_t ynr = (size_t)ne * nref;
         J0 = reinterpret_cast<T2<T> *>(s);
         J1 = J0 + AW;
@@ -107,20 +107,23 @@ struct AtmLds {
         Jacc = reinterpret_cast<T *>(Jyd + ynr * AW);
         Khi = Jacc + A_NF * AW;
         Klo = Khi + A_NF * AW;
-        Cy = Klo + A_NF * AW;
+        Cy = kl ? Klo + A_NF * AW : Khi;
         Cdy = Cy + ynr;
         Rt = Cdy + ynr;
         Jt = reinterpret_cast<int *>(Rt + (size_t)ne * ne);
     }
 };
 
-template <class T, int NR>
+// KL: the k side's tile sums go to lane-private LDS (two-sum pairs) and reach the FP64
+// accumulators once per k tile; otherwise they are added to them after every tile.  The host
+// picks KL when its larger LDS tile costs no resident blocks.
+template <class T, int NR, bool KL>
 __global__ __launch_bounds__(AW) void k_atm(Params<T> P) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr bool F32 = sizeof(T) == 4;
     const int lane = threadIdx.x;
     const int ne = P.ne, ynr = ne * NR;
-    AtmLds<T> S(smem, ne, NR);
+    AtmLds<T> S(smem, ne, NR, KL);
 
     const int cap = P.atm_cap;
     int *Lj = P.atm_i32 + (size_t)blockIdx.x * cap * 2;
@@ -262,7 +265,8 @@ __global__ __launch_bounds__(AW) void k_atm(Params<T> P) {
             const float kminz = uni(wmin(kval ? (float)kz : 3e38f)), kmaxz = uni(wmax(kval ? (float)kz : -3e38f));
             const int ykb = tk * NR * AW;
             // k-side sums over all j tiles: lane-private LDS columns, folded to FP64 once per kb
-            for (int q = 0; q < A_NF; ++q) S.Khi[q * AW + lane] = S.Klo[q * AW + lane] = T(0);
+            if constexpr (KL)
+                for (int q = 0; q < A_NF; ++q) S.Khi[q * AW + lane] = S.Klo[q * AW + lane] = T(0);
 
             for (int jt = 0; jt <= kb; ++jt) {
                 const int pj = jt * AW + lane;
@@ -412,22 +416,27 @@ __global__ __launch_bounds__(AW) void k_atm(Params<T> P) {
                 {
                     // fold the tile sums of neighbour k (exact two-sum in FP32, plain in FP64)
                     const T v[A_NF] = {kA, kBx, kBy, kBz, kH, kG};
-                    for (int q = 0; q < A_NF; ++q) {
-                        const T hi = S.Khi[q * AW + lane], s = hi + v[q];
-                        if constexpr (F32) {
-                            const T bp = s - hi;
-                            S.Klo[q * AW + lane] += (hi - (s - bp)) + (v[q] - bp);
+                    if constexpr (KL) {
+                        for (int q = 0; q < A_NF; ++q) {
+                            const T hi = S.Khi[q * AW + lane], s = hi + v[q];
+                            if constexpr (F32) {
+                                const T bp = s - hi;
+                                S.Klo[q * AW + lane] += (hi - (s - bp)) + (v[q] - bp);
+                            }
+                            S.Khi[q * AW + lane] = s;
                         }
-                        S.Khi[q * AW + lane] = s;
+                    } else if (kval) {
+                        for (int q = 0; q < A_NF; ++q) La[q * cap + k] += (double)v[q];
                     }
                 }
                 wave_sync();
                 if (jval)
                     for (int q = 0; q < A_NF; ++q) La[q * cap + pj] += (double)S.Jacc[q * AW + lane];
             }
-            if (kval)
-                for (int q = 0; q < A_NF; ++q)
-                    La[q * cap + k] += (double)S.Khi[q * AW + lane] + (double)S.Klo[q * AW + lane];
+            if constexpr (KL)
+                if (kval)
+                    for (int q = 0; q < A_NF; ++q)
+                        La[q * cap + k] += (double)S.Khi[q * AW + lane] + (double)S.Klo[q * AW + lane];
         }
 
         // ---- phase 4.  Per neighbour slot p: F_p = grad_p E = -d_p A_p + B_p, H_p = sum of e/2,
@@ -506,21 +515,35 @@ __global__ void k_merge(Params<T> P) {
 }
 
 template <class T>
-const void *atm_kernel(int nref) {
-    return nref == 5 ? reinterpret_cast<const void *>(&k_atm<T, 5>) : reinterpret_cast<const void *>(&k_atm<T, 7>);
+const void *atm_kernel(int nref, bool kl) {
+    if (nref == 5)
+        return kl ? reinterpret_cast<const void *>(&k_atm<T, 5, true>)
+                  : reinterpret_cast<const void *>(&k_atm<T, 5, false>);
+    return kl ? reinterpret_cast<const void *>(&k_atm<T, 7, true>) : reinterpret_cast<const void *>(&k_atm<T, 7, false>);
+}
+
+int blocks_per_cu(int ne, int nref, size_t ts, bool kl) {
+    const size_t lds = atm_lds_bytes(ne, nref, ts, kl);
+    const void *fn = ts == sizeof(double) ? atm_kernel<double>(nref, kl) : atm_kernel<float>(nref, kl);
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, AW, lds) != hipSuccess) per_cu = 0;
+    return per_cu;
 }
 
 }  // namespace
 
-size_t atm_lds_bytes(int ne, int nref, size_t ts) {
-    return ts == sizeof(double) ? AtmLds<double>::bytes(ne, nref) : AtmLds<float>::bytes(ne, nref);
+size_t atm_lds_bytes(int ne, int nref, size_t ts, bool kl) {
+    return ts == sizeof(double) ? AtmLds<double>::bytes(ne, nref, kl) : AtmLds<float>::bytes(ne, nref, kl);
 }
 
-int atm_resident_blocks(int ne, int nref, size_t ts) {
-    const size_t lds = atm_lds_bytes(ne, nref, ts);
-    const void *fn = ts == sizeof(double) ? atm_kernel<double>(nref) : atm_kernel<float>(nref);
-    int per_cu = 0, dev = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, AW, lds) != hipSuccess) per_cu = 0;
+bool atm_choose_kl(int ne, int nref, size_t ts) {
+    if (atm_lds_bytes(ne, nref, ts, true) > 65536) return false;
+    return blocks_per_cu(ne, nref, ts, true) >= blocks_per_cu(ne, nref, ts, false);
+}
+
+int atm_resident_blocks(int ne, int nref, size_t ts, bool kl) {
+    const int per_cu = blocks_per_cu(ne, nref, ts, kl);
+    int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         cus = 0;
@@ -533,11 +556,19 @@ void launch_atm(const Params<T> &P, int nblocks, hipStream_t s) {
     const size_t nz = (size_t)6 * P.n;
     if (nz) hipLaunchKernelGGL(k_fill_d, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, s, P.fa, nz, P.atm_next);
     if (P.n == 0 || !(P.rc_32 > T(0))) return;
-    const size_t lds = atm_lds_bytes(P.ne, P.nref, sizeof(T));
-    if (P.nref == 5)
-        hipLaunchKernelGGL((k_atm<T, 5>), dim3(nblocks), dim3(AW), lds, s, P);
-    else
-        hipLaunchKernelGGL((k_atm<T, 7>), dim3(nblocks), dim3(AW), lds, s, P);
+    const bool kl = P.atm_kl != 0;
+    const size_t lds = atm_lds_bytes(P.ne, P.nref, sizeof(T), kl);
+    if (P.nref == 5) {
+        if (kl)
+            hipLaunchKernelGGL((k_atm<T, 5, true>), dim3(nblocks), dim3(AW), lds, s, P);
+        else
+            hipLaunchKernelGGL((k_atm<T, 5, false>), dim3(nblocks), dim3(AW), lds, s, P);
+    } else {
+        if (kl)
+            hipLaunchKernelGGL((k_atm<T, 7, true>), dim3(nblocks), dim3(AW), lds, s, P);
+        else
+            hipLaunchKernelGGL((k_atm<T, 7, false>), dim3(nblocks), dim3(AW), lds, s, P);
+    }
     hipLaunchKernelGGL(k_merge<T>, dim3((P.n + 255) / 256), dim3(256), 0, s, P);
 }
 

@@ -151,7 +151,7 @@ Engine::Engine(const uint16_t *elements, uint64_t num_elements, uint64_t max_len
     ne_ = (int)elems_.size();
     if (ne_ == 0) fail(D3X_ERR_INPUT, "no elements given");
     if (ne_ > 64) fail(D3X_ERR_INPUT, "at most 64 distinct elements per handle");
-    if (rc3_ > 0 && atm_lds_bytes(ne_, nref_, precision == D3X_FP64 ? 8 : 4) > 65536)
+    if (rc3_ > 0 && atm_lds_bytes(ne_, nref_, precision == D3X_FP64 ? 8 : 4, false) > 65536)
         fail(D3X_ERR_INPUT, "too many distinct elements for the three-body kernel's LDS tile");
 
     // per-handle tables: C6ref rows and factorised CN references (src/d3_buffer.cu:192-277 gathers
@@ -562,6 +562,7 @@ Params<T> Engine::params() {
     P.rdamp = tab + tab_off_[4];
     P.r0 = tab + tab_off_[5];
     P.atm_cap = atm_cap_;
+    P.atm_kl = rc3_ > 0 ? atm_kl() : 0;
     P.atm_i32 = (int *)d_atm_i32;
     P.atm_T = (T *)d_atm_T;
     P.atm_acc = (double *)d_atm_acc;

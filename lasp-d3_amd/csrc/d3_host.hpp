@@ -171,9 +171,15 @@ class Engine {
     void *d_atm_pa = nullptr;  // per-atom ATM results: fa[3N] | ga[N] | eas[N] | eat[N] (double)
     int atm_blocks_ = 0, atm_cap_ = 0;
     int atm_resident_ = 0;  // three-body blocks resident on the device at once (0: not queried)
+    int atm_kl_ = -1;       // three-body kernel variant (-1: not chosen yet)
+    int atm_kl() {
+        if (atm_kl_ < 0) atm_kl_ = atm_choose_kl(ne_, nref_, precision_ == 0 ? sizeof(double) : sizeof(float)) ? 1 : 0;
+        return atm_kl_;
+    }
     int atm_resident() {
         if (atm_resident_ <= 0)
-            atm_resident_ = atm_resident_blocks(ne_, nref_, precision_ == 0 ? sizeof(double) : sizeof(float));
+            atm_resident_ =
+                atm_resident_blocks(ne_, nref_, precision_ == 0 ? sizeof(double) : sizeof(float), atm_kl() != 0);
         return atm_resident_;
     }
     size_t atm_alloc_entries_ = 0;  // allocated scratch entries (>= atm_blocks_ * atm_cap_)

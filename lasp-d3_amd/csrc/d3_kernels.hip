@@ -220,16 +220,65 @@ struct Queue {
     T dx[QCAP], dy[QCAP], dz[QCAP];
 };
 
+// Per-lane sums of the two-body terms: plain partial sums over at most FOLD pairs, folded into
+// the compensated totals (two-sum in FP32) — one compensated add per FOLD pairs instead of per
+// pair.
 template <class T>
 struct Acc2b {
+    static constexpr int FOLD = 16;
     Acc<T> e, g, f[3], v[6];
+    T pe, pg, pf[3], pv[6];
+    int np;
     __device__ void init() {
         e.init();
         g.init();
         for (int k = 0; k < 3; ++k) f[k].init();
         for (int k = 0; k < 6; ++k) v[k].init();
+        clear();
+    }
+    __device__ void clear() {
+        pe = pg = T(0);
+        for (int k = 0; k < 3; ++k) pf[k] = T(0);
+        for (int k = 0; k < 6; ++k) pv[k] = T(0);
+        np = 0;
+    }
+    __device__ void fold() {
+        e.add(pe);
+        g.add(pg);
+        for (int k = 0; k < 3; ++k) f[k].add(pf[k]);
+        for (int k = 0; k < 6; ++k) v[k].add(pv[k]);
+        clear();
+    }
+    __device__ void step() {
+        if (++np == FOLD) fold();
+    }
+    // one pair's terms: FP32 into the partial sums, FP64 straight into the totals
+    __device__ void add(T de, T dg, T f0, T f1, T f2, const T vv[6]) {
+        if constexpr (sizeof(T) == sizeof(float)) {
+            pe += de;
+            pg += dg;
+            pf[0] += f0;
+            pf[1] += f1;
+            pf[2] += f2;
+            for (int k = 0; k < 6; ++k) pv[k] += vv[k];
+            step();
+        } else {
+            e.add(de);
+            g.add(dg);
+            f[0].add(f0);
+            f[1].add(f1);
+            f[2].add(f2);
+            for (int k = 0; k < 6; ++k) v[k].add(vv[k]);
+        }
     }
 };
+
+// Reciprocal and reciprocal square root of the pair math: the FP32 hardware instructions
+// (1 ulp), correctly rounded FP64.
+__device__ __forceinline__ float prcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double prcp(double x) { return 1.0 / x; }
+__device__ __forceinline__ float prsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ double prsqrt(double x) { return rsqrt(x); }
 
 template <class T>
 __device__ __forceinline__ void pair_2b(const Params<T> &P, int j, T dx, T dy, T dz, const T *y, const T *dy_,
@@ -244,7 +293,7 @@ __device__ __forceinline__ void pair_2b(const Params<T> &P, int j, T dx, T dy, T
     const T q = rq[tj], R = rR[tj];
     const T c8 = T(3) * c6 * q, dc8 = T(3) * dc6 * q;
     const T r2 = dx * dx + dy * dy + dz * dz;
-    const T ir = xrsqrt(r2);
+    const T ir = prsqrt(r2);
     const T ir2 = ir * ir;
     const T ir6 = ir2 * ir2 * ir2, ir8 = ir6 * ir2;
     T f6, f8, df6, df8;
@@ -253,14 +302,14 @@ __device__ __forceinline__ void pair_2b(const Params<T> &P, int j, T dx, T dy, T
         const T b6 = P.p1 * t, b8 = P.p2 * t;
         const T b6_2 = b6 * b6, b6_4 = b6_2 * b6_2, b6_8 = b6_4 * b6_4, b6_14 = b6_8 * b6_4 * b6_2;
         const T b8_2 = b8 * b8, b8_4 = b8_2 * b8_2, b8_8 = b8_4 * b8_4, b8_16 = b8_8 * b8_8;
-        f6 = T(1) / (T(1) + T(6) * b6_14);
-        f8 = T(1) / (T(1) + T(6) * b8_16);
+        f6 = prcp(T(1) + T(6) * b6_14);
+        f8 = prcp(T(1) + T(6) * b8_16);
         df6 = T(84) * f6 * f6 * b6_14 * ir;
         df8 = T(96) * f8 * f8 * b8_16 * ir;
     } else {
         const T a2 = R * R, a6 = a2 * a2 * a2, a8 = a6 * a2;
         const T r6 = r2 * r2 * r2, r8 = r6 * r2;
-        const T d6 = T(1) / (r6 + a6), d8 = T(1) / (r8 + a8);
+        const T d6 = prcp(r6 + a6), d8 = prcp(r8 + a8);
         f6 = r6 * d6;
         f8 = r8 * d8;
         df6 = T(6) * a6 * r6 * ir * d6 * d6;
@@ -271,19 +320,10 @@ __device__ __forceinline__ void pair_2b(const Params<T> &P, int j, T dx, T dy, T
     df6 = finite_or_zero(df6);
     df8 = finite_or_zero(df8);
     const T e6 = P.s6 * c6 * f6 * ir6, e8 = P.s8 * c8 * f8 * ir8;
-    A.e.add(T(0.5) * (e6 + e8));
-    A.g.add(P.s6 * f6 * dc6 * ir6 + P.s8 * f8 * dc8 * ir8);
     const T phi = (P.s6 * c6 * (df6 - T(6) * f6 * ir) * ir6 + P.s8 * c8 * (df8 - T(8) * f8 * ir) * ir8) * ir;
-    A.f[0].add(phi * dx);
-    A.f[1].add(phi * dy);
-    A.f[2].add(phi * dz);
     const T h = T(-0.5) * phi;
-    A.v[0].add(h * dx * dx);
-    A.v[1].add(h * dy * dy);
-    A.v[2].add(h * dz * dz);
-    A.v[3].add(h * dx * dy);
-    A.v[4].add(h * dx * dz);
-    A.v[5].add(h * dy * dz);
+    const T vv[6] = {h * dx * dx, h * dy * dy, h * dz * dz, h * dx * dy, h * dx * dz, h * dy * dz};
+    A.add(T(0.5) * (e6 + e8), P.s6 * f6 * dc6 * ir6 + P.s8 * f8 * dc8 * ir8, phi * dx, phi * dy, phi * dz, vv);
 }
 
 template <class T>
@@ -387,6 +427,7 @@ __global__ __launch_bounds__(BLOCK) void k_2b(Params<T> P) {
         pair_2b(P, Q.j[lane], Q.dx[lane], Q.dy[lane], Q.dz[lane], y, dyv, rq, rR, A);
         cnt += 1.0f;
     }
+    A.fold();
     const double E = wave_total(A.e), G = wave_total(A.g);
     double F[3], V[6];
     for (int k = 0; k < 3; ++k) F[k] = wave_total(A.f[k]);
