@@ -85,7 +85,7 @@ def workload(name: str):
     raise SystemExit(f"unknown workload {name}")
 
 
-def make_handle(w, precision):
+def make_handle(w, precision, graph=False, autotune=False):
     n = len(w["Z"])
     h = d3_lib.Handle(np.unique(w["Z"]), n, cutoff=RC2, cn_cutoff=RCCN, atm_cutoff=RC3, damping=1, functional=0,
                       precision=precision)
@@ -95,6 +95,11 @@ def make_handle(w, precision):
         h.compute()  # sizes scratch (retries a neighbour-list overflow) and leaves inputs resident
     else:
         h.compute_batch(w["counts"], w["pos"], w["Z"], w["cells"])
+    if autotune:
+        h.autotune()  # times the candidates on this input and keeps the fastest
+    if graph:
+        h.set_graph(True)
+        h.compute_device()  # captures; the timed steps replay
     return h
 
 
@@ -219,6 +224,8 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the three-body network leg")
+    ap.add_argument("--graph", action="store_true", help="replay the evaluation as a captured hipGraph")
+    ap.add_argument("--autotune", action="store_true", help="autotune bin width and three-body grid first")
     ap.add_argument("--only-surrogate", action="store_true",
                     help="run only the three-body network leg and print its JSON (for counter runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -248,7 +255,8 @@ def main():
     print(f"[bench] workload {w['desc']} atoms={n}", file=sys.stderr, flush=True)
 
     hbm0 = hbm_used_gb(ev)
-    h32 = make_handle(w, d3_lib.D3X_FP32)
+    h32 = make_handle(w, d3_lib.D3X_FP32, graph=args.graph, autotune=args.autotune)
+    tuning32 = h32.tuning()
     hbm32 = hbm_used_gb(ev)
     ms32, wall32 = time_steps(h32, ev, args.steps, args.warmup)
     e32, f32, s32 = fetch(h32, w)
@@ -256,7 +264,7 @@ def main():
     prof32 = phase_profile(h32)
     counts = h32.counts(len(w["counts"]) if w["kind"] == "batch" else 1).sum(axis=0)
 
-    h64 = make_handle(w, d3_lib.D3X_FP64)
+    h64 = make_handle(w, d3_lib.D3X_FP64, graph=args.graph, autotune=args.autotune)
     k64 = args.fp64_steps or args.steps
     ms64, _ = time_steps(h64, ev, k64, 1)
     e64, f64, s64 = fetch(h64, w)
@@ -306,7 +314,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": w["desc"], "atoms": n, "functional": "PBE-D3(BJ)", "rc_2body_bohr": RC2,
                    "rc_cn_bohr": RCCN, "rc_atm_bohr": RC3, "terms": "2b+CN+ATM+chain", "precision": "fp32",
-                   "table": d3_lib.table()["name"]},
+                   "table": d3_lib.table()["name"], "hip_graph": bool(args.graph), "autotuned": bool(args.autotune),
+                   "bin_width_bohr": tuning32[0], "atm_blocks": tuning32[1]},
         "fp64_plain": {"atoms_per_s": n / (ms64 * 1e-3), "ms_per_step": ms64, "steps": k64},
         "speedup_fp32_vs_fp64": ms64 / ms32,
         "fp32_error_vs_fp64": err,
