@@ -32,6 +32,8 @@
 // per-pair parts; FP64 as y^3 y^(-1/3).
 #include <hip/hip_runtime.h>
 
+#include <utility>
+
 #include "d3_device.hpp"
 
 namespace d3 {
@@ -86,16 +88,17 @@ __device__ __forceinline__ float wmax(float v) {
 // LDS carve-up, shared by the kernel and the host-side size query.
 template <class T>
 struct AtmLds {
-    T2<T> *J0, *J1, *J2, *J3, *Jyd;  // j tile: (dx, dy) (dz, a) (1/a, a^-1/2) (sqrt C6_ij, lam); rows (y, dy)
+    // j tile: (dx, dy) (dz, a) (1/a, a^-1/2) (sqrt C6_ij, -); per type B of k, nref rows (y, dy)
+    // and one damping row (FP32: lam_j + (8/3) log2 R0_{A_j B}^2, FP64: lam_j R0_{A_j B}^2; 0)
+    T2<T> *J0, *J1, *J2, *J3, *Jyd;
     T *Jacc;                         // [A_NF][64] j-side tile sums
     T *Khi, *Klo;                    // [A_NF][64] k-side sums over the j tiles (two-sum pair in FP32; KL only)
     T *Cy, *Cdy;                     // centre rows [ne * nref]
     T *Rt;                           // per type pair: FP32 (8/3) log2 R0^2, FP64 R0^2
-    int *Jt;                         // j type x ne [64]
     __host__ __device__ static size_t bytes(int ne, int nref, bool kl) {
         const size_t ynr = (size_t)ne * nref;
-        return (4 + ynr) * AW * sizeof(T2<T>) + (size_t)(kl ? 3 : 1) * A_NF * AW * sizeof(T) +
-               2 * ynr * sizeof(T) + (size_t)ne * ne * sizeof(T) + AW * sizeof(int);
+        return (4 + (size_t)ne * (nref + 1)) * AW * sizeof(T2<T>) + (size_t)(kl ? 3 : 1) * A_NF * AW * sizeof(T) +
+               2 * ynr * sizeof(T) + (size_t)ne * ne * sizeof(T);
     }
     __device__ AtmLds(unsigned char *s, int ne, int nref, bool kl) {
         const size_t ynr = (size_t)ne * nref;
@@ -104,21 +107,28 @@ struct AtmLds {
         J2 = J1 + AW;
         J3 = J2 + AW;
         Jyd = J3 + AW;
-        Jacc = reinterpret_cast<T *>(Jyd + ynr * AW);
+        Jacc = reinterpret_cast<T *>(Jyd + (size_t)ne * (nref + 1) * AW);
         Khi = Jacc + A_NF * AW;
         Klo = Khi + A_NF * AW;
         Cy = kl ? Klo + A_NF * AW : Khi;
         Cdy = Cy + ynr;
         Rt = Cdy + ynr;
-        Jt = reinterpret_cast<int *>(Rt + (size_t)ne * ne);
     }
 };
+
+// Waves per SIMD the compiler must allow: FP32 with up to 3 references fits 5 waves (96 VGPRs)
+// without spilling and runs 3.4 % faster on the fcc Cu cell; larger variants spill under that
+// cap (12 % slower on the water box at 5 references) and keep the default.
+template <class T, int NR>
+constexpr int atm_min_waves() {
+    return sizeof(T) == 4 && NR <= 3 ? 5 : 1;
+}
 
 // KL: the k side's tile sums go to lane-private LDS (two-sum pairs) and reach the FP64
 // accumulators once per k tile; otherwise they are added to them after every tile.  The host
 // picks KL when its larger LDS tile costs no resident blocks.
 template <class T, int NR, bool KL>
-__global__ __launch_bounds__(AW) void k_atm(Params<T> P) {
+__global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> P) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr bool F32 = sizeof(T) == 4;
     const int lane = threadIdx.x;
@@ -263,7 +273,7 @@ __global__ __launch_bounds__(AW) void k_atm(Params<T> P) {
             const float kminx = uni(wmin(kval ? (float)kx : 3e38f)), kmaxx = uni(wmax(kval ? (float)kx : -3e38f));
             const float kminy = uni(wmin(kval ? (float)ky : 3e38f)), kmaxy = uni(wmax(kval ? (float)ky : -3e38f));
             const float kminz = uni(wmin(kval ? (float)kz : 3e38f)), kmaxz = uni(wmax(kval ? (float)kz : -3e38f));
-            const int ykb = tk * NR * AW;
+            const int ykb = tk * (NR + 1) * AW;  // rows of type(k) in the j tile
             // k-side sums over all j tiles: lane-private LDS columns, folded to FP64 once per kb
             if constexpr (KL)
                 for (int q = 0; q < A_NF; ++q) S.Khi[q * AW + lane] = S.Klo[q * AW + lane] = T(0);
@@ -310,9 +320,8 @@ __global__ __launch_bounds__(AW) void k_atm(Params<T> P) {
                     S.J0[lane] = T2<T>{jxs, jys};
                     S.J1[lane] = T2<T>{jzs, a};
                     S.J2[lane] = T2<T>{ia, ra};
-                    S.J3[lane] = T2<T>{sc6, lam};
-                    S.Jt[lane] = tj * ne;  // row of the per-type-pair table
-                    for (int B = 0; B < ne; ++B)
+                    S.J3[lane] = T2<T>{sc6, T(0)};
+                    for (int B = 0; B < ne; ++B) {
                         for (int bb = 0; bb < NR; ++bb) {
                             T sy = T(0), sdy = T(0);
                             for (int q = 0; q < NR; ++q) {
@@ -320,15 +329,20 @@ __global__ __launch_bounds__(AW) void k_atm(Params<T> P) {
                                 sy += c * wj[q];
                                 sdy += c * dwj[q];
                             }
-                            S.Jyd[(B * NR + bb) * AW + lane] = T2<T>{sy, sdy};
+                            S.Jyd[(B * (NR + 1) + bb) * AW + lane] = T2<T>{sy, sdy};
                         }
+                        const T rt = S.Rt[tj * ne + B];
+                        S.Jyd[(B * (NR + 1) + NR) * AW + lane] = T2<T>{F32 ? lam + rt : lam * rt, T(0)};
+                    }
                     for (int q = 0; q < A_NF; ++q) S.Jacc[q * AW + lane] = T(0);
                 }
                 wave_sync();
 
                 // tile-local sums of neighbour k (folded into FP64 per tile)
                 T kA = T(0), kBx = T(0), kBy = T(0), kBz = T(0), kH = T(0), kG = T(0);
-                const int jlim = (jt == kb) ? lane : AW;  // diagonal tile: j < k
+                int jlim = (jt == kb) ? lane : AW;  // diagonal tile: j < k
+                // opaque VGPR: j < jlim stays one compare (its ballot is then the compare's mask)
+                asm volatile("" : "+v"(jlim));
 #pragma unroll 1
                 for (int q = 0; q < AW; ++q) {
                     // FP32: lane l updates the j slot that lane l+1 updated one step earlier
@@ -338,14 +352,18 @@ __global__ __launch_bounds__(AW) void k_atm(Params<T> P) {
                     const T jx = j0.x, jy = j0.y, jz = j1.x;
                     const T vx = kx - jx, vy = ky - jy, vz = kz - jz;  // r_j - r_k
                     const T c = vx * vx + vy * vy + vz * vz;
-                    const bool ok = jl < jlim && c <= rc32 && c >= T(1e-12);
-                    const uint64_t m = __builtin_amdgcn_ballot_w64(ok);
+                    const bool okj = jl < jlim, okc = c <= rc32, okz = c >= T(1e-12);
+                    const bool ok = okj && okc && okz;
+                    // one ballot per compare: the ballot of the combined flag would be re-derived
+                    // from a VGPR copy of it
+                    const uint64_t m = __builtin_amdgcn_ballot_w64(okj) & __builtin_amdgcn_ballot_w64(okc) &
+                                       __builtin_amdgcn_ballot_w64(okz);
                     if (m == 0) continue;  // uniform
                     cnt += __popcll(m);
                     if (!ok) continue;
                     const T a = j1.y, b = kb2;
                     const T2<T> j2 = S.J2[jl], j3 = S.J3[jl];
-                    const int tjr = S.Jt[jl];
+                    const T lamjk = S.Jyd[ykb + NR * AW + jl].x;  // damping row of type(k)
                     // C6_jk and its CN derivatives from the j row for type(k)
                     T c6jk = T(0), dk6 = T(0), dj6 = T(0);
 #pragma unroll
@@ -372,10 +390,9 @@ __global__ __launch_bounds__(AW) void k_atm(Params<T> P) {
                     const T P1 = T(-1.5) * iR3 - T(0.9375) * QiR5;
                     T x16;
                     if constexpr (F32) {
-                        x16 = __builtin_amdgcn_exp2f(j3.y + klam + S.Rt[tjr + tk] -
-                                                     (8.0f / 3.0f) * __builtin_amdgcn_logf(c));
+                        x16 = __builtin_amdgcn_exp2f(lamjk + klam - (8.0f / 3.0f) * __builtin_amdgcn_logf(c));
                     } else {
-                        const T y = j3.y * klam * S.Rt[tjr + tk] * ic;
+                        const T y = lamjk * klam * ic;
                         x16 = y * y * y * inv_cbrt(y);
                     }
                     const T f = prcp(T(1) + T(6) * x16);
@@ -514,12 +531,19 @@ __global__ void k_merge(Params<T> P) {
     for (int q = 0; q < 3; ++q) P.f[3 * p + q] += P.fa[3 * p + q];
 }
 
+// One instantiation per reference count (1..MAX_NREF): the C6 dot products and the LDS rows
+// are sized by the references the handle's elements actually have.
+template <class T, bool KL, int... NR>
+const void *atm_pick(int nref, std::integer_sequence<int, NR...>) {
+    const void *fn = nullptr;
+    ((fn = (nref == NR + 1) ? reinterpret_cast<const void *>(&k_atm<T, NR + 1, KL>) : fn), ...);
+    return fn;
+}
+
 template <class T>
 const void *atm_kernel(int nref, bool kl) {
-    if (nref == 5)
-        return kl ? reinterpret_cast<const void *>(&k_atm<T, 5, true>)
-                  : reinterpret_cast<const void *>(&k_atm<T, 5, false>);
-    return kl ? reinterpret_cast<const void *>(&k_atm<T, 7, true>) : reinterpret_cast<const void *>(&k_atm<T, 7, false>);
+    return kl ? atm_pick<T, true>(nref, std::make_integer_sequence<int, MAX_NREF>{})
+              : atm_pick<T, false>(nref, std::make_integer_sequence<int, MAX_NREF>{});
 }
 
 int blocks_per_cu(int ne, int nref, size_t ts, bool kl) {
@@ -558,17 +582,11 @@ void launch_atm(const Params<T> &P, int nblocks, hipStream_t s) {
     if (P.n == 0 || !(P.rc_32 > T(0))) return;
     const bool kl = P.atm_kl != 0;
     const size_t lds = atm_lds_bytes(P.ne, P.nref, sizeof(T), kl);
-    if (P.nref == 5) {
-        if (kl)
-            hipLaunchKernelGGL((k_atm<T, 5, true>), dim3(nblocks), dim3(AW), lds, s, P);
-        else
-            hipLaunchKernelGGL((k_atm<T, 5, false>), dim3(nblocks), dim3(AW), lds, s, P);
-    } else {
-        if (kl)
-            hipLaunchKernelGGL((k_atm<T, 7, true>), dim3(nblocks), dim3(AW), lds, s, P);
-        else
-            hipLaunchKernelGGL((k_atm<T, 7, false>), dim3(nblocks), dim3(AW), lds, s, P);
-    }
+    const void *fn = atm_kernel<T>(P.nref, kl);
+    if (!fn) return;  // not reached: a handle's reference count is 1..MAX_NREF
+    Params<T> arg = P;
+    void *args[] = {&arg};
+    (void)hipLaunchKernel(fn, dim3(nblocks), dim3(AW), args, lds, s);
     hipLaunchKernelGGL(k_merge<T>, dim3((P.n + 255) / 256), dim3(256), 0, s, P);
 }
 

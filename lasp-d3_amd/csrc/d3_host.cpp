@@ -138,8 +138,7 @@ Engine::Engine(const uint16_t *elements, uint64_t num_elements, uint64_t max_len
     if (precision != D3X_FP64 && precision != D3X_FP32) fail(D3X_ERR_INPUT, "unknown precision");
     const Table &tab = active_table();
     table_name_ = tab.name;
-    nref_ = tab.nref;
-    if (nref_ != 5 && nref_ != 7)  // the reference's two table layouts (src/constants_include.h:14-27)
+    if (tab.nref != 5 && tab.nref != 7)  // the reference's two table layouts (src/constants_include.h:14-27)
         fail(D3X_ERR_INPUT, "table must have 5 or 7 references per element");
     for (uint64_t k = 0; k < num_elements; ++k) {
         if (!tab.supports(elements[k]))
@@ -151,35 +150,44 @@ Engine::Engine(const uint16_t *elements, uint64_t num_elements, uint64_t max_len
     ne_ = (int)elems_.size();
     if (ne_ == 0) fail(D3X_ERR_INPUT, "no elements given");
     if (ne_ > 64) fail(D3X_ERR_INPUT, "at most 64 distinct elements per handle");
-    if (rc3_ > 0 && atm_lds_bytes(ne_, nref_, precision == D3X_FP64 ? 8 : 4, false) > 65536)
-        fail(D3X_ERR_INPUT, "too many distinct elements for the three-body kernel's LDS tile");
-
     // per-handle tables: C6ref rows and factorised CN references (src/d3_buffer.cu:192-277 gathers
     // the same entries per unique element)
-    const int ne = ne_, nr = nref_;
-    std::vector<double> c6((size_t)ne * ne * nr * nr, 0.0), cnref((size_t)ne * nr, -1.0);
+    const int ne = ne_, tnr = tab.nref;
+    std::vector<double> tcnref((size_t)ne * tnr, -1.0);
     for (int A = 0; A < ne; ++A)
-        for (int a = 0; a < nr; ++a)
-            for (int B = 0; B < ne && cnref[A * nr + a] <= -1.0; ++B)
-                for (int b = 0; b < nr; ++b) {
+        for (int a = 0; a < tnr; ++a)
+            for (int B = 0; B < ne && tcnref[A * tnr + a] <= -1.0; ++B)
+                for (int b = 0; b < tnr; ++b) {
                     const size_t ix = tab.c6_index(tab.slot(elems_[A]), tab.slot(elems_[B]), a, b);
                     if (tab.c6ab[ix + 1] > -1.0f && tab.c6ab[ix + 2] > -1.0f) {
-                        cnref[A * nr + a] = tab.c6ab[ix + 1];
+                        tcnref[A * tnr + a] = tab.c6ab[ix + 1];
                         break;
                     }
                 }
+    // References no element of the handle has are dropped: their Gaussian weights are exactly
+    // zero, so every sum over references is unchanged, and the kernels loop over fewer terms.
+    nref_ = 1;
+    for (int A = 0; A < ne; ++A)
+        for (int a = 0; a < tnr; ++a)
+            if (tcnref[A * tnr + a] > -1.0) nref_ = std::max(nref_, a + 1);
+    const int nr = nref_;
+    std::vector<double> c6((size_t)ne * ne * nr * nr, 0.0), cnref((size_t)ne * nr, -1.0);
+    for (int A = 0; A < ne; ++A)
+        for (int a = 0; a < nr; ++a) cnref[A * nr + a] = tcnref[A * tnr + a];
     for (int A = 0; A < ne; ++A)
         for (int B = 0; B < ne; ++B)
-            for (int a = 0; a < nr; ++a)
-                for (int b = 0; b < nr; ++b) {
+            for (int a = 0; a < tnr; ++a)
+                for (int b = 0; b < tnr; ++b) {
                     const size_t ix = tab.c6_index(tab.slot(elems_[A]), tab.slot(elems_[B]), a, b);
                     const bool valid = tab.c6ab[ix + 1] > -1.0f && tab.c6ab[ix + 2] > -1.0f;
-                    const bool fvalid = cnref[A * nr + a] > -1.0 && cnref[B * nr + b] > -1.0;
-                    if (valid != fvalid || (valid && ((double)tab.c6ab[ix + 1] != cnref[A * nr + a] ||
-                                                      (double)tab.c6ab[ix + 2] != cnref[B * nr + b])))
+                    const bool fvalid = tcnref[A * tnr + a] > -1.0 && tcnref[B * tnr + b] > -1.0;
+                    if (valid != fvalid || (valid && ((double)tab.c6ab[ix + 1] != tcnref[A * tnr + a] ||
+                                                      (double)tab.c6ab[ix + 2] != tcnref[B * tnr + b])))
                         fail(D3X_ERR_INPUT, "parameter table CN references do not factorise per element");
                     if (valid) c6[(((size_t)A * ne + B) * nr + a) * nr + b] = tab.c6ab[ix + 0];
                 }
+    if (rc3_ > 0 && atm_lds_bytes(ne_, nref_, precision == D3X_FP64 ? 8 : 4, false) > 65536)
+        fail(D3X_ERR_INPUT, "too many distinct elements for the three-body kernel's LDS tile");
     // layout of h_tab_: c6 | cnref | rcov | r2r4 | r0ab  (q4 / rdamp derived at upload)
     h_tab_.clear();
     h_tab_.insert(h_tab_.end(), c6.begin(), c6.end());
