@@ -116,12 +116,13 @@ struct AtmLds {
     }
 };
 
-// Waves per SIMD the compiler must allow: FP32 with up to 3 references fits 5 waves (96 VGPRs)
-// without spilling and runs 3.4 % faster on the fcc Cu cell; larger variants spill under that
-// cap (12 % slower on the water box at 5 references) and keep the default.
+// Waves per SIMD the compiler must allow: FP32 with 3 references fits 5 waves (96 VGPRs) without
+// spilling and runs 2.6 % faster on the fcc Cu cell; under that cap 2 references spill (1 %
+// slower on diamond Si) and 5 spill more (12 % slower on the water box), so they keep the
+// default.
 template <class T, int NR>
 constexpr int atm_min_waves() {
-    return sizeof(T) == 4 && NR <= 3 ? 5 : 1;
+    return sizeof(T) == 4 && NR == 3 ? 5 : 1;
 }
 
 // KL: the k side's tile sums go to lane-private LDS (two-sum pairs) and reach the FP64

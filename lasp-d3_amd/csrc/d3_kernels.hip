@@ -11,6 +11,8 @@
 // T = double is the plain FP64 path, T = float the FP32 path with compensated accumulation.
 #include <hip/hip_runtime.h>
 
+#include <utility>
+
 #include <cfloat>
 
 #include "d3_device.hpp"
@@ -280,15 +282,20 @@ __device__ __forceinline__ double prcp(double x) { return 1.0 / x; }
 __device__ __forceinline__ float prsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ double prsqrt(double x) { return rsqrt(x); }
 
-template <class T>
+// NR = the handle's reference count: the weight gathers of a pair are unrolled and issued
+// together.
+template <class T, int NR>
 __device__ __forceinline__ void pair_2b(const Params<T> &P, int j, T dx, T dy, T dz, const T *y, const T *dy_,
                                         const T *rq, const T *rR, Acc2b<T> &A) {
     const int tj = P.type_s[j];
+    T wj[NR];
+#pragma unroll
+    for (int b = 0; b < NR; ++b) wj[b] = P.w[b * P.n + j];
     T c6 = T(0), dc6 = T(0);
-    for (int b = 0; b < P.nref; ++b) {
-        const T wj = P.w[b * P.n + j];
-        c6 += y[tj * P.nref + b] * wj;
-        dc6 += dy_[tj * P.nref + b] * wj;
+#pragma unroll
+    for (int b = 0; b < NR; ++b) {
+        c6 += y[tj * NR + b] * wj[b];
+        dc6 += dy_[tj * NR + b] * wj[b];
     }
     const T q = rq[tj], R = rR[tj];
     const T c8 = T(3) * c6 * q, dc8 = T(3) * dc6 * q;
@@ -331,30 +338,30 @@ __host__ __device__ constexpr size_t twobody_wave_lds(int ne, int nref) {
     return ((sizeof(Queue<T>) + (size_t)(2 * ne * nref + 2 * ne) * sizeof(T)) + 15) / 16 * 16;
 }
 
-template <class T>
+template <class T, int NR>
 __global__ __launch_bounds__(BLOCK) void k_2b(Params<T> P) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int wv = threadIdx.x >> 6, lane = lane_id();
     const int i = blockIdx.x * WPB + wv;
     if (i >= P.n) return;
-    const int ynr = P.ne * P.nref;
-    unsigned char *base = smem + wv * twobody_wave_lds<T>(P.ne, P.nref);
+    const int ynr = P.ne * NR;
+    unsigned char *base = smem + wv * twobody_wave_lds<T>(P.ne, NR);
     Queue<T> &Q = *reinterpret_cast<Queue<T> *>(base);
     T *y = reinterpret_cast<T *>(base + sizeof(Queue<T>));
     T *dyv = y + ynr, *rq = dyv + ynr, *rR = rq + P.ne;
 
     Centre<T> C;
     C.load(P, i);
-    T wi[MAX_NREF], dwi[MAX_NREF];
-    for (int a = 0; a < MAX_NREF; ++a) {
-        wi[a] = a < P.nref ? P.w[a * P.n + i] : T(0);
-        dwi[a] = a < P.nref ? P.dw[a * P.n + i] : T(0);
+    T wi[NR], dwi[NR];
+    for (int a = 0; a < NR; ++a) {
+        wi[a] = P.w[a * P.n + i];
+        dwi[a] = P.dw[a * P.n + i];
     }
     for (int idx = lane; idx < ynr; idx += WAVE) {
-        const int B = idx / P.nref, bb = idx % P.nref;
+        const int B = idx / NR, bb = idx % NR;
         T sy = T(0), sdy = T(0);
-        for (int a = 0; a < P.nref; ++a) {
-            const T c = P.c6[((C.ti * P.ne + B) * P.nref + a) * P.nref + bb];
+        for (int a = 0; a < NR; ++a) {
+            const T c = P.c6[((C.ti * P.ne + B) * NR + a) * NR + bb];
             sy += c * wi[a];
             sdy += c * dwi[a];
         }
@@ -399,7 +406,7 @@ __global__ __launch_bounds__(BLOCK) void k_2b(Params<T> P) {
             qn += __popcll(m);
             if (qn >= WAVE) {
                 wave_sync();
-                pair_2b(P, Q.j[lane], Q.dx[lane], Q.dy[lane], Q.dz[lane], y, dyv, rq, rR, A);
+                pair_2b<T, NR>(P, Q.j[lane], Q.dx[lane], Q.dy[lane], Q.dz[lane], y, dyv, rq, rR, A);
                 cnt += 1.0f;
                 const int rest = qn - WAVE;
                 int mj = 0;
@@ -424,7 +431,7 @@ __global__ __launch_bounds__(BLOCK) void k_2b(Params<T> P) {
     }
     wave_sync();
     if (lane < qn) {
-        pair_2b(P, Q.j[lane], Q.dx[lane], Q.dy[lane], Q.dz[lane], y, dyv, rq, rR, A);
+        pair_2b<T, NR>(P, Q.j[lane], Q.dx[lane], Q.dy[lane], Q.dz[lane], y, dyv, rq, rR, A);
         cnt += 1.0f;
     }
     A.fold();
@@ -519,11 +526,22 @@ void Launch<T>::cn(const Params<T> &P, hipStream_t s) {
     hipLaunchKernelGGL(k_cn<T>, dim3((P.n + WPB - 1) / WPB), dim3(BLOCK), 0, s, P);
 }
 
+template <class T, int... NR>
+const void *twobody_pick(int nref, std::integer_sequence<int, NR...>) {
+    const void *fn = nullptr;
+    ((fn = (nref == NR + 1) ? reinterpret_cast<const void *>(&k_2b<T, NR + 1>) : fn), ...);
+    return fn;
+}
+
 template <class T>
 void Launch<T>::twobody(const Params<T> &P, hipStream_t s) {
     if (P.n == 0) return;
     const size_t lds = WPB * twobody_wave_lds<T>(P.ne, P.nref);
-    hipLaunchKernelGGL(k_2b<T>, dim3((P.n + WPB - 1) / WPB), dim3(BLOCK), lds, s, P);
+    const void *fn = twobody_pick<T>(P.nref, std::make_integer_sequence<int, MAX_NREF>{});
+    if (!fn) return;  // not reached: a handle's reference count is 1..MAX_NREF
+    Params<T> arg = P;
+    void *args[] = {&arg};
+    (void)hipLaunchKernel(fn, dim3((P.n + WPB - 1) / WPB), dim3(BLOCK), args, lds, s);
 }
 
 template <class T>
