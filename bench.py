@@ -96,6 +96,7 @@ def make_handle(w, precision, graph=False, autotune=False):
     else:
         h.compute_batch(w["counts"], w["pos"], w["Z"], w["cells"])
     if autotune:
+        os.environ.setdefault("D3_TUNE_LOG", "1")  # a line per timed candidate (keeps long sweeps visible)
         h.autotune()  # times the candidates on this input and keeps the fastest
     if graph:
         h.set_graph(True)
@@ -257,6 +258,7 @@ def main():
     hbm0 = hbm_used_gb(ev)
     h32 = make_handle(w, d3_lib.D3X_FP32, graph=args.graph, autotune=args.autotune)
     tuning32 = h32.tuning()
+    variant32 = h32.atm_variant() if RC3 > 0 else None
     hbm32 = hbm_used_gb(ev)
     ms32, wall32 = time_steps(h32, ev, args.steps, args.warmup)
     e32, f32, s32 = fetch(h32, w)
@@ -315,7 +317,7 @@ def main():
         "config": {"workload": w["desc"], "atoms": n, "functional": "PBE-D3(BJ)", "rc_2body_bohr": RC2,
                    "rc_cn_bohr": RCCN, "rc_atm_bohr": RC3, "terms": "2b+CN+ATM+chain", "precision": "fp32",
                    "table": d3_lib.table()["name"], "hip_graph": bool(args.graph), "autotuned": bool(args.autotune),
-                   "bin_width_bohr": tuning32[0], "atm_blocks": tuning32[1]},
+                   "bin_width_bohr": tuning32[0], "atm_blocks": tuning32[1], "atm_variant": variant32},
         "fp64_plain": {"atoms_per_s": n / (ms64 * 1e-3), "ms_per_step": ms64, "steps": k64},
         "speedup_fp32_vs_fp64": ms64 / ms32,
         "fp32_error_vs_fp64": err,

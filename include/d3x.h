@@ -76,15 +76,20 @@ int d3x_phase_times(d3_handle_t *handle, double ms[6]);
 int d3x_set_graph(d3_handle_t *handle, int enable);
 
 /* Tuning parameters: bin width of the cell lists (bohr; default half the CN cutoff) and the
- * number of persistent blocks of the three-body kernel (default 4096).  Values <= 0 keep the
- * current setting.  d3x_autotune times full evaluations of the handle's current input over a
- * small candidate set (atoms and cell must be set), keeps the fastest, stores it in the JSON cache
- * at cache_path (NULL: $D3_TUNE_CACHE, if set) keyed by precision, cutoffs, density bucket and
- * element count, and returns the best time in ms (negative on error).  Handles consult
- * $D3_TUNE_CACHE for their input's key unless tuning was set explicitly.  Results do not depend
- * on these parameters beyond summation order. */
+ * number of persistent blocks of the three-body kernel (default: as many as the device holds at
+ * once).  Values <= 0 keep the current setting.  d3x_autotune times full evaluations of the
+ * handle's current input over a small candidate set -- bin widths, the three-body kernel's
+ * variants (k-side sums in LDS or not, damping rows staged or not) and block counts -- (atoms and
+ * cell must be set), keeps the fastest, stores it in the JSON cache at cache_path (NULL:
+ * $D3_TUNE_CACHE, if set) keyed by precision, cutoffs, density bucket and element count, and
+ * returns the best time in ms (negative on error); with D3_TUNE_LOG=1 in the environment it
+ * prints one line per candidate on stderr.  Handles consult $D3_TUNE_CACHE for their input's key
+ * unless tuning was set explicitly.  Results do not depend on these parameters beyond summation
+ * order.  d3x_get_atm_variant: the three-body variant in use (bit 0: k-side sums in LDS, bit 1:
+ * damping rows), chosen by occupancy unless the autotuner or the cache set it; -1 on error. */
 int d3x_set_tuning(d3_handle_t *handle, double bin_width_bohr, int atm_blocks);
 int d3x_get_tuning(d3_handle_t *handle, double *bin_width_bohr, int *atm_blocks);
+int d3x_get_atm_variant(d3_handle_t *handle);
 double d3x_autotune(d3_handle_t *handle, const char *cache_path);
 
 /* The HIP stream (hipStream_t) the handle launches on; d3x_set_stream replaces it (NULL =

@@ -312,6 +312,13 @@ int d3x_get_tuning(d3_handle_t *handle, double *bin_width_bohr, int *atm_blocks)
     return 0;
 }
 
+int d3x_get_atm_variant(d3_handle_t *handle) {
+    if (!handle) return -1;
+    int v = -1;
+    uint32_t st = guarded(H(handle), "d3x_get_atm_variant", [&] { v = H(handle)->eng->atm_var(); });
+    return st ? -1 : v;
+}
+
 double d3x_autotune(d3_handle_t *handle, const char *cache_path) {
     if (!handle) return -1.0;
     double best = -1.0;

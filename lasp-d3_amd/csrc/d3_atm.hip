@@ -88,27 +88,35 @@ __device__ __forceinline__ float wmax(float v) {
 // LDS carve-up, shared by the kernel and the host-side size query.
 template <class T>
 struct AtmLds {
-    // j tile: (dx, dy) (dz, a) (1/a, a^-1/2) (sqrt C6_ij, -); per type B of k, nref rows (y, dy)
-    // and one damping row (FP32: lam_j + (8/3) log2 R0_{A_j B}^2, FP64: lam_j R0_{A_j B}^2; 0)
-    T2<T> *J0, *J1, *J2, *J3, *Jyd;
-    T *Jacc;                         // [A_NF][64] j-side tile sums
+    // Rows of 64 pair-sized slots (T2), all at fixed offsets before the C6 rows, so that a pair
+    // step addresses every per-j value from one register (jl * sizeof(T2)) plus immediates:
+    //   J0..J3  j tile: (dx, dy) (dz, a) (1/a, a^-1/2) (sqrt C6_ij, lam_j)
+    //   Jp      j-side tile sums as pairs (A, Bx) (By, Bz) (H, G)       [3 rows]
+    //   Jt      without DR: the j type x ne, one int per slot            [1 row]
+    //   Jyd     per type B of k: nref rows (y, dy) and, with DR, one damping row (FP32: lam_j +
+    //           (8/3) log2 R0_{A_j B}^2, FP64: lam_j R0_{A_j B}^2; 0)
+    T2<T> *J0, *J1, *J2, *J3, *Jp, *Jyd;
+    int *Jt;
     T *Khi, *Klo;                    // [A_NF][64] k-side sums over the j tiles (two-sum pair in FP32; KL only)
     T *Cy, *Cdy;                     // centre rows [ne * nref]
     T *Rt;                           // per type pair: FP32 (8/3) log2 R0^2, FP64 R0^2
-    __host__ __device__ static size_t bytes(int ne, int nref, bool kl) {
+    static constexpr int TI = sizeof(T2<T>) / sizeof(int);  // ints per slot
+    __host__ __device__ static constexpr int fixed_rows(bool dr) { return dr ? 7 : 8; }
+    __host__ __device__ static size_t bytes(int ne, int nref, bool kl, bool dr) {
         const size_t ynr = (size_t)ne * nref;
-        return (4 + (size_t)ne * (nref + 1)) * AW * sizeof(T2<T>) + (size_t)(kl ? 3 : 1) * A_NF * AW * sizeof(T) +
-               2 * ynr * sizeof(T) + (size_t)ne * ne * sizeof(T);
+        return (fixed_rows(dr) + (size_t)ne * (nref + (dr ? 1 : 0))) * AW * sizeof(T2<T>) +
+               (size_t)(kl ? 2 : 0) * A_NF * AW * sizeof(T) + 2 * ynr * sizeof(T) + (size_t)ne * ne * sizeof(T);
     }
-    __device__ AtmLds(unsigned char *s, int ne, int nref, bool kl) {
+    __device__ AtmLds(unsigned char *s, int ne, int nref, bool kl, bool dr) {
         const size_t ynr = (size_t)ne * nref;
         J0 = reinterpret_cast<T2<T> *>(s);
         J1 = J0 + AW;
-        J2 = J1 + AW;
-        J3 = J2 + AW;
-        Jyd = J3 + AW;
-        Jacc = reinterpret_cast<T *>(Jyd + (size_t)ne * (nref + 1) * AW);
-        Khi = Jacc + A_NF * AW;
+        J2 = J0 + 2 * AW;
+        J3 = J0 + 3 * AW;
+        Jp = J0 + 4 * AW;
+        Jt = reinterpret_cast<int *>(J0 + 7 * AW);
+        Jyd = J0 + fixed_rows(dr) * AW;
+        Khi = reinterpret_cast<T *>(Jyd + (size_t)ne * (nref + (dr ? 1 : 0)) * AW);
         Klo = Khi + A_NF * AW;
         Cy = kl ? Klo + A_NF * AW : Khi;
         Cdy = Cy + ynr;
@@ -126,15 +134,18 @@ constexpr int atm_min_waves() {
 }
 
 // KL: the k side's tile sums go to lane-private LDS (two-sum pairs) and reach the FP64
-// accumulators once per k tile; otherwise they are added to them after every tile.  The host
-// picks KL when its larger LDS tile costs no resident blocks.
-template <class T, int NR, bool KL>
+// accumulators once per k tile; otherwise they are added to them after every tile.  DR: the
+// per-type-pair damping term is staged as an extra row per type of k (two dependent LDS reads and
+// three VALU fewer per pair, ne rows more LDS).  The host picks the pair of variants that keeps the
+// most blocks resident, preferring DR, then KL, on a tie.
+template <class T, int NR, bool KL, bool DR>
 __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> P) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr bool F32 = sizeof(T) == 4;
+    constexpr int NRD = NR + (DR ? 1 : 0);  // rows per type B in the j tile
     const int lane = threadIdx.x;
     const int ne = P.ne, ynr = ne * NR;
-    AtmLds<T> S(smem, ne, NR, KL);
+    AtmLds<T> S(smem, ne, NR, KL, DR);
 
     const int cap = P.atm_cap;
     int *Lj = P.atm_i32 + (size_t)blockIdx.x * cap * 2;
@@ -237,13 +248,18 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
             LT[F_LAM * cap + p] = F32 ? T((8.0f / 3.0f) * log2f((float)(r0 * r0 * ia))) : r0 * r0 * ia;
             LT[F_DLI * cap + p] = dci * ic6;
             LT[F_DLJ * cap + p] = dcj * ic6;
-            for (int q = 0; q < A_NF; ++q) La[q * cap + p] = 0.0;
+            // the accumulators take FP64 atomic adds in phase 3 (no returned value, so the wave
+            // never waits on them) and are read back with atomic loads (L2, not a stale L1 line)
+            for (int q = 0; q < A_NF; ++q)
+                __hip_atomic_store(&La[q * cap + p], 0.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         __syncthreads();
 
         // ---- phase 3: pairs {j, k}, j < k, of forward neighbours.  Per-pair terms are summed in T
         // over one 64 x 64 tile (at most 64 terms per sum); tile sums go to the FP64 per-neighbour
-        // accumulators La, which lane l alone touches for its slots.
+        // accumulators La, which lane l alone touches for its slots, as atomic adds: one lane's
+        // adds to one slot reach L2 in issue order, so the sums stay bitwise repeatable, and
+        // they cost no read latency (a read-modify-write here took 7.6 % of the kernel on Cu).
         uint64_t cnt = 0;
         const int ntile = (nf + AW - 1) / AW;
         for (int kb = 0; kb < ntile; ++kb) {
@@ -262,7 +278,7 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                 kb2 = LT[F_R2 * cap + k];
                 kib = LT[F_IR2 * cap + k];
                 krb = LT[F_RA * cap + k];
-                ksc6 = LT[F_SC6 * cap + k];
+                ksc6 = T(-0.5) * LT[F_SC6 * cap + k];  // -sqrt(C6_ik) / 2: C9 / 2 below
                 klam = F32 ? LT[F_LAM * cap + k] + T(L2K6_83) : T(K6) * LT[F_LAM * cap + k];
                 tk = Lt[k];
                 for (int a = 0; a < NR; ++a) {
@@ -274,7 +290,7 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
             const float kminx = uni(wmin(kval ? (float)kx : 3e38f)), kmaxx = uni(wmax(kval ? (float)kx : -3e38f));
             const float kminy = uni(wmin(kval ? (float)ky : 3e38f)), kmaxy = uni(wmax(kval ? (float)ky : -3e38f));
             const float kminz = uni(wmin(kval ? (float)kz : 3e38f)), kmaxz = uni(wmax(kval ? (float)kz : -3e38f));
-            const int ykb = tk * (NR + 1) * AW;  // rows of type(k) in the j tile
+            const T2<T> *Yk = S.Jyd + tk * NRD * AW;  // rows of type(k) in the j tile
             // k-side sums over all j tiles: lane-private LDS columns, folded to FP64 once per kb
             if constexpr (KL)
                 for (int q = 0; q < A_NF; ++q) S.Khi[q * AW + lane] = S.Klo[q * AW + lane] = T(0);
@@ -321,7 +337,8 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                     S.J0[lane] = T2<T>{jxs, jys};
                     S.J1[lane] = T2<T>{jzs, a};
                     S.J2[lane] = T2<T>{ia, ra};
-                    S.J3[lane] = T2<T>{sc6, T(0)};
+                    S.J3[lane] = T2<T>{sc6, lam};
+                    if constexpr (!DR) S.Jt[lane * S.TI] = tj * ne;  // row of the per-type-pair table
                     for (int B = 0; B < ne; ++B) {
                         for (int bb = 0; bb < NR; ++bb) {
                             T sy = T(0), sdy = T(0);
@@ -330,12 +347,14 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                                 sy += c * wj[q];
                                 sdy += c * dwj[q];
                             }
-                            S.Jyd[(B * (NR + 1) + bb) * AW + lane] = T2<T>{sy, sdy};
+                            S.Jyd[(B * NRD + bb) * AW + lane] = T2<T>{sy, sdy};
                         }
-                        const T rt = S.Rt[tj * ne + B];
-                        S.Jyd[(B * (NR + 1) + NR) * AW + lane] = T2<T>{F32 ? lam + rt : lam * rt, T(0)};
+                        if constexpr (DR) {
+                            const T rt = S.Rt[tj * ne + B];
+                            S.Jyd[(B * NRD + NR) * AW + lane] = T2<T>{F32 ? lam + rt : lam * rt, T(0)};
+                        }
                     }
-                    for (int q = 0; q < A_NF; ++q) S.Jacc[q * AW + lane] = T(0);
+                    for (int q = 0; q < 3; ++q) S.Jp[q * AW + lane] = T2<T>{T(0), T(0)};
                 }
                 wave_sync();
 
@@ -364,18 +383,24 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                     if (!ok) continue;
                     const T a = j1.y, b = kb2;
                     const T2<T> j2 = S.J2[jl], j3 = S.J3[jl];
-                    const T lamjk = S.Jyd[ykb + NR * AW + jl].x;  // damping row of type(k)
+                    T lamjk;  // damping term of the pair's types (with lam_j)
+                    if constexpr (DR) {
+                        lamjk = Yk[NR * AW + jl].x;
+                    } else {
+                        const T rt = S.Rt[S.Jt[jl * S.TI] + tk];
+                        lamjk = F32 ? j3.y + rt : j3.y * rt;
+                    }
                     // C6_jk and its CN derivatives from the j row for type(k)
                     T c6jk = T(0), dk6 = T(0), dj6 = T(0);
 #pragma unroll
                     for (int bb = 0; bb < NR; ++bb) {
-                        const T2<T> yd = S.Jyd[ykb + bb * AW + jl];
+                        const T2<T> yd = Yk[bb * AW + jl];
                         c6jk += yd.x * kw[bb];
                         dk6 += yd.x * kdw[bb];
                         dj6 += yd.y * kw[bb];
                     }
                     const T rs = prsqrt(c6jk);
-                    const T C9 = -(j3.x * ksc6) * (c6jk * rs);
+                    const T C9h = (j3.x * ksc6) * (c6jk * rs);  // C9 / 2
                     const T ic6 = rs * rs;
                     const T rcq = prsqrt(c);
                     const T ic = rcq * rcq;
@@ -388,7 +413,7 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                     const T dQa = p23 + u1 * d32, dQb = p23 - u1 * d32, dQc = u1 * (u2 + u3) - p23;
                     const T QiR5 = Q * iR5;
                     const T A = iR3 + T(0.375) * QiR5;
-                    const T P1 = T(-1.5) * iR3 - T(0.9375) * QiR5;
+                    const T P4 = T(-6) * iR3 - T(3.75) * QiR5;  // 4 P1
                     T x16;
                     if constexpr (F32) {
                         x16 = __builtin_amdgcn_exp2f(lamjk + klam - (8.0f / 3.0f) * __builtin_amdgcn_logf(c));
@@ -397,10 +422,10 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                         x16 = y * y * y * inv_cbrt(y);
                     }
                     const T f = prcp(T(1) + T(6) * x16);
-                    const T cf = C9 * f;
-                    const T hC = T(0.5) * cf * A;  // e / 2
-                    const T G2 = T(2) * cf * (P1 + T(8.0 / 3.0) * A * (T(1) - f));
-                    const T H2 = T(0.75) * cf * iR5;
+                    const T cfh = C9h * f;
+                    const T hC = cfh * A;  // e / 2
+                    const T G2 = cfh * (P4 + T(32.0 / 3.0) * A * (T(1) - f));
+                    const T H2 = T(1.5) * cfh * iR5;
                     const T tij = G2 * j2.x + H2 * dQa;
                     const T tik = G2 * kib + H2 * dQb;
                     const T tjk = G2 * ic + H2 * dQc;
@@ -409,19 +434,23 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                     // read-modify-write (LDS float atomics measured ~6x slower on gfx950); FP64:
                     // LDS atomics (no per-step barrier, measured ~4% faster than the barrier form).
                     if constexpr (F32) {
-                        S.Jacc[A_A * AW + jl] += tij + tjk;
-                        S.Jacc[A_BX * AW + jl] += tjk * kx;
-                        S.Jacc[A_BY * AW + jl] += tjk * ky;
-                        S.Jacc[A_BZ * AW + jl] += tjk * kz;
-                        S.Jacc[A_H * AW + jl] += hC;
-                        S.Jacc[A_G * AW + jl] += hc6 * dj6;
+                        T2<T> p0 = S.Jp[jl], p1 = S.Jp[AW + jl], p2 = S.Jp[2 * AW + jl];
+                        p0.x += tij + tjk;
+                        p0.y += tjk * kx;
+                        p1.x += tjk * ky;
+                        p1.y += tjk * kz;
+                        p2.x += hC;
+                        p2.y += hc6 * dj6;
+                        S.Jp[jl] = p0;
+                        S.Jp[AW + jl] = p1;
+                        S.Jp[2 * AW + jl] = p2;
                     } else {
-                        atomicAdd(&S.Jacc[A_A * AW + jl], tij + tjk);
-                        atomicAdd(&S.Jacc[A_BX * AW + jl], tjk * kx);
-                        atomicAdd(&S.Jacc[A_BY * AW + jl], tjk * ky);
-                        atomicAdd(&S.Jacc[A_BZ * AW + jl], tjk * kz);
-                        atomicAdd(&S.Jacc[A_H * AW + jl], hC);
-                        atomicAdd(&S.Jacc[A_G * AW + jl], hc6 * dj6);
+                        atomicAdd(&S.Jp[jl].x, tij + tjk);
+                        atomicAdd(&S.Jp[jl].y, tjk * kx);
+                        atomicAdd(&S.Jp[AW + jl].x, tjk * ky);
+                        atomicAdd(&S.Jp[AW + jl].y, tjk * kz);
+                        atomicAdd(&S.Jp[2 * AW + jl].x, hC);
+                        atomicAdd(&S.Jp[2 * AW + jl].y, hc6 * dj6);
                     }
                     // neighbour k: registers
                     kA += tik + tjk;
@@ -444,17 +473,21 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                             S.Khi[q * AW + lane] = s;
                         }
                     } else if (kval) {
-                        for (int q = 0; q < A_NF; ++q) La[q * cap + k] += (double)v[q];
+                        for (int q = 0; q < A_NF; ++q) unsafeAtomicAdd(&La[q * cap + k], (double)v[q]);
                     }
                 }
                 wave_sync();
                 if (jval)
-                    for (int q = 0; q < A_NF; ++q) La[q * cap + pj] += (double)S.Jacc[q * AW + lane];
+                    for (int q = 0; q < 3; ++q) {
+                        const T2<T> pq = S.Jp[q * AW + lane];
+                        unsafeAtomicAdd(&La[(2 * q) * cap + pj], (double)pq.x);
+                        unsafeAtomicAdd(&La[(2 * q + 1) * cap + pj], (double)pq.y);
+                    }
             }
             if constexpr (KL)
                 if (kval)
                     for (int q = 0; q < A_NF; ++q)
-                        La[q * cap + k] += (double)S.Khi[q * AW + lane] + (double)S.Klo[q * AW + lane];
+                        unsafeAtomicAdd(&La[q * cap + k], (double)S.Khi[q * AW + lane] + (double)S.Klo[q * AW + lane]);
         }
 
         // ---- phase 4.  Per neighbour slot p: F_p = grad_p E = -d_p A_p + B_p, H_p = sum of e/2,
@@ -472,10 +505,12 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
             const int j = Lj[p];
             const double dx = (double)LT[F_DX * cap + p], dy = (double)LT[F_DY * cap + p],
                          dz = (double)LT[F_DZ * cap + p];
-            const double Ap = La[A_A * cap + p];
-            const double fx = La[A_BX * cap + p] - dx * Ap, fy = La[A_BY * cap + p] - dy * Ap,
-                         fz = La[A_BZ * cap + p] - dz * Ap;
-            const double H = La[A_H * cap + p], Gc = La[A_G * cap + p];
+            double acc[A_NF];
+            for (int q = 0; q < A_NF; ++q)
+                acc[q] = __hip_atomic_load(&La[q * cap + p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const double Ap = acc[A_A];
+            const double fx = acc[A_BX] - dx * Ap, fy = acc[A_BY] - dy * Ap, fz = acc[A_BZ] - dz * Ap;
+            const double H = acc[A_H], Gc = acc[A_G];
             atomicAdd(&P.fa[3 * j + 0], fx);
             atomicAdd(&P.fa[3 * j + 1], fy);
             atomicAdd(&P.fa[3 * j + 2], fz);
@@ -533,23 +568,28 @@ __global__ void k_merge(Params<T> P) {
 }
 
 // One instantiation per reference count (1..MAX_NREF): the C6 dot products and the LDS rows
-// are sized by the references the handle's elements actually have.
-template <class T, bool KL, int... NR>
+// are sized by the references the handle's elements actually have.  var: bit 0 = KL, bit 1 = DR.
+template <class T, bool KL, bool DR, int... NR>
 const void *atm_pick(int nref, std::integer_sequence<int, NR...>) {
     const void *fn = nullptr;
-    ((fn = (nref == NR + 1) ? reinterpret_cast<const void *>(&k_atm<T, NR + 1, KL>) : fn), ...);
+    ((fn = (nref == NR + 1) ? reinterpret_cast<const void *>(&k_atm<T, NR + 1, KL, DR>) : fn), ...);
     return fn;
 }
 
 template <class T>
-const void *atm_kernel(int nref, bool kl) {
-    return kl ? atm_pick<T, true>(nref, std::make_integer_sequence<int, MAX_NREF>{})
-              : atm_pick<T, false>(nref, std::make_integer_sequence<int, MAX_NREF>{});
+const void *atm_kernel(int nref, int var) {
+    using S = std::make_integer_sequence<int, MAX_NREF>;
+    switch (var & 3) {
+        case 0: return atm_pick<T, false, false>(nref, S{});
+        case 1: return atm_pick<T, true, false>(nref, S{});
+        case 2: return atm_pick<T, false, true>(nref, S{});
+        default: return atm_pick<T, true, true>(nref, S{});
+    }
 }
 
-int blocks_per_cu(int ne, int nref, size_t ts, bool kl) {
-    const size_t lds = atm_lds_bytes(ne, nref, ts, kl);
-    const void *fn = ts == sizeof(double) ? atm_kernel<double>(nref, kl) : atm_kernel<float>(nref, kl);
+int blocks_per_cu(int ne, int nref, size_t ts, int var) {
+    const size_t lds = atm_lds_bytes(ne, nref, ts, var);
+    const void *fn = ts == sizeof(double) ? atm_kernel<double>(nref, var) : atm_kernel<float>(nref, var);
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, AW, lds) != hipSuccess) per_cu = 0;
     return per_cu;
@@ -557,17 +597,27 @@ int blocks_per_cu(int ne, int nref, size_t ts, bool kl) {
 
 }  // namespace
 
-size_t atm_lds_bytes(int ne, int nref, size_t ts, bool kl) {
-    return ts == sizeof(double) ? AtmLds<double>::bytes(ne, nref, kl) : AtmLds<float>::bytes(ne, nref, kl);
+size_t atm_lds_bytes(int ne, int nref, size_t ts, int var) {
+    const bool kl = var & ATM_KL, dr = var & ATM_DR;
+    return ts == sizeof(double) ? AtmLds<double>::bytes(ne, nref, kl, dr) : AtmLds<float>::bytes(ne, nref, kl, dr);
 }
 
-bool atm_choose_kl(int ne, int nref, size_t ts) {
-    if (atm_lds_bytes(ne, nref, ts, true) > 65536) return false;
-    return blocks_per_cu(ne, nref, ts, true) >= blocks_per_cu(ne, nref, ts, false);
+int atm_choose_variant(int ne, int nref, size_t ts) {
+    // most resident blocks; on a tie the variants that save work per pair (DR, then KL)
+    int best = 0, best_blocks = -1;
+    for (int var : {int(ATM_KL | ATM_DR), int(ATM_DR), int(ATM_KL), 0}) {
+        if (atm_lds_bytes(ne, nref, ts, var) > 65536) continue;
+        const int b = blocks_per_cu(ne, nref, ts, var);
+        if (b > best_blocks) {
+            best = var;
+            best_blocks = b;
+        }
+    }
+    return best;
 }
 
-int atm_resident_blocks(int ne, int nref, size_t ts, bool kl) {
-    const int per_cu = blocks_per_cu(ne, nref, ts, kl);
+int atm_resident_blocks(int ne, int nref, size_t ts, int var) {
+    const int per_cu = blocks_per_cu(ne, nref, ts, var);
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
@@ -581,9 +631,8 @@ void launch_atm(const Params<T> &P, int nblocks, hipStream_t s) {
     const size_t nz = (size_t)6 * P.n;
     if (nz) hipLaunchKernelGGL(k_fill_d, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, s, P.fa, nz, P.atm_next);
     if (P.n == 0 || !(P.rc_32 > T(0))) return;
-    const bool kl = P.atm_kl != 0;
-    const size_t lds = atm_lds_bytes(P.ne, P.nref, sizeof(T), kl);
-    const void *fn = atm_kernel<T>(P.nref, kl);
+    const size_t lds = atm_lds_bytes(P.ne, P.nref, sizeof(T), P.atm_var);
+    const void *fn = atm_kernel<T>(P.nref, P.atm_var);
     if (!fn) return;  // not reached: a handle's reference count is 1..MAX_NREF
     Params<T> arg = P;
     void *args[] = {&arg};

@@ -86,7 +86,7 @@ struct Params {
     // triangles anchored at each centre (deterministic total).  atm_next: the next centre to
     // take (zeroed before every launch).
     int atm_cap;
-    int atm_kl;  // k-side tile sums in LDS (1) or straight to the FP64 accumulators (0)
+    int atm_var;  // three-body kernel variant: ATM_KL | ATM_DR bits (d3_atm.hip)
     int *atm_i32;
     T *atm_T;
     double *atm_acc;
@@ -119,13 +119,15 @@ struct Launch {
 // Three-body term (d3_atm.hip): zeroes the ATM per-atom arrays, runs the triangle kernel on
 // nblocks persistent one-wave blocks (when P.rc_32 > 0) and merges forces and dE/dCN into
 // P.f / P.g.  atm_resident_blocks: how many of those blocks the current device holds at once.
-// kl selects where the k side's tile sums go (P.atm_kl, see d3_atm.hip); atm_choose_kl picks
-// it for a handle's element count, reference count and precision.
+// The kernel variant (P.atm_var) holds two bits: ATM_KL keeps the k side's tile sums in LDS,
+// ATM_DR stages the per-type damping term with the C6 rows; atm_choose_variant picks them for a
+// handle's element count, reference count and precision.
+enum { ATM_KL = 1, ATM_DR = 2 };
 template <class T>
 void launch_atm(const Params<T> &P, int nblocks, hipStream_t s);
-size_t atm_lds_bytes(int ne, int nref, size_t tsize, bool kl);
-bool atm_choose_kl(int ne, int nref, size_t tsize);
-int atm_resident_blocks(int ne, int nref, size_t tsize, bool kl);
+size_t atm_lds_bytes(int ne, int nref, size_t tsize, int var);
+int atm_choose_variant(int ne, int nref, size_t tsize);
+int atm_resident_blocks(int ne, int nref, size_t tsize, int var);
 // P.f += fa, P.g += ga (what launch_atm ends with).
 template <class T>
 void launch_atm_merge(const Params<T> &P, hipStream_t s);

@@ -5,6 +5,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 
@@ -186,7 +188,7 @@ Engine::Engine(const uint16_t *elements, uint64_t num_elements, uint64_t max_len
                         fail(D3X_ERR_INPUT, "parameter table CN references do not factorise per element");
                     if (valid) c6[(((size_t)A * ne + B) * nr + a) * nr + b] = tab.c6ab[ix + 0];
                 }
-    if (rc3_ > 0 && atm_lds_bytes(ne_, nref_, precision == D3X_FP64 ? 8 : 4, false) > 65536)
+    if (rc3_ > 0 && atm_lds_bytes(ne_, nref_, precision == D3X_FP64 ? 8 : 4, 0) > 65536)
         fail(D3X_ERR_INPUT, "too many distinct elements for the three-body kernel's LDS tile");
     // layout of h_tab_: c6 | cnref | rcov | r2r4 | r0ab  (q4 / rdamp derived at upload)
     h_tab_.clear();
@@ -570,7 +572,7 @@ Params<T> Engine::params() {
     P.rdamp = tab + tab_off_[4];
     P.r0 = tab + tab_off_[5];
     P.atm_cap = atm_cap_;
-    P.atm_kl = rc3_ > 0 ? atm_kl() : 0;
+    P.atm_var = rc3_ > 0 ? atm_var() : 0;
     P.atm_i32 = (int *)d_atm_i32;
     P.atm_T = (T *)d_atm_T;
     P.atm_acc = (double *)d_atm_acc;
@@ -698,8 +700,9 @@ std::string Engine::tune_key() const {
 }
 
 namespace {
-// Cache file: one JSON object, one entry per line:  "key": {"bin_width": w, "atm_blocks": b, "ms": t}
-bool cache_lookup(const std::string &path, const std::string &key, double &w, int &b) {
+// Cache file: one JSON object, one entry per line:
+//   "key": {"bin_width": w, "atm_blocks": b, "atm_variant": v, "ms": t}   (v = -1: not tuned)
+bool cache_lookup(const std::string &path, const std::string &key, double &w, int &b, int &v) {
     FILE *f = std::fopen(path.c_str(), "r");
     if (!f) return false;
     char line[512];
@@ -712,12 +715,14 @@ bool cache_lookup(const std::string &path, const std::string &key, double &w, in
         if (pw == std::string::npos || pb == std::string::npos) continue;
         w = std::atof(s.c_str() + pw + 12);
         b = std::atoi(s.c_str() + pb + 13);
+        const size_t pv = s.find("\"atm_variant\":");
+        v = pv == std::string::npos ? -1 : std::atoi(s.c_str() + pv + 14);
         found = w > 0 && b > 0;
     }
     std::fclose(f);
     return found;
 }
-void cache_store(const std::string &path, const std::string &key, double w, int b, double ms) {
+void cache_store(const std::string &path, const std::string &key, double w, int b, int v, double ms) {
     std::vector<std::string> keep;
     if (FILE *f = std::fopen(path.c_str(), "r")) {
         char line[512];
@@ -731,8 +736,8 @@ void cache_store(const std::string &path, const std::string &key, double w, int 
         std::fclose(f);
     }
     char buf[512];
-    std::snprintf(buf, sizeof(buf), "  \"%s\": {\"bin_width\": %.6f, \"atm_blocks\": %d, \"ms\": %.4f}", key.c_str(), w,
-                  b, ms);
+    std::snprintf(buf, sizeof(buf), "  \"%s\": {\"bin_width\": %.6f, \"atm_blocks\": %d, \"atm_variant\": %d, \"ms\": %.4f}",
+                  key.c_str(), w, b, v, ms);
     keep.push_back(buf);
     const std::string tmp = path + ".tmp";
     FILE *f = std::fopen(tmp.c_str(), "w");
@@ -758,10 +763,12 @@ void Engine::set_tuning(double bin_width, int atm_blocks) {
 void Engine::apply_tune_cache() {
     if (tune_cache_.empty()) return;
     double w;
-    int b;
-    if (cache_lookup(tune_cache_, tune_key(), w, b)) {
+    int b, v;
+    if (cache_lookup(tune_cache_, tune_key(), w, b, v)) {
         bin_width_ = w;
         atm_blocks_req_ = b;
+        const size_t ts = precision_ == D3X_FP64 ? sizeof(double) : sizeof(float);
+        if (v >= 0 && v <= 3 && atm_lds_bytes(ne_, nref_, ts, v) <= 65536) set_atm_variant(v);
     }
 }
 
@@ -785,6 +792,17 @@ double Engine::timed_evaluation() {
 
 double Engine::autotune(const std::string &cache_path) {
     if (!atoms_set_) fail(D3X_ERR_STATE, "set_atoms has not been called");
+    // D3_TUNE_LOG=1: one line per timed candidate on stderr (a sweep over a large batch in FP64
+    // runs for minutes)
+    const char *lg = std::getenv("D3_TUNE_LOG");
+    const bool verbose = lg && *lg && std::strcmp(lg, "0") != 0;
+    auto report = [&](double w, int b, double t, int var) {
+        if (!verbose) return;
+        if (var >= 0)
+            std::fprintf(stderr, "d3 autotune: bin %.2f bohr, three-body variant %d, blocks %d: %.2f ms\n", w, var, b, t);
+        else
+            std::fprintf(stderr, "d3 autotune: bin %.2f bohr, three-body blocks %d: %.2f ms\n", w, b, t);
+    };
     const bool graph = graph_mode_;
     set_graph(false);
     const double base = 0.5 * std::min(rccn_, rc2_);
@@ -796,6 +814,7 @@ double Engine::autotune(const std::string &cache_path) {
         bin_width_ = w;
         geom_dirty_ = true;
         const double t = timed_evaluation();
+        report(w, atm_blocks_effective(), t, -1);
         if (t < best) {
             best = t;
             best_w = w;
@@ -803,6 +822,26 @@ double Engine::autotune(const std::string &cache_path) {
     }
     bin_width_ = best_w;
     geom_dirty_ = true;
+    if (rc3_ > 0) {
+        // three-body kernel variants (KL, DR) that fit, each at its own resident block count
+        const size_t ts = precision_ == D3X_FP64 ? sizeof(double) : sizeof(float);
+        const int req = atm_blocks_req_;
+        int best_v = atm_var();
+        best = 1e300;
+        for (int v = 0; v < 4; ++v) {
+            if (atm_lds_bytes(ne_, nref_, ts, v) > 65536) continue;
+            set_atm_variant(v);
+            atm_blocks_req_ = 0;
+            const double t = timed_evaluation();
+            report(bin_width_, atm_resident(), t, v);
+            if (t < best) {
+                best = t;
+                best_v = v;
+            }
+        }
+        set_atm_variant(best_v);
+        atm_blocks_req_ = req;
+    }
     const int res = atm_resident();
     int best_b = atm_blocks_req_ > 0 ? atm_blocks_req_ : res;
     if (rc3_ > 0) {
@@ -812,6 +851,7 @@ double Engine::autotune(const std::string &cache_path) {
         for (int b : {res / 2, (3 * res) / 4, res, 2 * res}) {
             atm_blocks_req_ = b;
             const double t = timed_evaluation();
+            report(bin_width_, b, t, -1);
             if (t < best) {
                 best = t;
                 best_b = b;
@@ -823,7 +863,7 @@ double Engine::autotune(const std::string &cache_path) {
     atm_blocks_req_ = best_b;
     tuning_explicit_ = true;
     const std::string path = cache_path.empty() ? tune_cache_ : cache_path;
-    if (!path.empty()) cache_store(path, tune_key(), best_w, best_b, best);
+    if (!path.empty()) cache_store(path, tune_key(), best_w, best_b, rc3_ > 0 ? atm_var() : -1, best);
     set_graph(graph);
     return best;
 }

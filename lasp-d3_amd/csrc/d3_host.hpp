@@ -81,6 +81,10 @@ class Engine {
     int atm_blocks_request() const { return atm_blocks_req_; }
     // Three-body grid of the next evaluation: the request, else the device's resident count.
     int atm_blocks_effective() { return atm_blocks_req_ > 0 ? atm_blocks_req_ : atm_resident(); }
+    int atm_var() {
+        if (atm_var_ < 0) atm_var_ = atm_choose_variant(ne_, nref_, precision_ == 0 ? sizeof(double) : sizeof(float));
+        return atm_var_;
+    }
 
     // Three-body stand-in (modes of include/d3x_sur.h: 0 exact, 1 network with judge, 2 exact +
     // features).  The judge itself runs in the C API's evaluation (it needs host results):
@@ -171,15 +175,14 @@ class Engine {
     void *d_atm_pa = nullptr;  // per-atom ATM results: fa[3N] | ga[N] | eas[N] | eat[N] (double)
     int atm_blocks_ = 0, atm_cap_ = 0;
     int atm_resident_ = 0;  // three-body blocks resident on the device at once (0: not queried)
-    int atm_kl_ = -1;       // three-body kernel variant (-1: not chosen yet)
-    int atm_kl() {
-        if (atm_kl_ < 0) atm_kl_ = atm_choose_kl(ne_, nref_, precision_ == 0 ? sizeof(double) : sizeof(float)) ? 1 : 0;
-        return atm_kl_;
+    int atm_var_ = -1;      // three-body kernel variant, ATM_KL | ATM_DR (-1: not chosen yet)
+    void set_atm_variant(int v) {  // a fixed variant (autotuner, tuning cache); resident count follows
+        atm_var_ = v;
+        atm_resident_ = 0;
     }
     int atm_resident() {
         if (atm_resident_ <= 0)
-            atm_resident_ =
-                atm_resident_blocks(ne_, nref_, precision_ == 0 ? sizeof(double) : sizeof(float), atm_kl() != 0);
+            atm_resident_ = atm_resident_blocks(ne_, nref_, precision_ == 0 ? sizeof(double) : sizeof(float), atm_var());
         return atm_resident_;
     }
     size_t atm_alloc_entries_ = 0;  // allocated scratch entries (>= atm_blocks_ * atm_cap_)

@@ -66,6 +66,7 @@ SIGNATURES = {
     "d3x_set_graph": (C.c_int, [_vp, C.c_int]),
     "d3x_set_tuning": (C.c_int, [_vp, C.c_double, C.c_int]),
     "d3x_get_tuning": (C.c_int, [_vp, _dp, _ip]),
+    "d3x_get_atm_variant": (C.c_int, [_vp]),
     "d3x_autotune": (C.c_double, [_vp, C.c_char_p]),
     "d3x_get_stream": (_vp, [_vp]),
     "d3x_set_stream": (C.c_int, [_vp, _vp]),
@@ -239,6 +240,13 @@ class Handle:
         w, b = C.c_double(), C.c_int()
         self._check(self.lib.d3x_get_tuning(self.h, C.byref(w), C.byref(b)), "d3x_get_tuning")
         return w.value, b.value
+
+    def atm_variant(self) -> int:
+        """Three-body kernel variant in use (bit 0: k-side sums in LDS, bit 1: damping rows)."""
+        v = self.lib.d3x_get_atm_variant(self.h)
+        if v < 0:
+            raise D3Error("d3x_get_atm_variant", v)
+        return v
 
     def autotune(self, cache_path: str | None = None) -> float:
         t = self.lib.d3x_autotune(self.h, cache_path.encode() if cache_path else None)

@@ -56,6 +56,7 @@ def test_autotune_and_cache(tmp_path):
     assert len(data) == 1
     (key, entry), = data.items()
     assert entry["bin_width"] == pytest.approx(w) and entry["atm_blocks"] == b
+    assert entry["atm_variant"] in (0, 1, 2, 3)
     # a new handle with the cache in the environment picks the tuned values for the same input
     os.environ["D3_TUNE_CACHE"] = cache
     try:
@@ -65,7 +66,55 @@ def test_autotune_and_cache(tmp_path):
         h2.set_cell(s["cell"])
         e2, _, _ = h2.compute()
         assert h2.tuning() == (pytest.approx(w), b)
+        assert h2.atm_variant() == entry["atm_variant"]
         assert abs(e2 - e0) <= 1e-12 * abs(e0)
         h2.close()
+    finally:
+        del os.environ["D3_TUNE_CACHE"]
+
+
+def write_cache(path, data):
+    """The engine's cache layout: a JSON object with one entry per line."""
+    rows = [f"  {json.dumps(k)}: {json.dumps(v)}" for k, v in data.items()]
+    with open(path, "w") as f:
+        f.write("{\n" + ",\n".join(rows) + "\n}\n")
+
+
+def test_every_three_body_variant(tmp_path):
+    """Each three-body kernel variant (k-side sums in LDS or not, damping rows or not), forced
+    through the tuning cache, gives the same result: FP64 at the parity bar, FP32 within its error
+    budget of the FP64 result."""
+    s = systems.triclinic_mixed(n=24, seed=9)
+    kw = dict(cutoff=s["cutoff"], cn_cutoff=s["cn_cutoff"], atm_cutoff=s["atm"], damping=1, functional=2)
+    cache = str(tmp_path / "tune.json")
+    ref = None
+    for prec in (d3_lib.D3X_FP64, d3_lib.D3X_FP32):
+        h = d3_lib.Handle(s["Z"], len(s["Z"]), precision=prec, **kw)
+        h.set_atoms(s["pos"], s["Z"])
+        h.set_cell(s["cell"])
+        h.compute()
+        h.autotune(cache)  # writes the entry (and its key) for this input and precision
+        h.close()
+    data = json.load(open(cache))
+    assert len(data) == 2
+    os.environ["D3_TUNE_CACHE"] = cache
+    try:
+        for prec in (d3_lib.D3X_FP64, d3_lib.D3X_FP32):
+            for v in range(4):
+                for key in data:
+                    data[key]["atm_variant"] = v
+                write_cache(cache, data)
+                h = d3_lib.Handle(s["Z"], len(s["Z"]), precision=prec, **kw)
+                h.set_atoms(s["pos"], s["Z"])
+                h.set_cell(s["cell"])
+                e, f, st = h.compute()
+                assert h.atm_variant() == v
+                h.close()
+                if ref is None:
+                    ref = (e, f, st)
+                    continue
+                fp64 = prec == d3_lib.D3X_FP64
+                assert abs(e - ref[0]) <= (1e-12 if fp64 else 2e-6) * abs(ref[0]), (prec, v)
+                assert np.max(np.abs(f - ref[1])) <= (1e-9 if fp64 else 2e-5 * np.max(np.abs(ref[1]))), (prec, v)
     finally:
         del os.environ["D3_TUNE_CACHE"]
