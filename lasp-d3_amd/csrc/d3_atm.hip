@@ -32,6 +32,7 @@
 // per-pair parts; FP64 as y^3 y^(-1/3).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
 #include <utility>
 
 #include "d3_device.hpp"
@@ -360,9 +361,10 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
 
                 // tile-local sums of neighbour k (folded into FP64 per tile)
                 T kA = T(0), kBx = T(0), kBy = T(0), kBz = T(0), kH = T(0), kG = T(0);
-                int jlim = (jt == kb) ? lane : AW;  // diagonal tile: j < k
-                // opaque VGPR: j < jlim stays one compare (its ballot is then the compare's mask)
-                asm volatile("" : "+v"(jlim));
+                // The diagonal tile (jt == kb) keeps only j < k; the other tiles need no order
+                // test, so the step loop is instantiated for both.
+                auto steps = [&](auto diag) {
+                constexpr bool DIAG = decltype(diag)::value;
 #pragma unroll 1
                 for (int q = 0; q < AW; ++q) {
                     // FP32: lane l updates the j slot that lane l+1 updated one step earlier
@@ -372,12 +374,16 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                     const T jx = j0.x, jy = j0.y, jz = j1.x;
                     const T vx = kx - jx, vy = ky - jy, vz = kz - jz;  // r_j - r_k
                     const T c = vx * vx + vy * vy + vz * vz;
-                    const bool okj = jl < jlim, okc = c <= rc32, okz = c >= T(1e-12);
-                    const bool ok = okj && okc && okz;
+                    const bool okc = c <= rc32, okz = c >= T(1e-12);
+                    bool ok = okc && okz;
                     // one ballot per compare: the ballot of the combined flag would be re-derived
                     // from a VGPR copy of it
-                    const uint64_t m = __builtin_amdgcn_ballot_w64(okj) & __builtin_amdgcn_ballot_w64(okc) &
-                                       __builtin_amdgcn_ballot_w64(okz);
+                    uint64_t m = __builtin_amdgcn_ballot_w64(okc) & __builtin_amdgcn_ballot_w64(okz);
+                    if constexpr (DIAG) {
+                        const bool okj = jl < lane;
+                        m &= __builtin_amdgcn_ballot_w64(okj);
+                        ok = ok && okj;
+                    }
                     if (m == 0) continue;  // uniform
                     cnt += __popcll(m);
                     if (!ok) continue;
@@ -406,11 +412,13 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                     const T ic = rcq * rcq;
                     const T iR = j2.y * krb * rcq;
                     const T iR2 = iR * iR, iR3 = iR2 * iR, iR5 = iR3 * iR2;
-                    const T tsum = a + b + c;
-                    const T u1 = tsum - T(2) * c, u2 = tsum - T(2) * b, u3 = tsum - T(2) * a;
-                    const T p23 = u2 * u3, d32 = u3 - u2;
-                    const T Q = u1 * p23;
-                    const T dQa = p23 + u1 * d32, dQb = p23 - u1 * d32, dQc = u1 * (u2 + u3) - p23;
+                    // Q = x y z, x = a + b - c, y = c - d, z = c + d with d = b - a:
+                    // dQ/da = yz + 2 x d, dQ/db = yz - 2 x d, dQ/dc = 2 x c - yz
+                    const T d = b - a, x = (a + b) - c;
+                    const T yz = (c - d) * (c + d);
+                    const T Q = x * yz;
+                    const T x2 = x + x;
+                    const T dQa = yz + x2 * d, dQb = yz - x2 * d, dQc = x2 * c - yz;
                     const T QiR5 = Q * iR5;
                     const T A = iR3 + T(0.375) * QiR5;
                     const T P4 = T(-6) * iR3 - T(3.75) * QiR5;  // 4 P1
@@ -460,6 +468,11 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                     kH += hC;
                     kG += hc6 * dk6;
                 }
+                };
+                if (jt == kb)
+                    steps(std::true_type{});
+                else
+                    steps(std::false_type{});
                 {
                     // fold the tile sums of neighbour k (exact two-sum in FP32, plain in FP64)
                     const T v[A_NF] = {kA, kBx, kBy, kBz, kH, kG};
