@@ -187,37 +187,41 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
         // ---- phase 1: forward neighbours within rc3 (1e-12 <= r^2 <= rc3^2, src/d3_kernel.cu:1268)
         int count = 0;
         const int s0 = C.S->st_off[CUT_ATM], s1 = s0 + C.S->st_cnt[CUT_ATM];
-        for (int s = s0; s < s1; ++s) {
-            int j0, j1;
-            T ox, oy, oz;
-            C.bin(P, s, j0, j1, ox, oy, oz);
-            if (j1 <= i) continue;  // bins are sorted: every atom here has index < i
-            const StEntry d = P.st_d[s];
-            const bool dlex = d.dx > 0 || (d.dx == 0 && (d.dy > 0 || (d.dy == 0 && d.dz > 0)));
-            for (int jb = j0; jb < j1; jb += AW) {
-                const int j = jb + lane;
-                bool in = false;
-                T dx = T(0), dy = T(0), dz = T(0), r2 = T(0);
-                if (j < j1 && (j > i || (j == i && dlex))) {
-                    dx = ox - P.lx[j];
-                    dy = oy - P.ly[j];
-                    dz = oz - P.lz[j];
-                    r2 = dx * dx + dy * dy + dz * dz;
-                    in = r2 <= rc32 && r2 >= T(1e-12);
-                }
-                const uint64_t m = ballot64(in);
-                if (m == 0) continue;
-                if (in) {
-                    const int pos = count + mbcnt64(m);
-                    if (pos < cap) {
-                        Lj[pos] = j;
-                        LT[F_DX * cap + pos] = dx;
-                        LT[F_DY * cap + pos] = dy;
-                        LT[F_DZ * cap + pos] = dz;
-                        LT[F_R2 * cap + pos] = r2;
+        for (int sb = s0; sb < s1; sb += AW) {
+            const auto Bn = C.bins(P, sb, s1);
+            const int nsb = s1 - sb < AW ? s1 - sb : AW;
+            for (int sk = 0; sk < nsb; ++sk) {
+                int j0, j1;
+                T ox, oy, oz;
+                Bn.get(sk, j0, j1, ox, oy, oz);
+                if (j1 <= i) continue;  // bins are sorted: every atom here has index < i
+                const StEntry d = P.st_d[sb + sk];
+                const bool dlex = d.dx > 0 || (d.dx == 0 && (d.dy > 0 || (d.dy == 0 && d.dz > 0)));
+                for (int jb = j0; jb < j1; jb += AW) {
+                    const int j = jb + lane;
+                    bool in = false;
+                    T dx = T(0), dy = T(0), dz = T(0), r2 = T(0);
+                    if (j < j1 && (j > i || (j == i && dlex))) {
+                        dx = ox - P.lx[j];
+                        dy = oy - P.ly[j];
+                        dz = oz - P.lz[j];
+                        r2 = dx * dx + dy * dy + dz * dz;
+                        in = r2 <= rc32 && r2 >= T(1e-12);
                     }
+                    const uint64_t m = ballot64(in);
+                    if (m == 0) continue;
+                    if (in) {
+                        const int pos = count + mbcnt64(m);
+                        if (pos < cap) {
+                            Lj[pos] = j;
+                            LT[F_DX * cap + pos] = dx;
+                            LT[F_DY * cap + pos] = dy;
+                            LT[F_DZ * cap + pos] = dz;
+                            LT[F_R2 * cap + pos] = r2;
+                        }
+                    }
+                    count += __popcll(m);
                 }
-                count += __popcll(m);
             }
         }
         // Over capacity: flag it and run the centre with no neighbours (its results are zeros);

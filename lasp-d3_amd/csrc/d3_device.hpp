@@ -26,6 +26,17 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
+// Lane k's value of v, as a wave-uniform value (k uniform).
+__device__ __forceinline__ float readlane_t(float v, int k) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), k));
+}
+__device__ __forceinline__ double readlane_t(double v, int k) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, k);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), k);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
 __device__ __forceinline__ int wrapi(int x, int n) {
     int r = x % n;
     return r < 0 ? r + n : r;
@@ -127,6 +138,26 @@ struct Centre {
         ox = x + P.st_D[4 * s + 0];
         oy = y + P.st_D[4 * s + 1];
         oz = z + P.st_D[4 * s + 2];
+    }
+    // Up to 64 stencil entries at once: lane l holds entry s + l, so the wave issues one batch of
+    // vector loads (and the wrap's integer divisions) for 64 entries instead of a chain of
+    // dependent scalar loads per entry; get(k) reads entry s + k back as wave-uniform values.
+    struct Bins {
+        int j0, j1;
+        T ox, oy, oz;
+        __device__ void get(int k, int &a, int &b, T &px, T &py, T &pz) const {
+            a = __builtin_amdgcn_readlane(j0, k);
+            b = __builtin_amdgcn_readlane(j1, k);
+            px = readlane_t(ox, k);
+            py = readlane_t(oy, k);
+            pz = readlane_t(oz, k);
+        }
+    };
+    __device__ Bins bins(const Params<T> &P, int s, int s_end) const {
+        Bins r{0, 0, T(0), T(0), T(0)};
+        const int t = s + lane_id();
+        if (t < s_end) bin(P, t, r.j0, r.j1, r.ox, r.oy, r.oz);
+        return r;
     }
 };
 

@@ -156,18 +156,22 @@ __global__ __launch_bounds__(BLOCK) void k_cn(Params<T> P) {
     acc.init();
     float cnt = 0.0f;
     const int s0 = C.S->st_off[CUT_CN], s1 = s0 + C.S->st_cnt[CUT_CN];
-    for (int s = s0; s < s1; ++s) {
-        int j0, j1;
-        T ox, oy, oz;
-        C.bin(P, s, j0, j1, ox, oy, oz);
-        for (int j = j0 + lane; j < j1; j += WAVE) {
-            const T dx = ox - P.lx[j], dy = oy - P.ly[j], dz = oz - P.lz[j];
-            const T r2 = dx * dx + dy * dy + dz * dz;
-            if (r2 <= P.rc_cn2 && r2 > T(0)) {
-                const T r = xsqrt(r2);
-                const T ex = xexp(-T(K1) * ((rci + P.rcov[P.type_s[j]]) / r - T(1)));
-                acc.add(T(1) / (T(1) + ex));
-                cnt += 1.0f;
+    for (int sb = s0; sb < s1; sb += WAVE) {
+        const auto Bn = C.bins(P, sb, s1);
+        const int nsb = s1 - sb < WAVE ? s1 - sb : WAVE;
+        for (int sk = 0; sk < nsb; ++sk) {
+            int j0, j1;
+            T ox, oy, oz;
+            Bn.get(sk, j0, j1, ox, oy, oz);
+            for (int j = j0 + lane; j < j1; j += WAVE) {
+                const T dx = ox - P.lx[j], dy = oy - P.ly[j], dz = oz - P.lz[j];
+                const T r2 = dx * dx + dy * dy + dz * dz;
+                if (r2 <= P.rc_cn2 && r2 > T(0)) {
+                    const T r = xsqrt(r2);
+                    const T ex = xexp(-T(K1) * ((rci + P.rcov[P.type_s[j]]) / r - T(1)));
+                    acc.add(T(1) / (T(1) + ex));
+                    cnt += 1.0f;
+                }
             }
         }
     }
@@ -286,7 +290,7 @@ __device__ __forceinline__ double prsqrt(double x) { return rsqrt(x); }
 // together.
 template <class T, int NR>
 __device__ __forceinline__ void pair_2b(const Params<T> &P, int j, T dx, T dy, T dz, const T *y, const T *dy_,
-                                        const T *rq, const T *rR, Acc2b<T> &A) {
+                                        const T *rq, const T *rR, const T *rX, Acc2b<T> &A) {
     const int tj = P.type_s[j];
     T wj[NR];
 #pragma unroll
@@ -297,45 +301,45 @@ __device__ __forceinline__ void pair_2b(const Params<T> &P, int j, T dx, T dy, T
         c6 += y[tj * NR + b] * wj[b];
         dc6 += dy_[tj * NR + b] * wj[b];
     }
-    const T q = rq[tj], R = rR[tj];
-    const T c8 = T(3) * c6 * q, dc8 = T(3) * dc6 * q;
     const T r2 = dx * dx + dy * dy + dz * dz;
-    const T ir = prsqrt(r2);
-    const T ir2 = ir * ir;
-    const T ir6 = ir2 * ir2 * ir2, ir8 = ir6 * ir2;
-    T f6, f8, df6, df8;
+    T e, g, phi;  // E+_i share (half the pair), dE+/dCN_i, (1/r) dE+/dr
     if (P.damping == 0) {
+        const T q = rq[tj], R = rR[tj];
+        const T c8 = T(3) * c6 * q, dc8 = T(3) * dc6 * q;
+        const T ir = prsqrt(r2);
+        const T ir2 = ir * ir;
+        const T ir6 = ir2 * ir2 * ir2, ir8 = ir6 * ir2;
         const T t = R * ir;
         const T b6 = P.p1 * t, b8 = P.p2 * t;
         const T b6_2 = b6 * b6, b6_4 = b6_2 * b6_2, b6_8 = b6_4 * b6_4, b6_14 = b6_8 * b6_4 * b6_2;
         const T b8_2 = b8 * b8, b8_4 = b8_2 * b8_2, b8_8 = b8_4 * b8_4, b8_16 = b8_8 * b8_8;
-        f6 = prcp(T(1) + T(6) * b6_14);
-        f8 = prcp(T(1) + T(6) * b8_16);
-        df6 = T(84) * f6 * f6 * b6_14 * ir;
-        df8 = T(96) * f8 * f8 * b8_16 * ir;
+        const T f6 = finite_or_zero(prcp(T(1) + T(6) * b6_14)), f8 = finite_or_zero(prcp(T(1) + T(6) * b8_16));
+        const T df6 = finite_or_zero(T(84) * f6 * f6 * b6_14 * ir), df8 = finite_or_zero(T(96) * f8 * f8 * b8_16 * ir);
+        e = T(0.5) * (P.s6 * c6 * f6 * ir6 + P.s8 * c8 * f8 * ir8);
+        g = P.s6 * f6 * dc6 * ir6 + P.s8 * f8 * dc8 * ir8;
+        phi = (P.s6 * c6 * (df6 - T(6) * f6 * ir) * ir6 + P.s8 * c8 * (df8 - T(8) * f8 * ir) * ir8) * ir;
     } else {
-        const T a2 = R * R, a6 = a2 * a2 * a2, a8 = a6 * a2;
-        const T r6 = r2 * r2 * r2, r8 = r6 * r2;
-        const T d6 = prcp(r6 + a6), d8 = prcp(r8 + a8);
-        f6 = r6 * d6;
-        f8 = r8 * d8;
-        df6 = T(6) * a6 * r6 * ir * d6 * d6;
-        df8 = T(8) * a8 * r8 * ir * d8 * d8;
+        // BJ: f_n r^-n = 1 / (r^n + a^n), so with rq = 3 s8 r2r4_A r2r4_B, rR = a^6, rX = a^8:
+        //   t6 = s6 / (r^6 + a^6), t8 = 3 s8 q / (r^8 + a^8)
+        //   E+ = C6 (t6 + t8),  dE+/dCN = dC6 (t6 + t8),
+        //   (1/r) dE+/dr = -C6 (6 t6 r^4 / (r^6 + a^6) + 8 t8 r^6 / (r^8 + a^8))
+        // (no 1/r anywhere; a denominator that underflows to 0 contributes nothing)
+        const T r4 = r2 * r2, r6 = r4 * r2, r8 = r4 * r4;
+        const T d6 = finite_or_zero(prcp(r6 + rR[tj])), d8 = finite_or_zero(prcp(r8 + rX[tj]));
+        const T t6 = P.s6 * d6, t8 = rq[tj] * d8;
+        const T tt = t6 + t8;
+        e = T(0.5) * c6 * tt;
+        g = dc6 * tt;
+        phi = -c6 * (T(6) * t6 * d6 * r4 + T(8) * t8 * d8 * r6);
     }
-    f6 = finite_or_zero(f6);
-    f8 = finite_or_zero(f8);
-    df6 = finite_or_zero(df6);
-    df8 = finite_or_zero(df8);
-    const T e6 = P.s6 * c6 * f6 * ir6, e8 = P.s8 * c8 * f8 * ir8;
-    const T phi = (P.s6 * c6 * (df6 - T(6) * f6 * ir) * ir6 + P.s8 * c8 * (df8 - T(8) * f8 * ir) * ir8) * ir;
     const T h = T(-0.5) * phi;
     const T vv[6] = {h * dx * dx, h * dy * dy, h * dz * dz, h * dx * dy, h * dx * dz, h * dy * dz};
-    A.add(T(0.5) * (e6 + e8), P.s6 * f6 * dc6 * ir6 + P.s8 * f8 * dc8 * ir8, phi * dx, phi * dy, phi * dz, vv);
+    A.add(e, g, phi * dx, phi * dy, phi * dz, vv);
 }
 
 template <class T>
 __host__ __device__ constexpr size_t twobody_wave_lds(int ne, int nref) {
-    return ((sizeof(Queue<T>) + (size_t)(2 * ne * nref + 2 * ne) * sizeof(T)) + 15) / 16 * 16;
+    return ((sizeof(Queue<T>) + (size_t)(2 * ne * nref + 3 * ne) * sizeof(T)) + 15) / 16 * 16;
 }
 
 template <class T, int NR>
@@ -348,7 +352,7 @@ __global__ __launch_bounds__(BLOCK) void k_2b(Params<T> P) {
     unsigned char *base = smem + wv * twobody_wave_lds<T>(P.ne, NR);
     Queue<T> &Q = *reinterpret_cast<Queue<T> *>(base);
     T *y = reinterpret_cast<T *>(base + sizeof(Queue<T>));
-    T *dyv = y + ynr, *rq = dyv + ynr, *rR = rq + P.ne;
+    T *dyv = y + ynr, *rq = dyv + ynr, *rR = rq + P.ne, *rX = rR + P.ne;
 
     Centre<T> C;
     C.load(P, i);
@@ -369,8 +373,16 @@ __global__ __launch_bounds__(BLOCK) void k_2b(Params<T> P) {
         dyv[idx] = sdy;
     }
     for (int B = lane; B < P.ne; B += WAVE) {
-        rq[B] = P.q4[C.ti * P.ne + B];
-        rR[B] = P.rdamp[C.ti * P.ne + B];
+        const T q = P.q4[C.ti * P.ne + B], R = P.rdamp[C.ti * P.ne + B];
+        if (P.damping == 0) {  // zero damping: q and R0
+            rq[B] = q;
+            rR[B] = R;
+        } else {  // BJ: 3 s8 q, a^6, a^8
+            const T a2 = R * R, a6 = a2 * a2 * a2;
+            rq[B] = T(3) * P.s8 * q;
+            rR[B] = a6;
+            rX[B] = a6 * a2;
+        }
     }
     wave_sync();
 
@@ -379,59 +391,63 @@ __global__ __launch_bounds__(BLOCK) void k_2b(Params<T> P) {
     int qn = 0;
     float cnt = 0.0f;
     const int s0 = C.S->st_off[CUT_2B], s1 = s0 + C.S->st_cnt[CUT_2B];
-    for (int s = s0; s < s1; ++s) {
-        int j0, j1;
-        T ox, oy, oz;
-        C.bin(P, s, j0, j1, ox, oy, oz);
-        for (int jb = j0; jb < j1; jb += WAVE) {
-            const int j = jb + lane;
-            bool in = false;
-            T dx = T(0), dy = T(0), dz = T(0);
-            if (j < j1) {
-                dx = ox - P.lx[j];
-                dy = oy - P.ly[j];
-                dz = oz - P.lz[j];
-                const T r2 = dx * dx + dy * dy + dz * dz;
-                in = r2 < P.rc_22 && r2 > T(0);
-            }
-            const uint64_t m = ballot64(in);
-            if (m == 0) continue;
-            if (in) {
-                const int pos = qn + mbcnt64(m);
-                Q.j[pos] = j;
-                Q.dx[pos] = dx;
-                Q.dy[pos] = dy;
-                Q.dz[pos] = dz;
-            }
-            qn += __popcll(m);
-            if (qn >= WAVE) {
-                wave_sync();
-                pair_2b<T, NR>(P, Q.j[lane], Q.dx[lane], Q.dy[lane], Q.dz[lane], y, dyv, rq, rR, A);
-                cnt += 1.0f;
-                const int rest = qn - WAVE;
-                int mj = 0;
-                T mx = T(0), my = T(0), mz = T(0);
-                if (lane < rest) {
-                    mj = Q.j[WAVE + lane];
-                    mx = Q.dx[WAVE + lane];
-                    my = Q.dy[WAVE + lane];
-                    mz = Q.dz[WAVE + lane];
+    for (int sb = s0; sb < s1; sb += WAVE) {
+        const auto Bn = C.bins(P, sb, s1);
+        const int nsb = s1 - sb < WAVE ? s1 - sb : WAVE;
+        for (int sk = 0; sk < nsb; ++sk) {
+            int j0, j1;
+            T ox, oy, oz;
+            Bn.get(sk, j0, j1, ox, oy, oz);
+            for (int jb = j0; jb < j1; jb += WAVE) {
+                const int j = jb + lane;
+                bool in = false;
+                T dx = T(0), dy = T(0), dz = T(0);
+                if (j < j1) {
+                    dx = ox - P.lx[j];
+                    dy = oy - P.ly[j];
+                    dz = oz - P.lz[j];
+                    const T r2 = dx * dx + dy * dy + dz * dz;
+                    in = r2 < P.rc_22 && r2 > T(0);
                 }
-                wave_sync();
-                if (lane < rest) {
-                    Q.j[lane] = mj;
-                    Q.dx[lane] = mx;
-                    Q.dy[lane] = my;
-                    Q.dz[lane] = mz;
+                const uint64_t m = ballot64(in);
+                if (m == 0) continue;
+                if (in) {
+                    const int pos = qn + mbcnt64(m);
+                    Q.j[pos] = j;
+                    Q.dx[pos] = dx;
+                    Q.dy[pos] = dy;
+                    Q.dz[pos] = dz;
                 }
-                wave_sync();
-                qn = rest;
+                qn += __popcll(m);
+                if (qn >= WAVE) {
+                    wave_sync();
+                    pair_2b<T, NR>(P, Q.j[lane], Q.dx[lane], Q.dy[lane], Q.dz[lane], y, dyv, rq, rR, rX, A);
+                    cnt += 1.0f;
+                    const int rest = qn - WAVE;
+                    int mj = 0;
+                    T mx = T(0), my = T(0), mz = T(0);
+                    if (lane < rest) {
+                        mj = Q.j[WAVE + lane];
+                        mx = Q.dx[WAVE + lane];
+                        my = Q.dy[WAVE + lane];
+                        mz = Q.dz[WAVE + lane];
+                    }
+                    wave_sync();
+                    if (lane < rest) {
+                        Q.j[lane] = mj;
+                        Q.dx[lane] = mx;
+                        Q.dy[lane] = my;
+                        Q.dz[lane] = mz;
+                    }
+                    wave_sync();
+                    qn = rest;
+                }
             }
         }
     }
     wave_sync();
     if (lane < qn) {
-        pair_2b<T, NR>(P, Q.j[lane], Q.dx[lane], Q.dy[lane], Q.dz[lane], y, dyv, rq, rR, A);
+        pair_2b<T, NR>(P, Q.j[lane], Q.dx[lane], Q.dy[lane], Q.dz[lane], y, dyv, rq, rR, rX, A);
         cnt += 1.0f;
     }
     A.fold();
@@ -468,30 +484,34 @@ __global__ __launch_bounds__(BLOCK) void k_chain(Params<T> P) {
     for (int k = 0; k < 3; ++k) F[k].init();
     for (int k = 0; k < 6; ++k) V[k].init();
     const int s0 = C.S->st_off[CUT_CN], s1 = s0 + C.S->st_cnt[CUT_CN];
-    for (int s = s0; s < s1; ++s) {
-        int j0, j1;
-        T ox, oy, oz;
-        C.bin(P, s, j0, j1, ox, oy, oz);
-        for (int j = j0 + lane; j < j1; j += WAVE) {
-            const T dx = ox - P.lx[j], dy = oy - P.ly[j], dz = oz - P.lz[j];
-            const T r2 = dx * dx + dy * dy + dz * dz;
-            if (r2 <= P.rc_cn2 && r2 > T(0)) {
-                const T ir = xrsqrt(r2);
-                const T rr = rci + P.rcov[P.type_s[j]];
-                const T ex = xexp(-T(K1) * (rr * ir - T(1)));
-                const T ope = T(1) + ex;
-                const T dcn = -T(K1) * ex * rr * ir * ir * ir / (ope * ope);
-                const T sc = finite_or_zero((gi + (T)P.g[j]) * dcn);
-                F[0].add(sc * dx);
-                F[1].add(sc * dy);
-                F[2].add(sc * dz);
-                const T h = T(-0.5) * sc;
-                V[0].add(h * dx * dx);
-                V[1].add(h * dy * dy);
-                V[2].add(h * dz * dz);
-                V[3].add(h * dx * dy);
-                V[4].add(h * dx * dz);
-                V[5].add(h * dy * dz);
+    for (int sb = s0; sb < s1; sb += WAVE) {
+        const auto Bn = C.bins(P, sb, s1);
+        const int nsb = s1 - sb < WAVE ? s1 - sb : WAVE;
+        for (int sk = 0; sk < nsb; ++sk) {
+            int j0, j1;
+            T ox, oy, oz;
+            Bn.get(sk, j0, j1, ox, oy, oz);
+            for (int j = j0 + lane; j < j1; j += WAVE) {
+                const T dx = ox - P.lx[j], dy = oy - P.ly[j], dz = oz - P.lz[j];
+                const T r2 = dx * dx + dy * dy + dz * dz;
+                if (r2 <= P.rc_cn2 && r2 > T(0)) {
+                    const T ir = xrsqrt(r2);
+                    const T rr = rci + P.rcov[P.type_s[j]];
+                    const T ex = xexp(-T(K1) * (rr * ir - T(1)));
+                    const T ope = T(1) + ex;
+                    const T dcn = -T(K1) * ex * rr * ir * ir * ir / (ope * ope);
+                    const T sc = finite_or_zero((gi + (T)P.g[j]) * dcn);
+                    F[0].add(sc * dx);
+                    F[1].add(sc * dy);
+                    F[2].add(sc * dz);
+                    const T h = T(-0.5) * sc;
+                    V[0].add(h * dx * dx);
+                    V[1].add(h * dy * dy);
+                    V[2].add(h * dz * dz);
+                    V[3].add(h * dx * dy);
+                    V[4].add(h * dx * dz);
+                    V[5].add(h * dy * dz);
+                }
             }
         }
     }

@@ -41,30 +41,34 @@ template <class T, class Push, class Flush>
 __device__ void scan_neighbours(const Params<T> &P, const Centre<T> &C, int lane, int &qn, Push push, Flush flush) {
     const T rc2 = T(RCF * RCF);
     const int s0 = C.S->st_off[CUT_SUR], s1 = s0 + C.S->st_cnt[CUT_SUR];
-    for (int s = s0; s < s1; ++s) {
-        int j0, j1;
-        T ox, oy, oz;
-        C.bin(P, s, j0, j1, ox, oy, oz);
-        for (int jb = j0; jb < j1; jb += WAVE) {
-            const int j = jb + lane;
-            bool in = false;
-            T dx = T(0), dy = T(0), dz = T(0), r2 = T(0);
-            if (j < j1) {
-                dx = ox - P.lx[j];
-                dy = oy - P.ly[j];
-                dz = oz - P.lz[j];
-                r2 = dx * dx + dy * dy + dz * dz;
-                in = r2 <= rc2 && r2 > T(1e-12);
-            }
-            const uint64_t m = ballot64(in);
-            if (m == 0) continue;
-            if (in) push(qn + mbcnt64(m), j, (float)dx, (float)dy, (float)dz, (float)xsqrt(r2));
-            qn += __popcll(m);
-            if (qn > SQ - WAVE) {
-                wave_sync();
-                flush(qn);
-                wave_sync();
-                qn = 0;
+    for (int sb = s0; sb < s1; sb += WAVE) {
+        const auto Bn = C.bins(P, sb, s1);
+        const int nsb = s1 - sb < WAVE ? s1 - sb : WAVE;
+        for (int sk = 0; sk < nsb; ++sk) {
+            int j0, j1;
+            T ox, oy, oz;
+            Bn.get(sk, j0, j1, ox, oy, oz);
+            for (int jb = j0; jb < j1; jb += WAVE) {
+                const int j = jb + lane;
+                bool in = false;
+                T dx = T(0), dy = T(0), dz = T(0), r2 = T(0);
+                if (j < j1) {
+                    dx = ox - P.lx[j];
+                    dy = oy - P.ly[j];
+                    dz = oz - P.lz[j];
+                    r2 = dx * dx + dy * dy + dz * dz;
+                    in = r2 <= rc2 && r2 > T(1e-12);
+                }
+                const uint64_t m = ballot64(in);
+                if (m == 0) continue;
+                if (in) push(qn + mbcnt64(m), j, (float)dx, (float)dy, (float)dz, (float)xsqrt(r2));
+                qn += __popcll(m);
+                if (qn > SQ - WAVE) {
+                    wave_sync();
+                    flush(qn);
+                    wave_sync();
+                    qn = 0;
+                }
             }
         }
     }
