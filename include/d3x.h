@@ -90,6 +90,11 @@ int d3x_set_graph(d3_handle_t *handle, int enable);
 int d3x_set_tuning(d3_handle_t *handle, double bin_width_bohr, int atm_blocks);
 int d3x_get_tuning(d3_handle_t *handle, double *bin_width_bohr, int *atm_blocks);
 int d3x_get_atm_variant(d3_handle_t *handle);
+
+/* Forward-neighbour capacity per centre of the three-body scratch (0 before the first
+ * evaluation).  A centre with more forward neighbours within the ATM cutoff makes the
+ * evaluation grow it (x4) and run again; the result is unaffected. */
+int d3x_get_atm_capacity(d3_handle_t *handle);
 double d3x_autotune(d3_handle_t *handle, const char *cache_path);
 
 /* The HIP stream (hipStream_t) the handle launches on; d3x_set_stream replaces it (NULL =

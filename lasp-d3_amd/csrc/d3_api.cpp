@@ -312,6 +312,11 @@ int d3x_get_tuning(d3_handle_t *handle, double *bin_width_bohr, int *atm_blocks)
     return 0;
 }
 
+int d3x_get_atm_capacity(d3_handle_t *handle) {
+    if (!handle) return -1;
+    return H(handle)->eng->atm_cap();
+}
+
 int d3x_get_atm_variant(d3_handle_t *handle) {
     if (!handle) return -1;
     int v = -1;

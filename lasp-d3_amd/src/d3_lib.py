@@ -67,6 +67,7 @@ SIGNATURES = {
     "d3x_set_tuning": (C.c_int, [_vp, C.c_double, C.c_int]),
     "d3x_get_tuning": (C.c_int, [_vp, _dp, _ip]),
     "d3x_get_atm_variant": (C.c_int, [_vp]),
+    "d3x_get_atm_capacity": (C.c_int, [_vp]),
     "d3x_autotune": (C.c_double, [_vp, C.c_char_p]),
     "d3x_get_stream": (_vp, [_vp]),
     "d3x_set_stream": (C.c_int, [_vp, _vp]),
@@ -247,6 +248,10 @@ class Handle:
         if v < 0:
             raise D3Error("d3x_get_atm_variant", v)
         return v
+
+    def atm_capacity(self) -> int:
+        """Forward-neighbour capacity per centre of the three-body scratch."""
+        return int(self.lib.d3x_get_atm_capacity(self.h))
 
     def autotune(self, cache_path: str | None = None) -> float:
         t = self.lib.d3x_autotune(self.h, cache_path.encode() if cache_path else None)
