@@ -45,7 +45,10 @@ constexpr double K6 = 4096.0 / 729.0;           // (4/3)^6
 constexpr double L2K6_83 = 6.6405999884615;     // (8/3) log2((4/3)^6)
 constexpr float FAR = 1e18f;                    // coordinate of padding lanes (fails every cutoff)
 // record fields (global scratch, one array of cap entries per field)
-enum { F_DX, F_DY, F_DZ, F_R2, F_IR2, F_RA, F_SC6, F_LAM, F_DLI, F_DLJ };
+// (r^2, 1/r^2, 1/r, sqrt C6 and the damping term are recomputed where a tile is staged: they
+// cost a few instructions per slot and tile, against twice the record bytes read back from HBM)
+enum { F_DX, F_DY, F_DZ, F_DLI, F_DLJ };
+static_assert(F_DLJ + 1 == ATM_NF, "ATM_NF record fields per neighbour slot");
 // per-neighbour FP64 accumulators: A_p = sum (t_ip + t_pq), B_p = sum t_pq d_q, H_p = sum e/2,
 // Gc_p = sum (e/2) dlnC6_pq/dCN_p over the triangles {i, p, q} of the centre
 enum { A_A, A_BX, A_BY, A_BZ, A_H, A_G, A_NF };
@@ -217,7 +220,6 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                             LT[F_DX * cap + pos] = dx;
                             LT[F_DY * cap + pos] = dy;
                             LT[F_DZ * cap + pos] = dz;
-                            LT[F_R2 * cap + pos] = r2;
                         }
                     }
                     count += __popcll(m);
@@ -243,14 +245,7 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                 dcj += S.Cy[tj * NR + b] * dwj;
             }
             const T ic6 = c6 > T(0) ? T(1) / c6 : T(0);
-            const T a = LT[F_R2 * cap + p];
-            const T ia = T(1) / a;
-            const T r0 = P.r0[ti * ne + tj];
             Lt[p] = tj;
-            LT[F_IR2 * cap + p] = ia;
-            LT[F_RA * cap + p] = T(1) / xsqrt(a);
-            LT[F_SC6 * cap + p] = xsqrt(c6);
-            LT[F_LAM * cap + p] = F32 ? T((8.0f / 3.0f) * log2f((float)(r0 * r0 * ia))) : r0 * r0 * ia;
             LT[F_DLI * cap + p] = dci * ic6;
             LT[F_DLJ * cap + p] = dcj * ic6;
             // the accumulators take FP64 atomic adds in phase 3 (no returned value, so the wave
@@ -267,6 +262,19 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
         // they cost no read latency (a read-modify-write here took 7.6 % of the kernel on Cu).
         uint64_t cnt = 0;
         const int ntile = (nf + AW - 1) / AW;
+        // Pair (i, p) terms of a forward neighbour p from its offset, type and weights, the same
+        // expressions for the k and the j side: r^2, 1/r^2, 1/r, sqrt C6_ip, damping term
+        // (FP32: (8/3) log2(R0^2 / r^2), FP64: R0^2 / r^2).
+        auto pair_terms = [&](T dx, T dy, T dz, int tp, const T *wp, T &a, T &ia, T &ra, T &sc6, T &lam) {
+            a = dx * dx + dy * dy + dz * dz;
+            ia = T(1) / a;
+            ra = T(1) / xsqrt(a);
+            T c6 = T(0);
+            for (int b = 0; b < NR; ++b) c6 += S.Cy[tp * NR + b] * wp[b];
+            sc6 = xsqrt(c6);
+            const T r0 = P.r0[ti * ne + tp];
+            lam = F32 ? T((8.0f / 3.0f) * log2f((float)(r0 * r0 * ia))) : r0 * r0 * ia;
+        };
         for (int kb = 0; kb < ntile; ++kb) {
             const int k = kb * AW + lane;
             const bool kval = k < nf;
@@ -280,16 +288,15 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                 kx = LT[F_DX * cap + k];
                 ky = LT[F_DY * cap + k];
                 kz = LT[F_DZ * cap + k];
-                kb2 = LT[F_R2 * cap + k];
-                kib = LT[F_IR2 * cap + k];
-                krb = LT[F_RA * cap + k];
-                ksc6 = T(-0.5) * LT[F_SC6 * cap + k];  // -sqrt(C6_ik) / 2: C9 / 2 below
-                klam = F32 ? LT[F_LAM * cap + k] + T(L2K6_83) : T(K6) * LT[F_LAM * cap + k];
                 tk = Lt[k];
                 for (int a = 0; a < NR; ++a) {
                     kw[a] = P.w[a * P.n + jk];
                     kdw[a] = P.dw[a * P.n + jk];
                 }
+                T sc6, lam;
+                pair_terms(kx, ky, kz, tk, kw, kb2, kib, krb, sc6, lam);
+                ksc6 = T(-0.5) * sc6;  // -sqrt(C6_ik) / 2: C9 / 2 below
+                klam = F32 ? lam + T(L2K6_83) : T(K6) * lam;
             }
             // bounding box of the k wave (valid lanes), wave-uniform
             const float kminx = uni(wmin(kval ? (float)kx : 3e38f)), kmaxx = uni(wmax(kval ? (float)kx : -3e38f));
@@ -327,17 +334,13 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                     T wj[NR], dwj[NR];
                     for (int q = 0; q < NR; ++q) wj[q] = dwj[q] = T(0);
                     if (jval) {
-                        a = LT[F_R2 * cap + pj];
-                        ia = LT[F_IR2 * cap + pj];
-                        ra = LT[F_RA * cap + pj];
-                        sc6 = LT[F_SC6 * cap + pj];
-                        lam = LT[F_LAM * cap + pj];
                         const int jj = Lj[pj];
                         tj = Lt[pj];
                         for (int q = 0; q < NR; ++q) {
                             wj[q] = P.w[q * P.n + jj];
                             dwj[q] = P.dw[q * P.n + jj];
                         }
+                        pair_terms(jxs, jys, jzs, tj, wj, a, ia, ra, sc6, lam);
                     }
                     S.J0[lane] = T2<T>{jxs, jys};
                     S.J1[lane] = T2<T>{jzs, a};

@@ -103,7 +103,7 @@ struct Params {
     unsigned *status;
 };
 
-constexpr int ATM_NF = 10;  // record fields per ATM neighbour
+constexpr int ATM_NF = 5;  // record fields per ATM neighbour (offset, CN log-derivatives)
 // doubles of ATM accumulator scratch per block: 6 per neighbour slot
 __host__ __device__ constexpr size_t atm_acc_stride(int cap) { return 6 * (size_t)cap; }
 
