@@ -255,7 +255,7 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
         }
         __syncthreads();
 
-        // ---- phase 3: pairs {j, k}, j < k, of forward neighbours.  Per-pair terms are summed in T
+        // ---- phase 3: unordered pairs {j, k} of forward neighbours.  Per-pair terms are summed in T
         // over one 64 x 64 tile (at most 64 terms per sum); tile sums go to the FP64 per-neighbour
         // accumulators La, which lane l alone touches for its slots, as atomic adds: one lane's
         // adds to one slot reach L2 in issue order, so the sums stay bitwise repeatable, and
@@ -368,12 +368,16 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
 
                 // tile-local sums of neighbour k (folded into FP64 per tile)
                 T kA = T(0), kBx = T(0), kBy = T(0), kBz = T(0), kH = T(0), kG = T(0);
-                // The diagonal tile (jt == kb) keeps only j < k; the other tiles need no order
-                // test, so the step loop is instantiated for both.
+                // On the diagonal tile (jt == kb) every unordered pair {j, k} comes up at two
+                // steps, q and 64 - q; the terms are symmetric in j and k, so steps 1..31 and the
+                // lower half of the lanes at step 32 take each pair once (32 steps, not 64).  The
+                // other tiles take every step and need no order test; the step loop is
+                // instantiated for both.
                 auto steps = [&](auto diag) {
                 constexpr bool DIAG = decltype(diag)::value;
+                constexpr int Q0 = DIAG ? 1 : 0, Q1 = DIAG ? AW / 2 + 1 : AW;
 #pragma unroll 1
-                for (int q = 0; q < AW; ++q) {
+                for (int q = Q0; q < Q1; ++q) {
                     // FP32: lane l updates the j slot that lane l+1 updated one step earlier
                     if constexpr (F32) wave_sync();
                     const int jl = (lane + q) & (AW - 1);
@@ -387,9 +391,10 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                     // from a VGPR copy of it
                     uint64_t m = __builtin_amdgcn_ballot_w64(okc) & __builtin_amdgcn_ballot_w64(okz);
                     if constexpr (DIAG) {
-                        const bool okj = jl < lane;
-                        m &= __builtin_amdgcn_ballot_w64(okj);
-                        ok = ok && okj;
+                        if (q == AW / 2) {  // lanes l and l + 32 share the pair
+                            m &= 0xffffffffull;
+                            ok = ok && lane < AW / 2;
+                        }
                     }
                     if (m == 0) continue;  // uniform
                     cnt += __popcll(m);
