@@ -4,7 +4,9 @@
 * the 12,288-atom water box: translation and permutation invariance, sum of forces, stress
   symmetry, and the FP32 path against the FP64 path (<= 1e-6 relative energy);
 * the 256-cell triclinic batch: per-cell results of the batched evaluation equal single-cell runs
-  for a sample of cells, sum of forces per cell, FP32 against FP64 per cell.
+  for a sample of cells, sum of forces per cell, FP32 against FP64 per cell;
+* the 1,000,188-atom fcc Cu cell without jitter: per-atom energy, CN, dE/dCN and stress equal the
+  4-atom cell's (oracle), forces vanish.
 """
 
 import numpy as np
@@ -102,3 +104,33 @@ def test_batch256_equals_single_cells():
     rel = np.max(np.abs(e32 - eb) / np.abs(eb))
     print(f"  batch fp32: max per-cell rel dE {rel:.2e}")
     assert rel <= 1e-6
+
+
+def test_perfect_fcc_cu_million_atoms(table):
+    """The 1,000,188-atom fcc Cu cell of the headline benchmark without its jitter: every atom is
+    equivalent, so the FP64 engine's per-atom energy, CN and dE/dCN must be those of the 4-atom
+    conventional cell (FP64 oracle, same cutoffs), the forces vanish and the stress (intensive)
+    equals the small cell's -- a size-independent check of the full-size path, three-body term
+    included (8.9e11 triangles).  The FP32 path stays within 1e-6 of the FP64 energy."""
+    pos, Z, cell = S.fcc_cu(63, jitter=0.0)
+    spos, sZ, scell = S.fcc_cu(1, jitter=0.0)
+    print("case perfect fcc Cu: 4-atom oracle", flush=True)
+    o = d3_oracle.compute(spos, sZ, scell, table, **RC)
+    e_atom = o["energy"] / len(sZ)
+    n = len(Z)
+    print(f"case perfect fcc Cu: {n} atoms fp64", flush=True)
+    h = handle(Z, n)
+    h.set_atoms(pos, Z)
+    h.set_cell(cell)
+    e, f, st = h.compute()
+    cn, g, ea = h.atom_data()
+    h.close()
+    assert abs(e / n - e_atom) <= 1e-12 * abs(e_atom), (e / n, e_atom)
+    assert np.max(np.abs(ea - e_atom)) <= 1e-11 * abs(e_atom)
+    np.testing.assert_allclose(cn, o["cn"][0], rtol=1e-12)
+    np.testing.assert_allclose(g, o["dedcn"][0], rtol=1e-11, atol=1e-20)  # 0 here: one reference dominates
+    assert np.max(np.abs(f)) <= 1e-9
+    assert np.max(np.abs(st - o["stress"])) <= 1e-12 * np.max(np.abs(o["stress"]))
+    print(f"case perfect fcc Cu: {n} atoms fp32", flush=True)
+    e32 = run(pos, Z, cell, d3_lib.D3X_FP32)[0]
+    assert abs(e32 - e) <= 1e-6 * abs(e)
