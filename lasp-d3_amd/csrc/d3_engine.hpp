@@ -79,7 +79,7 @@ struct Params {
     // r0[ne][ne] (ATM damping radii)
     const T *c6, *cnref, *rcov, *q4, *rdamp, *r0;
     // ATM: per-block scratch for one centre's forward neighbours (capacity atm_cap):
-    //   atm_i32: j, type (2 x cap); atm_T: record fields (ATM_NF x cap);
+    //   atm_i32: j, type (2 x cap); atm_T: record fields (ATM_NF x cap: offset, CN log-derivatives);
     //   atm_acc: per-neighbour accumulators (6 x cap, double)
     // and per-atom results (sorted order): fa[3N], ga[N], eas[N] filled with atomics (every
     // triangle's gradient, dE/dCN and e/3 for each vertex), eat[N] = sum of e over the
