@@ -49,12 +49,18 @@ int d3x_set_atm_mode(d3_handle_t *handle, int mode, int judge_every, double tol_
 int d3x_sur_state(d3_handle_t *handle, int *mode, int *fallbacks, double *last_dev_ev_per_atom);
 /* Features of the last evaluation (SURROGATE or EXACT_FEATURES), caller order, atoms x 64. */
 int d3x_sur_features(d3_handle_t *handle, float *features);
+/* Push-forward of a direction field through the features, for force matching: U[i] =
+ * sum_j (dx_i / dr_j) . dir[j] (caller order; dir atoms x 3 per Angstrom of position step, U atoms
+ * x 64), at the geometry of the last evaluation (SURROGATE or EXACT_FEATURES; a later set_atoms or
+ * set_cell invalidates it).  With dir = F_net - F_ref (eV/A), U is the input of
+ * d3x_sur_train_force_grad. */
+int d3x_sur_tangent(d3_handle_t *handle, const double *dir, float *U);
 
 /* Network kernels on device buffers (pointers from hipMalloc or a framework's allocator, stream
  * = hipStream_t or NULL for the default stream).  They enqueue and return; errors of the launch
  * are returned as D3X status bits. */
 size_t d3x_sur_packed_bytes(void);
-/* bytes of scratch for d3x_sur_train_grad on `rows` rows */
+/* bytes of scratch for d3x_sur_train_grad or d3x_sur_train_force_grad on `rows` rows */
 size_t d3x_sur_train_bytes(uint64_t rows);
 /* bf16 MFMA fragments of every weight matrix from the fp32 parameters */
 int d3x_sur_pack(const float *params, void *packed, void *stream);
@@ -66,6 +72,13 @@ int d3x_sur_infer(const float *params, const void *packed, uint64_t n, const flo
  * grad (D3X_SUR_NPARAMS floats, accumulated) += dloss/dparams */
 int d3x_sur_train_grad(const float *params, const void *packed, int cls, uint64_t n, const float *X, const float *y,
                        const float *w, float *E, void *work, float *grad, float *loss, void *stream);
+/* one force-matching gradient on n rows of class cls: D[n] (out) = dE/dx . U of each row (the
+ * network's derivative along U, eV per unit of U) and grad += d(sum_rows w D)/dparams.  Since
+ * F_net = -J^T dE/dx, the force loss L = sum |F_net - F_ref|^2 has dL/dparams = -2 sum_rows
+ * d(dE/dx . U)/dparams with U = J (F_net - F_ref) (d3x_sur_tangent): pass w = -2 lambda for
+ * lambda L.  D summed over a cell's rows is -sum F_net . (F_net - F_ref). */
+int d3x_sur_train_force_grad(const float *params, const void *packed, int cls, uint64_t n, const float *X,
+                             const float *U, const float *w, float *D, void *work, float *grad, void *stream);
 /* Adam step (bias-corrected, step >= 1) on the trainable entries; clears grad */
 int d3x_sur_adam(float *params, float *grad, float *m, float *v, float lr, float beta1, float beta2, float eps,
                  int step, void *stream);

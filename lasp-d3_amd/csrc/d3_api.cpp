@@ -375,6 +375,11 @@ int d3x_sur_features(d3_handle_t *handle, float *features) {
     return (int)guarded(H(handle), "d3x_sur_features", [&] { H(handle)->eng->fetch_features(features); });
 }
 
+int d3x_sur_tangent(d3_handle_t *handle, const double *dir, float *U) {
+    if (!handle || !dir || !U) return (int)(D3X_ERR_GENERIC | D3X_ERR_INPUT);
+    return (int)guarded(H(handle), "d3x_sur_tangent", [&] { H(handle)->eng->sur_tangent(dir, U); });
+}
+
 int d3x_set_graph(d3_handle_t *handle, int enable) {
     if (!handle) return (int)(D3X_ERR_GENERIC | D3X_ERR_INPUT);
     return (int)guarded(H(handle), "d3x_set_graph", [&] { H(handle)->eng->set_graph(enable != 0); });

@@ -138,6 +138,9 @@ template <class T>
 void launch_sur_features(const Params<T> &P, hipStream_t s);
 template <class T>
 void launch_surrogate(const Params<T> &P, hipStream_t s);
+// u = J dir through the features (dir: per atom in sorted order, 3 floats; U: 64 floats per atom)
+template <class T>
+void launch_sur_tangent(const Params<T> &P, const float *dir, float *U, hipStream_t s);
 
 // Reduction of per-atom energy / virial / pair counts into per-cell totals (deterministic),
 // and scatter of per-atom results to caller order with unit conversion.

@@ -224,7 +224,7 @@ Engine::~Engine() {
                     &d_npair, &d_out_f, &d_out_e, &d_out_cn, &d_out_g, &d_bin_count, &d_bin_start, &d_cursor,
                     &d_sys, &d_st_d, &d_st_D, &d_chunk_b, &d_chunk_e, &d_chunk_s, &d_sys_chunk, &d_partial,
                     &d_out_sys, &d_tab, &d_status, &d_atm_i32, &d_atm_T, &d_atm_acc, &d_atm_pa, &d_type_cls,
-                    &d_feat, &d_sur_G, &d_sur_E, &d_sur_cls, &d_sur_P, &d_sur_PK};
+                    &d_feat, &d_sur_G, &d_sur_E, &d_sur_cls, &d_sur_P, &d_sur_PK, &d_tan_dir, &d_tan_U};
     for (void **p : all) dfree(*p);
     for (auto &e : ev_)
         if (e) (void)hipEventDestroy(e);
@@ -306,6 +306,7 @@ void Engine::set_atoms(const double *coords_A, const uint16_t *Z, uint64_t n) {
     n_ = n;
     atoms_set_ = true;
     geom_dirty_ = true;
+    feat_valid_ = false;
 }
 
 void Engine::set_cell(const double cell_A[9]) {
@@ -320,6 +321,7 @@ void Engine::set_cell(const double cell_A[9]) {
     sys_[0].set = true;
     sys_[0].n = n_;
     geom_dirty_ = true;
+    feat_valid_ = false;
 }
 
 void Engine::set_batch(uint64_t nsys, const uint64_t *counts, const double *coords_A, const uint16_t *Z,
@@ -364,6 +366,7 @@ void Engine::set_batch(uint64_t nsys, const uint64_t *counts, const double *coor
     n_ = tot;
     atoms_set_ = true;
     geom_dirty_ = true;
+    feat_valid_ = false;
 }
 
 void Engine::clear() {
@@ -941,6 +944,36 @@ void Engine::fetch_features(float *X) {
     HIPCHK(hipStreamSynchronize(stream_));
     for (uint64_t i = 0; i < n_; ++i)
         std::memcpy(X + (size_t)orig[i] * sur::NFEAT, f.data() + i * sur::NFEAT, sur::NFEAT * sizeof(float));
+}
+
+void Engine::sur_tangent(const double *dir, float *U) {
+    if (!feat_valid_ || !d_feat) fail(D3X_ERR_STATE, "no features: evaluate in a mode that computes them first");
+    if (n_ == 0) return;
+    std::vector<int> orig(n_);
+    HIPCHK(hipMemcpyAsync(orig.data(), d_orig_s, n_ * sizeof(int), hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    std::vector<float> ds(3 * n_);
+    for (uint64_t i = 0; i < n_; ++i)
+        for (int k = 0; k < 3; ++k) {
+            const double v = dir[3 * (size_t)orig[i] + k];
+            if (!std::isfinite(v)) fail(D3X_ERR_INPUT, "non-finite direction");
+            ds[3 * i + k] = (float)v;
+        }
+    if (!d_tan_dir) {
+        d_tan_dir = dalloc(max_len_ * 3 * sizeof(float));
+        d_tan_U = dalloc(max_len_ * sur::NFEAT * sizeof(float));
+    }
+    HIPCHK(hipMemcpyAsync(d_tan_dir, ds.data(), ds.size() * sizeof(float), hipMemcpyHostToDevice, stream_));
+    if (precision_ == D3X_FP64)
+        launch_sur_tangent<double>(params<double>(), (const float *)d_tan_dir, (float *)d_tan_U, stream_);
+    else
+        launch_sur_tangent<float>(params<float>(), (const float *)d_tan_dir, (float *)d_tan_U, stream_);
+    HIPCHK(hipGetLastError());
+    std::vector<float> u(n_ * sur::NFEAT);
+    HIPCHK(hipMemcpyAsync(u.data(), d_tan_U, u.size() * sizeof(float), hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    for (uint64_t i = 0; i < n_; ++i)
+        std::memcpy(U + (size_t)orig[i] * sur::NFEAT, u.data() + i * sur::NFEAT, sur::NFEAT * sizeof(float));
 }
 
 void Engine::set_graph(bool on) {

@@ -100,6 +100,8 @@ class Engine {
     int fallbacks() const { return fallbacks_; }
     double last_deviation() const { return last_dev_; }
     void fetch_features(float *X);
+    // u = J dir of the last evaluation's geometry (caller order; dir per A, atoms x 3; U atoms x 64).
+    void sur_tangent(const double *dir, float *U);
 
    private:
     struct Cell {
@@ -127,7 +129,7 @@ class Engine {
     uint64_t sur_evals_ = 0;
     std::vector<float> h_sur_P_;
     void *d_type_cls = nullptr, *d_feat = nullptr, *d_sur_G = nullptr, *d_sur_E = nullptr, *d_sur_cls = nullptr,
-         *d_sur_P = nullptr, *d_sur_PK = nullptr;
+         *d_sur_P = nullptr, *d_sur_PK = nullptr, *d_tan_dir = nullptr, *d_tan_U = nullptr;
 
     // configuration
     std::vector<uint16_t> elems_;  // sorted unique atomic numbers of the handle

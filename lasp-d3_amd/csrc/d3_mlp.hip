@@ -1,7 +1,7 @@
 // Per-atom network of the learned three-body stand-in on MFMA (bf16 operands, fp32 accumulate):
-// weight packing, inference (energies and input gradients), and training (loss gradient,
-// per-row backward, weight gradients as split-K MFMA contractions, Adam).  Layout constants in
-// d3_sur.hpp.
+// weight packing, inference (energies and input gradients), and training (energy-loss gradient,
+// per-row backward, weight gradients as split-K MFMA contractions; force-loss gradient through a
+// tangent pass; Adam).  Layout constants in d3_sur.hpp.
 //
 // One wave owns a tile of 16 rows (atoms).  v_mfma_f32_16x16x16_bf16 operand layout (lane l):
 //   A (16 x 16, M x K): A[l % 16][4 (l / 16) + q],  B (K x N): B[4 (l / 16) + q][l % 16],
@@ -35,6 +35,10 @@ __device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z));
 __device__ __forceinline__ float dsilu(float z) {
     const float s = 1.0f / (1.0f + __expf(-z));
     return s * (1.0f + z * (1.0f - s));
+}
+__device__ __forceinline__ float d2silu(float z) {
+    const float s = 1.0f / (1.0f + __expf(-z));
+    return s * (1.0f - s) * (2.0f + z * (1.0f - 2.0f * s));
 }
 __device__ __forceinline__ v4f mfma(v4s a, v4s b, v4f c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
@@ -221,6 +225,161 @@ __device__ __forceinline__ void load_x(const float *__restrict__ P, const float 
     }
 }
 
+// u' = u istd fragments of the tile (A layout, as load_x).
+__device__ __forceinline__ void load_u(const float *__restrict__ P, const float *__restrict__ U, int n, int r0,
+                                       int lane, v4s U1[4]) {
+    const int row = r0 + (lane & 15), lg = lane >> 4;
+    for (int kk = 0; kk < 4; ++kk) {
+        const int k0 = 16 * kk + 4 * lg;
+        float v[4] = {0.f, 0.f, 0.f, 0.f};
+        if (row < n) {
+            const float4 u = *reinterpret_cast<const float4 *>(U + (size_t)row * NFEAT + k0);
+            v[0] = u.x * P[NFEAT + k0 + 0];
+            v[1] = u.y * P[NFEAT + k0 + 1];
+            v[2] = u.z * P[NFEAT + k0 + 2];
+            v[3] = u.w * P[NFEAT + k0 + 3];
+        }
+        U1[kk] = pack4(v[0], v[1], v[2], v[3]);
+    }
+}
+
+// Tangent pass of class c on the wave's 16-row tile: the network's derivative along u per row,
+// D = dE/dx . u = scale w3 . (silu'(z2) t2), t2 = W2^T (silu'(z1) t1), t1 = W1^T u', and the
+// reverse pass of sum_rows wt D:
+//   dt2 = g w3 silu'(z2),  dz2 = g w3 silu''(z2) t2  (g = scale wt of the row),
+//   dt1 = (W2 dt2) silu'(z1),  dz1 = (W2 dz2) silu'(z1) + (W2 dt2) silu''(z1) t1,
+//   dW2 = H1^T dZ2 + H1'^T dT2,  dW1 = X'^T dZ1 + U'^T dT1,  db2 = sum dz2,  db1 = sum dz1,
+//   dw3 = sum g silu'(z2) t2 (b3 does not enter D).
+// The D fragments of h1, h1' = silu'(z1) t1, dz2, dt2, dz1, dt1 go to fr[0..47] (8 each).
+__device__ __forceinline__ void tile_tangent(const float *__restrict__ pc, const v4s *__restrict__ pk,
+                                             const v4s A1[4], const v4s U1[4], unsigned short (*T)[TP], int lane,
+                                             const float wt[4], float Dq[4], v4s *fr, float *cs_b1, float *cs_b2,
+                                             float *cs_w3) {
+    const int lr = lane & 15, lg = lane >> 4;
+    const float scale = pc[O_TAIL + 1];
+    v4f z1[8], t1[8];
+#pragma unroll
+    for (int nb = 0; nb < 8; ++nb) {
+        v4f a = {0.f, 0.f, 0.f, 0.f}, t = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+            const v4s b = pk[PK_W1 + (kk * 8 + nb) * 64 + lane];
+            a = mfma(A1[kk], b, a);
+            t = mfma(U1[kk], b, t);
+        }
+        const float bb = pc[O_B1 + 16 * nb + lr];
+        for (int q = 0; q < 4; ++q) a[q] += bb;
+        z1[nb] = a;
+        t1[nb] = t;
+    }
+    v4s A[8], At[8];
+    wsync();
+#pragma unroll
+    for (int nb = 0; nb < 8; ++nb) {
+        float h[4];
+        for (int q = 0; q < 4; ++q) {
+            h[q] = silu(z1[nb][q]);
+            T[4 * lg + q][16 * nb + lr] = bf16(h[q]);
+        }
+        fr[nb * 64] = pack4(h[0], h[1], h[2], h[3]);
+    }
+    wsync();
+#pragma unroll
+    for (int kk = 0; kk < 8; ++kk) A[kk] = *reinterpret_cast<const v4s *>(&T[lr][16 * kk + 4 * lg]);
+    wsync();
+#pragma unroll
+    for (int nb = 0; nb < 8; ++nb) {
+        float h[4];
+        for (int q = 0; q < 4; ++q) {
+            h[q] = dsilu(z1[nb][q]) * t1[nb][q];
+            T[4 * lg + q][16 * nb + lr] = bf16(h[q]);
+        }
+        fr[(8 + nb) * 64] = pack4(h[0], h[1], h[2], h[3]);
+    }
+    wsync();
+#pragma unroll
+    for (int kk = 0; kk < 8; ++kk) At[kk] = *reinterpret_cast<const v4s *>(&T[lr][16 * kk + 4 * lg]);
+    // layer 2, primal and tangent
+    v4f z2[8], t2[8];
+#pragma unroll
+    for (int nb = 0; nb < 8; ++nb) {
+        v4f a = {0.f, 0.f, 0.f, 0.f}, t = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kk = 0; kk < 8; ++kk) {
+            const v4s b = pk[PK_W2 + (kk * 8 + nb) * 64 + lane];
+            a = mfma(A[kk], b, a);
+            t = mfma(At[kk], b, t);
+        }
+        const float bb = pc[O_B2 + 16 * nb + lr];
+        for (int q = 0; q < 4; ++q) a[q] += bb;
+        z2[nb] = a;
+        t2[nb] = t;
+    }
+    float g[4], s[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int q = 0; q < 4; ++q) g[q] = scale * wt[q];
+    wsync();
+    // D, dt2 (to LDS and fragments), dz2 (kept in z2 for the second product), bias / w3 sums
+#pragma unroll
+    for (int nb = 0; nb < 8; ++nb) {
+        const float w = pc[O_W3 + 16 * nb + lr];
+        float dt[4], dz[4], cw3 = 0.f;
+        for (int q = 0; q < 4; ++q) {
+            const float d1 = dsilu(z2[nb][q]), ht = d1 * t2[nb][q];
+            s[q] += w * ht;
+            cw3 += g[q] * ht;
+            dt[q] = g[q] * w * d1;
+            dz[q] = g[q] * w * d2silu(z2[nb][q]) * t2[nb][q];
+            T[4 * lg + q][16 * nb + lr] = bf16(dt[q]);
+        }
+        fr[(16 + nb) * 64] = pack4(dz[0], dz[1], dz[2], dz[3]);
+        fr[(24 + nb) * 64] = pack4(dt[0], dt[1], dt[2], dt[3]);
+        cs_b2[nb] = dz[0] + dz[1] + dz[2] + dz[3];
+        cs_w3[nb] = cw3;
+        z2[nb] = v4f{dz[0], dz[1], dz[2], dz[3]};
+    }
+    for (int q = 0; q < 4; ++q) {
+        float v = s[q];
+        v += __shfl_xor(v, 1, 64);
+        v += __shfl_xor(v, 2, 64);
+        v += __shfl_xor(v, 4, 64);
+        v += __shfl_xor(v, 8, 64);
+        Dq[q] = scale * v;
+    }
+    wsync();
+#pragma unroll
+    for (int kk = 0; kk < 8; ++kk) A[kk] = *reinterpret_cast<const v4s *>(&T[lr][16 * kk + 4 * lg]);
+    v4f dh1t[8];
+#pragma unroll
+    for (int nb = 0; nb < 8; ++nb) {
+        v4f a = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kk = 0; kk < 8; ++kk) a = mfma(A[kk], pk[PK_W2T + (kk * 8 + nb) * 64 + lane], a);
+        dh1t[nb] = a;
+    }
+    wsync();
+#pragma unroll
+    for (int nb = 0; nb < 8; ++nb)
+        for (int q = 0; q < 4; ++q) T[4 * lg + q][16 * nb + lr] = bf16(z2[nb][q]);
+    wsync();
+#pragma unroll
+    for (int kk = 0; kk < 8; ++kk) A[kk] = *reinterpret_cast<const v4s *>(&T[lr][16 * kk + 4 * lg]);
+#pragma unroll
+    for (int nb = 0; nb < 8; ++nb) {
+        v4f a = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kk = 0; kk < 8; ++kk) a = mfma(A[kk], pk[PK_W2T + (kk * 8 + nb) * 64 + lane], a);
+        float dz[4], dt[4];
+        for (int q = 0; q < 4; ++q) {
+            const float d1 = dsilu(z1[nb][q]);
+            dt[q] = dh1t[nb][q] * d1;
+            dz[q] = a[q] * d1 + dh1t[nb][q] * d2silu(z1[nb][q]) * t1[nb][q];
+        }
+        fr[(32 + nb) * 64] = pack4(dz[0], dz[1], dz[2], dz[3]);
+        fr[(40 + nb) * 64] = pack4(dt[0], dt[1], dt[2], dt[3]);
+        cs_b1[nb] = dz[0] + dz[1] + dz[2] + dz[3];
+    }
+}
+
 // ------------------------------------------------------------------------------- inference
 
 // E[row] (eV) and G[row][64] = dE/dx for rows 0..n-1, each row with its class cls[row].
@@ -361,6 +520,86 @@ __global__ __launch_bounds__(64) void k_wgrad(const float *__restrict__ P, int c
     }
 }
 
+// Force loss, rows 0..n-1 all of class c: D[row] = dE/dx . u (tangent pass) and the bias / w3
+// parts of d(sum wt D)/dparams; the 48 fragments of each tile go to `frag` for k_wgrad_tan.
+__global__ __launch_bounds__(256) void k_train_tan(const float *__restrict__ P, const v4s *__restrict__ PK, int c,
+                                                   int n, const float *__restrict__ X, const float *__restrict__ U,
+                                                   const float *__restrict__ Wt, float *__restrict__ D,
+                                                   v4s *__restrict__ frag, float *__restrict__ grad) {
+    __shared__ __align__(16) unsigned short Ts[4][16][TP];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int tile = blockIdx.x * 4 + w, r0 = tile * 16;
+    if (r0 >= n) return;  // whole wave; no block barrier follows
+    const int lr = lane & 15, lg = lane >> 4;
+    v4s A1[4], U1[4];
+    load_x(P, X, n, r0, lane, A1);
+    load_u(P, U, n, r0, lane, U1);
+    float wt[4], Dq[4];
+    for (int q = 0; q < 4; ++q) {  // rows past n get weight 0: every gradient term of theirs is 0
+        const int row = r0 + 4 * lg + q;
+        wt[q] = row < n ? Wt[row] : 0.f;
+    }
+    float cs_b1[8], cs_b2[8], cs_w3[8];
+    tile_tangent(P + HEAD + c * CLASS_SIZE, PK + c * PK_CLASS, A1, U1, Ts[w], lane, wt, Dq,
+                 frag + (size_t)tile * 48 * 64 + lane, cs_b1, cs_b2, cs_w3);
+    for (int q = 0; q < 4; ++q) {
+        const int row = r0 + 4 * lg + q;
+        if (row < n && lr == 0) D[row] = Dq[q];
+    }
+    float *g = grad + HEAD + c * CLASS_SIZE;
+    for (int nb = 0; nb < 8; ++nb) {
+        float a = cs_b1[nb], b = cs_b2[nb], d = cs_w3[nb];
+        a += __shfl_xor(a, 16, 64);
+        a += __shfl_xor(a, 32, 64);
+        b += __shfl_xor(b, 16, 64);
+        b += __shfl_xor(b, 32, 64);
+        d += __shfl_xor(d, 16, 64);
+        d += __shfl_xor(d, 32, 64);
+        if (lg == 0) {
+            atomicAdd(&g[O_B1 + 16 * nb + lr], a);
+            atomicAdd(&g[O_B2 + 16 * nb + lr], b);
+            atomicAdd(&g[O_W3 + 16 * nb + lr], d);
+        }
+    }
+}
+
+// Weight gradients of the force loss, as k_wgrad with two products per block:
+// dW2 = H1^T dZ2 + H1'^T dT2 (64 blocks), dW1 = X'^T dZ1 + U'^T dT1 (32 blocks).
+__global__ __launch_bounds__(64) void k_wgrad_tan(const float *__restrict__ P, int c, int n,
+                                                  const float *__restrict__ X, const float *__restrict__ U,
+                                                  const v4s *__restrict__ frag, int tpc, float *__restrict__ grad) {
+    const int lane = threadIdx.x, lr = lane & 15, lg = lane >> 4;
+    const int blk = blockIdx.x, ntile = (n + 15) / 16;
+    const int t0 = blockIdx.y * tpc, t1 = min(ntile, t0 + tpc);
+    v4f acc = {0.f, 0.f, 0.f, 0.f};
+    float *g = grad + HEAD + c * CLASS_SIZE;
+    if (blk < 64) {
+        const int kb = blk / 8, nb = blk % 8;
+        for (int t = t0; t < t1; ++t) {
+            const v4s *f = frag + (size_t)t * 48 * 64;
+            acc = mfma(f[kb * 64 + lane], f[(16 + nb) * 64 + lane], acc);
+            acc = mfma(f[(8 + kb) * 64 + lane], f[(24 + nb) * 64 + lane], acc);
+        }
+        for (int q = 0; q < 4; ++q) atomicAdd(&g[O_W2 + (16 * kb + 4 * lg + q) * NHID + 16 * nb + lr], acc[q]);
+    } else {
+        const int b = blk - 64, fb = b / 8, nb = b % 8;
+        const int f = 16 * fb + lr;
+        const float mu = P[f], is = P[NFEAT + f];
+        for (int t = t0; t < t1; ++t) {
+            float v[4], u[4];
+            for (int q = 0; q < 4; ++q) {
+                const int row = 16 * t + 4 * lg + q;
+                v[q] = row < n ? (X[(size_t)row * NFEAT + f] - mu) * is : 0.f;
+                u[q] = row < n ? U[(size_t)row * NFEAT + f] * is : 0.f;
+            }
+            const v4s *fr = frag + (size_t)t * 48 * 64;
+            acc = mfma(pack4(v[0], v[1], v[2], v[3]), fr[(32 + nb) * 64 + lane], acc);
+            acc = mfma(pack4(u[0], u[1], u[2], u[3]), fr[(40 + nb) * 64 + lane], acc);
+        }
+        for (int q = 0; q < 4; ++q) atomicAdd(&g[O_W1 + (16 * fb + 4 * lg + q) * NHID + 16 * nb + lr], acc[q]);
+    }
+}
+
 // Adam on the trainable entries (W1, b1, W2, b2, w3, b3 of every class); clears the gradient.
 __global__ void k_adam(float *__restrict__ P, float *__restrict__ grad, float *__restrict__ m, float *__restrict__ v,
                        float lr, float b1, float b2, float eps, float c1, float c2) {
@@ -397,6 +636,17 @@ void launch_train(const float *P, const void *PK, int c, int n, const float *X, 
     const int tpc = 64;
     hipLaunchKernelGGL(k_wgrad, dim3(96, (ntile + tpc - 1) / tpc), dim3(64), 0, s, P, c, n, X, (const v4s *)work,
                        tpc, grad);
+}
+
+void launch_train_force(const float *P, const void *PK, int c, int n, const float *X, const float *U, const float *Wt,
+                        float *D, void *work, float *grad, hipStream_t s) {
+    if (n <= 0) return;
+    const int ntile = (n + 15) / 16;
+    hipLaunchKernelGGL(k_train_tan, dim3((ntile + 3) / 4), dim3(256), 0, s, P, (const v4s *)PK, c, n, X, U, Wt, D,
+                       (v4s *)work, grad);
+    const int tpc = 64;
+    hipLaunchKernelGGL(k_wgrad_tan, dim3(96, (ntile + tpc - 1) / tpc), dim3(64), 0, s, P, c, n, X, U,
+                       (const v4s *)work, tpc, grad);
 }
 
 void launch_adam(float *P, float *grad, float *m, float *v, float lr, float b1, float b2, float eps, int step,

@@ -1,5 +1,5 @@
 // C ABI of the stand-in network's kernels on caller-owned device buffers (include/d3x_sur.h):
-// packing, inference, training gradients, Adam.  Used by the trainer
+// packing, inference, training gradients (energy and force losses), Adam.  Used by the trainer
 // (lasp-d3_amd/surrogate/train.py) and by tests; the engine calls the launchers directly.
 #include <hip/hip_runtime_api.h>
 
@@ -34,7 +34,10 @@ extern "C" {
 
 size_t d3x_sur_packed_bytes(void) { return PACKED_BYTES; }
 
-size_t d3x_sur_train_bytes(uint64_t rows) { return (size_t)((rows + 15) / 16) * 16 * TRAIN_ROW_BYTES; }
+size_t d3x_sur_train_bytes(uint64_t rows) {
+    return (size_t)((rows + 15) / 16) * 16 * (TRAIN_FORCE_ROW_BYTES > TRAIN_ROW_BYTES ? TRAIN_FORCE_ROW_BYTES
+                                                                                    : TRAIN_ROW_BYTES);
+}
 
 int d3x_sur_pack(const float *params, void *packed, void *stream) {
     if (!params || !packed) return bad("d3x_sur_pack", "NULL buffer");
@@ -60,6 +63,17 @@ int d3x_sur_train_grad(const float *params, const void *packed, int cls, uint64_
     if (n > (uint64_t)INT32_MAX) return bad("d3x_sur_train_grad", "too many rows");
     launch_train(params, packed, cls, (int)n, X, y, w, E, work, grad, loss, (hipStream_t)stream);
     return launched("d3x_sur_train_grad");
+}
+
+int d3x_sur_train_force_grad(const float *params, const void *packed, int cls, uint64_t n, const float *X,
+                             const float *U, const float *w, float *D, void *work, float *grad, void *stream) {
+    if (n == 0) return 0;
+    if (!params || !packed || !X || !U || !w || !D || !work || !grad)
+        return bad("d3x_sur_train_force_grad", "NULL buffer");
+    if (cls < 0 || cls >= NCLASS) return bad("d3x_sur_train_force_grad", "class out of range");
+    if (n > (uint64_t)INT32_MAX) return bad("d3x_sur_train_force_grad", "too many rows");
+    launch_train_force(params, packed, cls, (int)n, X, U, w, D, work, grad, (hipStream_t)stream);
+    return launched("d3x_sur_train_force_grad");
 }
 
 int d3x_sur_adam(float *params, float *grad, float *m, float *v, float lr, float beta1, float beta2, float eps,

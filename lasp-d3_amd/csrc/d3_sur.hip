@@ -17,6 +17,12 @@
 // both depend on r_ij), gradient of atom i = sum_j dE/dr_ij d_ij / r_ij (no scatter: each atom's
 // wave gathers its own), virial = 1/2 sum_j dE/dr_ij d d^T / r (each pair is seen from both ends).
 // The CN part of the forces goes through the chain-rule pass: dE/dCN_i = G_i[0] is added to g.
+//
+// k_sur_tan: the push-forward u_i = sum_j (dx_i/dr_j) . dir_j of a direction field dir (per A of
+// position step) through the features, for force matching: with dir = F - F_ref and w = -2,
+// sum_i d(G_i . u_i)/dparams is the gradient of sum |F - F_ref|^2.  One wave per centre: the
+// radial sums as in k_sur_feat with phi_m'(r) (d_ij / r) . (dir_i - dir_j) in place of phi_m(r),
+// then u_i[0] = sum_j c'(r) (d_ij / r) . (dir_i - dir_j) over the CN pair set (as k_chain).
 #include <hip/hip_runtime.h>
 
 #include "d3_device.hpp"
@@ -198,7 +204,92 @@ __global__ __launch_bounds__(BLOCK) void k_sur_back(Params<T> P) {
     }
 }
 
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_sur_tan(Params<T> P, const float *__restrict__ dir, float *__restrict__ U) {
+    __shared__ float Qr[WPB][SQ], Qp[WPB][SQ];
+    __shared__ int Qc[WPB][SQ];
+    __shared__ float Red[WPB][3][64];
+    const int lane = lane_id(), wv = threadIdx.x / WAVE;
+    const int i = blockIdx.x * WPB + wv;
+    if (i >= P.n) return;  // whole wave; no block barrier below
+    Centre<T> C;
+    C.load(P, i);
+    const float di0 = dir[3 * i], di1 = dir[3 * i + 1], di2 = dir[3 * i + 2];
+    const int m = lane % NRBF, e = lane / NRBF;  // lane 63: e = 3, idle in the sums
+    const float mu = (float)MU0 + (float)DMU * m, is2 = (float)(0.5 / (SIGMA * SIGMA)),
+                isig2 = (float)(1.0 / (SIGMA * SIGMA));
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+    auto push = [&](int pos, int j, float dx, float dy, float dz, float r) {
+        Qr[wv][pos] = r;
+        Qp[wv][pos] = (dx * (di0 - dir[3 * j]) + dy * (di1 - dir[3 * j + 1]) + dz * (di2 - dir[3 * j + 2])) / r;
+        Qc[wv][pos] = P.type_cls[P.type_s[j]];
+    };
+    auto flush = [&](int qn) {
+        for (int q = 0; q < qn; q += 3) {
+            const int k = q + e;
+            if (e < 3 && k < qn) {
+                const float r = Qr[wv][k], u = r - mu;
+                float dfc;
+                const float fc = cutf(r, dfc);
+                const float v = __expf(-u * u * is2) * (dfc - fc * u * isig2) * Qp[wv][k];
+                const int c = Qc[wv][k];
+                a0 += c == 0 ? v : 0.f;
+                a1 += c == 1 ? v : 0.f;
+                a2 += c == 2 ? v : 0.f;
+            }
+        }
+    };
+    int qn = 0;
+    scan_neighbours(P, C, lane, qn, push, flush);
+    // CN row
+    const T rci = P.rcov[C.ti];
+    float cacc = 0.f;
+    const int s0 = C.S->st_off[CUT_CN], s1 = s0 + C.S->st_cnt[CUT_CN];
+    for (int sb = s0; sb < s1; sb += WAVE) {
+        const auto Bn = C.bins(P, sb, s1);
+        const int nsb = s1 - sb < WAVE ? s1 - sb : WAVE;
+        for (int sk = 0; sk < nsb; ++sk) {
+            int j0, j1;
+            T ox, oy, oz;
+            Bn.get(sk, j0, j1, ox, oy, oz);
+            for (int j = j0 + lane; j < j1; j += WAVE) {
+                const T dx = ox - P.lx[j], dy = oy - P.ly[j], dz = oz - P.lz[j];
+                const T r2 = dx * dx + dy * dy + dz * dz;
+                if (r2 <= P.rc_cn2 && r2 > T(0)) {
+                    const T ir = xrsqrt(r2);
+                    const T rr = rci + P.rcov[P.type_s[j]];
+                    const T ex = xexp(-T(K1) * (rr * ir - T(1)));
+                    const T ope = T(1) + ex;
+                    const T dcn = finite_or_zero(-T(K1) * ex * rr * ir * ir * ir / (ope * ope));  // c'(r) / r
+                    cacc += (float)dcn * ((float)dx * (di0 - dir[3 * j]) + (float)dy * (di1 - dir[3 * j + 1]) +
+                                          (float)dz * (di2 - dir[3 * j + 2]));
+                }
+            }
+        }
+    }
+    cacc = wave_sumf(cacc);
+    Red[wv][0][lane] = a0;
+    Red[wv][1][lane] = a1;
+    Red[wv][2][lane] = a2;
+    wave_sync();
+    // per A of step: the features are functions of bohr distances
+    const float ib = (float)(1.0 / sur::BOHR_A);
+    float *u = U + (size_t)i * NFEAT;
+    if (lane < 3 * NRBF) {
+        const int c = lane / NRBF, mm = lane % NRBF;
+        u[1 + lane] = ib * (Red[wv][c][mm] + Red[wv][c][NRBF + mm] + Red[wv][c][2 * NRBF + mm]);
+    } else {
+        u[0] = ib * cacc;
+    }
+}
+
 }  // namespace
+
+template <class T>
+void launch_sur_tangent(const Params<T> &P, const float *dir, float *U, hipStream_t s) {
+    if (P.n == 0) return;
+    hipLaunchKernelGGL(k_sur_tan<T>, dim3((P.n + WPB - 1) / WPB), dim3(BLOCK), 0, s, P, dir, U);
+}
 
 template <class T>
 void launch_sur_features(const Params<T> &P, hipStream_t s) {
@@ -217,6 +308,8 @@ void launch_surrogate(const Params<T> &P, hipStream_t s) {
 
 template void launch_sur_features<double>(const Params<double> &, hipStream_t);
 template void launch_sur_features<float>(const Params<float> &, hipStream_t);
+template void launch_sur_tangent<double>(const Params<double> &, const float *, float *, hipStream_t);
+template void launch_sur_tangent<float>(const Params<float> &, const float *, float *, hipStream_t);
 template void launch_surrogate<double>(const Params<double> &, hipStream_t);
 template void launch_surrogate<float>(const Params<float> &, hipStream_t);
 

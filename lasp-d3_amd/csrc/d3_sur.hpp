@@ -42,8 +42,9 @@ constexpr size_t PACKED_BYTES = (size_t)NCLASS * PK_CLASS * 8;
 // element class of atomic number Z: 0 = H, He; 1 = Li..Ne; 2 = the rest
 inline int element_class(int Z) { return Z <= 2 ? 0 : (Z <= 10 ? 1 : 2); }
 
-// Training scratch per row of a batch (bytes): fragments of h1, dz2, dz1 (bf16).
-constexpr size_t TRAIN_ROW_BYTES = 3 * NHID * 2;
+// Training scratch per row of a batch (bytes): fragments of h1, dz2, dz1 (bf16) for the energy
+// loss; of h1, h1', dz2, dt2, dz1, dt1 for the force loss (tangent pass).
+constexpr size_t TRAIN_ROW_BYTES = 3 * NHID * 2, TRAIN_FORCE_ROW_BYTES = 6 * NHID * 2;
 
 // Launchers (d3_mlp.hip).  P: fp32 parameters, PK: packed bf16 fragments (PACKED_BYTES).
 void launch_pack(const float *P, void *PK, hipStream_t s);
@@ -51,6 +52,8 @@ void launch_infer(const float *P, const void *PK, int n, const float *X, const i
                   hipStream_t s);
 void launch_train(const float *P, const void *PK, int c, int n, const float *X, const float *Y, const float *Wt,
                   float *E, void *work, float *grad, float *loss, hipStream_t s);
+void launch_train_force(const float *P, const void *PK, int c, int n, const float *X, const float *U, const float *Wt,
+                        float *D, void *work, float *grad, hipStream_t s);
 void launch_adam(float *P, float *grad, float *m, float *v, float lr, float b1, float b2, float eps, int step,
                  hipStream_t s);
 

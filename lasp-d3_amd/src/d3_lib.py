@@ -84,11 +84,13 @@ SIGNATURES = {
     "d3x_set_atm_mode": (C.c_int, [_vp, C.c_int, C.c_int, C.c_double]),
     "d3x_sur_state": (C.c_int, [_vp, _ip, _ip, _dp]),
     "d3x_sur_features": (C.c_int, [_vp, _fp]),
+    "d3x_sur_tangent": (C.c_int, [_vp, _dp, _fp]),
     "d3x_sur_packed_bytes": (C.c_size_t, []),
     "d3x_sur_train_bytes": (C.c_size_t, [_u64]),
     "d3x_sur_pack": (C.c_int, [_vp, _vp, _vp]),
     "d3x_sur_infer": (C.c_int, [_vp, _vp, _u64, _vp, _vp, _vp, _vp, _vp]),
     "d3x_sur_train_grad": (C.c_int, [_vp, _vp, C.c_int, _u64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "d3x_sur_train_force_grad": (C.c_int, [_vp, _vp, C.c_int, _u64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "d3x_sur_adam": (C.c_int, [_vp, _vp, _vp, _vp, _f, _f, _f, _f, C.c_int, _vp]),
 }
 
@@ -301,6 +303,13 @@ class Handle:
         x = np.zeros((self.n, SUR_NFEAT), np.float32)
         self._check(self.lib.d3x_sur_features(self.h, ptr(x, C.c_float)), "d3x_sur_features")
         return x
+
+    def sur_tangent(self, direction) -> np.ndarray:
+        """U = J dir (atoms x 64) at the last evaluation's geometry; dir atoms x 3 per A."""
+        d = np.ascontiguousarray(direction, dtype=np.float64).reshape(self.n, 3)
+        u = np.zeros((self.n, SUR_NFEAT), np.float32)
+        self._check(self.lib.d3x_sur_tangent(self.h, ptr(d, C.c_double), ptr(u, C.c_float)), "d3x_sur_tangent")
+        return u
 
     def status(self) -> int:
         return int(self.lib.d3x_last_status(self.h))

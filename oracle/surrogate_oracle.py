@@ -16,6 +16,10 @@ src/d3_kernel.cu:1199-1509):
     E_i = scale_c (w3 . h2 + b3) + shift_c   [eV]
   energy E = sum_i E_i; forces and stress by the chain rule through the features
   (and through CN_i for x_i[0]).
+Force matching: with F = -J^T G (J: Jacobian of the features of every atom with respect to every
+  position, G = dE_i/dx_i), the force loss sum |F - F_ref|^2 has the parameter gradient
+  -2 sum_i d(G_i . u_i)/dparams at fixed u = J (F - F_ref) (feature_tangent); G_i . u_i is the
+  network's derivative along u_i (mlp_tangent: forward tangent pass, then its reverse pass).
 Parameter vector (float32 on the device, float64 here): mean[64], istd[64], then per class
   W1[64][128], b1[128], W2[128][128], b2[128], w3[128], (b3, scale, shift, pad).
 The D3 CN here restates the oracle's C code (oracle/d3_oracle.cpp:156-170).
@@ -76,6 +80,11 @@ def silu(z):
 def dsilu(z):
     s = 1.0 / (1.0 + np.exp(-z))
     return s * (1.0 + z * (1.0 - s))
+
+
+def d2silu(z):
+    s = 1.0 / (1.0 + np.exp(-z))
+    return s * (1.0 - s) * (2.0 + z * (1.0 - 2.0 * s))
 
 
 # ---------------------------------------------------------------------------- geometry
@@ -239,6 +248,86 @@ def mlp_bf16(p, X, cls, dl=None):
     return E, G, g
 
 
+def mlp_tangent(p, X, U, cls, w=None):
+    """D_i = dE_i/dx_i . u_i (the network's derivative along U, eV per unit of U) and, when w is
+    given, the gradient of sum_i w_i D_i with respect to the trainable parameters (float64)."""
+    istd = p[N_FEAT:HEAD]
+    xs = (X - p[:N_FEAT]) * istd
+    us = U * istd
+    D = np.zeros(len(X))
+    g = np.zeros_like(p)
+    for c in range(N_CLASS):
+        m = cls == c
+        if not m.any():
+            continue
+        k = class_block(p, c)
+        z1 = xs[m] @ k["W1"] + k["b1"]
+        t1 = us[m] @ k["W1"]
+        h1, s1 = silu(z1), dsilu(z1)
+        h1t = s1 * t1
+        z2 = h1 @ k["W2"] + k["b2"]
+        t2 = h1t @ k["W2"]
+        s2 = dsilu(z2)
+        h2t = s2 * t2
+        D[m] = k["scale"] * (h2t @ k["w3"])
+        if w is None:
+            continue
+        o = HEAD + c * CLASS_SIZE
+        gam = k["scale"] * w[m]
+        g[o + O_W3:o + O_TAIL] = h2t.T @ gam
+        dt2 = gam[:, None] * k["w3"][None, :] * s2
+        dz2 = gam[:, None] * k["w3"][None, :] * d2silu(z2) * t2
+        g[o + O_W2:o + O_B2] = (h1.T @ dz2 + h1t.T @ dt2).ravel()
+        g[o + O_B2:o + O_W3] = dz2.sum(0)
+        dh1, dh1t = dz2 @ k["W2"].T, dt2 @ k["W2"].T
+        dt1 = dh1t * s1
+        dz1 = dh1 * s1 + dh1t * d2silu(z1) * t1
+        g[o + O_W1:o + O_B1] = (xs[m].T @ dz1 + us[m].T @ dt1).ravel()
+        g[o + O_B1:o + O_W2] = dz1.sum(0)
+    return D, g
+
+
+def mlp_tangent_bf16(p, X, U, cls, w):
+    """mlp_tangent with the MFMA kernel's roundings (lasp-d3_amd/csrc/d3_mlp.hip, tangent pass):
+    bf16 operands for x', u' = u istd (formed in fp32), W1, W2, h1, h1' = silu'(z1) t1, dt2, dz2,
+    dt1 and dz1; the rest wider.  Returns D and the parameter gradient of sum w D."""
+    p32 = np.asarray(p, np.float32).astype(np.float64)
+    X32, U32 = np.asarray(X, np.float32), np.asarray(U, np.float32)
+    mu, istd = p32[:N_FEAT].astype(np.float32), p32[N_FEAT:HEAD].astype(np.float32)
+    xs, us = bf16((X32 - mu) * istd), bf16(U32 * istd)
+    D = np.zeros(len(X))
+    g = np.zeros_like(p32)
+    for c in range(N_CLASS):
+        m = cls == c
+        if not m.any():
+            continue
+        k = class_block(p32, c)
+        W1b, W2b = bf16(k["W1"]), bf16(k["W2"])
+        z1 = xs[m] @ W1b + k["b1"]
+        t1 = us[m] @ W1b
+        s1 = dsilu(z1)
+        h1b, h1tb = bf16(silu(z1)), bf16(s1 * t1)
+        z2 = h1b @ W2b + k["b2"]
+        t2 = h1tb @ W2b
+        s2 = dsilu(z2)
+        h2t = s2 * t2
+        D[m] = k["scale"] * (h2t @ k["w3"])
+        o = HEAD + c * CLASS_SIZE
+        gam = k["scale"] * w[m]
+        g[o + O_W3:o + O_TAIL] = h2t.T @ gam
+        dt2 = gam[:, None] * k["w3"][None, :] * s2
+        dz2 = gam[:, None] * k["w3"][None, :] * d2silu(z2) * t2
+        dt2b, dz2b = bf16(dt2), bf16(dz2)
+        g[o + O_W2:o + O_B2] = (h1b.T @ dz2b + h1tb.T @ dt2b).ravel()
+        g[o + O_B2:o + O_W3] = dz2.sum(0)
+        dh1, dh1t = dz2b @ W2b.T, dt2b @ W2b.T
+        dt1 = dh1t * s1
+        dz1 = dh1 * s1 + dh1t * d2silu(z1) * t1
+        g[o + O_W1:o + O_B1] = (xs[m].T @ bf16(dz1) + us[m].T @ bf16(dt1)).ravel()
+        g[o + O_B1:o + O_W2] = dz1.sum(0)
+    return D, g
+
+
 # ---------------------------------------------------------------------------- energy and forces
 
 def evaluate(p, pos_A, Z, cell_A, rcov_of_Z, rc_cn, bf16_net=False):
@@ -270,3 +359,22 @@ def evaluate(p, pos_A, Z, cell_A, rcov_of_Z, rc_cn, bf16_net=False):
     V = np.linalg.det(cell_b)
     return dict(energy=E.sum(), atom_energy=E, forces=-grad / BOHR, stress=vir / V / BOHR ** 3, features=X,
                 dedcn=G[:, 0], cn=cn)
+
+
+def feature_tangent(pos_A, Z, cell_A, rcov_of_Z, rc_cn, dirv):
+    """u_i = sum_j (dx_i / dr_j) . dir_j: the change of every atom's features per unit step of
+    the positions along dir (dir per A; u in feature units per A), through the radial sums and
+    through CN_i (x_i[0])."""
+    Z = np.asarray(Z)
+    pos_b = np.asarray(pos_A, float) / BOHR
+    cell_b = np.asarray(cell_A, float) / BOHR
+    dirv = np.asarray(dirv, float)
+    cn, (ci, cj, cd, cr, dc) = coordination(pos_b, Z, cell_b, rcov_of_Z, rc_cn)
+    _, (i, j, d, r, dphi, col) = features(pos_b, Z, cell_b, cn)
+    U = np.zeros((len(Z), N_FEAT))
+    # dr_ij along dir (bohr per A of step): d_ij / r_ij . (dir_i - dir_j) / BOHR
+    pr = np.einsum("pk,pk->p", d, dirv[i] - dirv[j]) / r / BOHR
+    np.add.at(U, (np.repeat(i, N_RBF), col.ravel()), (dphi * pr[:, None]).ravel())
+    pc = np.einsum("pk,pk->p", cd, dirv[ci] - dirv[cj]) / cr / BOHR
+    U[:, 0] = np.bincount(ci, weights=dc * pc, minlength=len(Z))
+    return U

@@ -154,3 +154,125 @@ def test_bf16_rounding_and_emulated_network():
     _, _, g16 = SO.mlp_bf16(p, X, cls, dl=dl)
     g64 = SO.mlp_param_grad(p, cache, dl)
     assert np.max(np.abs(g16 - g64)) <= 5e-2 * np.max(np.abs(g64))
+
+
+# ---------------------------------------------------------------------------- force matching
+
+def test_tangent_matches_fd():
+    """D = dE/dx . u equals the finite difference of the forward pass along u."""
+    p = random_params(seed=21)
+    rng = np.random.default_rng(22)
+    X = rng.normal(0, 1, (9, SO.N_FEAT))
+    U = rng.normal(0, 1, (9, SO.N_FEAT))
+    cls = np.array([0, 1, 2, 0, 1, 2, 2, 1, 0])
+    D, _ = SO.mlp_tangent(p, X, U, cls)
+    h = 1e-6
+    fd = (SO.mlp_forward(p, X + h * U, cls)[0] - SO.mlp_forward(p, X - h * U, cls)[0]) / (2 * h)
+    assert np.max(np.abs(D - fd)) <= 1e-7 * max(1e-3, np.max(np.abs(D)))
+    # and equals G . u from the input gradient
+    E, cache = SO.mlp_forward(p, X, cls)
+    G = SO.mlp_input_grad(p, cache, len(X))
+    np.testing.assert_allclose(D, np.einsum("if,if->i", G, U), rtol=1e-12, atol=1e-15)
+
+
+def test_tangent_param_gradient_matches_fd():
+    """The reverse pass over the tangent: d(sum w D)/dparams equals finite differences."""
+    p = random_params(seed=23)
+    rng = np.random.default_rng(24)
+    X = rng.normal(0, 1, (8, SO.N_FEAT))
+    U = rng.normal(0, 1, (8, SO.N_FEAT))
+    cls = np.array([0, 1, 2, 2, 1, 0, 2, 1])
+    w = rng.normal(0, 1, len(X))
+
+    def obj(q):
+        return np.sum(w * SO.mlp_tangent(q, X, U, cls)[0])
+
+    _, g = SO.mlp_tangent(p, X, U, cls, w)
+    h = 1e-6
+    idx = []
+    for c in range(SO.N_CLASS):
+        o = SO.HEAD + c * SO.CLASS_SIZE
+        idx += [o + SO.O_W1 + 5, o + SO.O_W1 + 64 * 128 - 3, o + SO.O_W1 + 1000, o + SO.O_B1 + 7, o + SO.O_W2 + 300,
+                o + SO.O_W2 + 128 * 128 - 1, o + SO.O_B2 + 1, o + SO.O_W3 + 9, o + SO.O_TAIL]
+    scale = np.max(np.abs(g))
+    for k in idx:
+        pp, pm = p.copy(), p.copy()
+        pp[k] += h
+        pm[k] -= h
+        fd = (obj(pp) - obj(pm)) / (2 * h)
+        assert abs(fd - g[k]) <= 1e-6 * max(1e-3 * scale, abs(g[k])), k
+    assert np.all(g[:SO.HEAD] == 0)
+    for c in range(SO.N_CLASS):  # b3 does not enter a derivative; scale, shift are not trained
+        o = SO.HEAD + c * SO.CLASS_SIZE + SO.O_TAIL
+        assert g[o] == 0 and g[o + 1] == 0 and g[o + 2] == 0 and g[o + 3] == 0
+
+
+def test_feature_tangent_matches_fd(rcov):
+    """u = J dir equals the finite difference of the features (radial sums and CN) along dir."""
+    pos, Z, cell = small(seed=9, n=10)
+    rng = np.random.default_rng(25)
+    dirv = rng.normal(0, 1, pos.shape)
+    U = SO.feature_tangent(pos, Z, cell, rcov, 25.0, dirv)
+
+    def feats(q):
+        qb = q / SO.BOHR
+        cn, _ = SO.coordination(qb, Z, cell / SO.BOHR, rcov, 25.0)
+        return SO.features(qb, Z, cell / SO.BOHR, cn)[0]
+
+    h = 1e-6
+    fd = (feats(pos + h * dirv) - feats(pos - h * dirv)) / (2 * h)
+    assert np.max(np.abs(U[:, 0] - fd[:, 0])) <= 1e-6 * max(1.0, np.max(np.abs(fd[:, 0])))
+    assert np.max(np.abs(U[:, 1:] - fd[:, 1:])) <= 1e-6 * np.max(np.abs(fd[:, 1:]))
+    # a rigid translation changes no feature
+    Ut = SO.feature_tangent(pos, Z, cell, rcov, 25.0, np.tile([0.3, -1.0, 2.0], (len(Z), 1)))
+    assert np.max(np.abs(Ut)) <= 1e-12 * np.max(np.abs(U))
+
+
+def test_force_matching_gradient_matches_fd(rcov):
+    """End to end: for L = sum |F(params) - F_ref|^2 of the stand-in on a small cell,
+    dL/dparams = -2 sum_i d(G_i . u_i)/dparams with u = J (F - F_ref), which is mlp_tangent's
+    gradient with w = -2 on the features' tangent along F - F_ref."""
+    pos, Z, cell = small(seed=10, n=8)
+    p = random_params(seed=26)
+    rng = np.random.default_rng(27)
+    o = SO.evaluate(p, pos, Z, cell, rcov, 25.0)
+    F_ref = o["forces"] + rng.normal(0, np.std(o["forces"]), o["forces"].shape)
+
+    def L(q):
+        return np.sum((SO.evaluate(q, pos, Z, cell, rcov, 25.0)["forces"] - F_ref) ** 2)
+
+    dF = o["forces"] - F_ref
+    U = SO.feature_tangent(pos, Z, cell, rcov, 25.0, dF)
+    cls = SO.element_class(Z)
+    _, g = SO.mlp_tangent(p, o["features"], U, cls, np.full(len(Z), -2.0))
+    # the identity behind it: sum_i G_i . u_i = -sum_i F_i . dF_i
+    E, cache = SO.mlp_forward(p, o["features"], cls)
+    G = SO.mlp_input_grad(p, cache, len(Z))
+    assert abs(np.sum(G * U) + np.sum(o["forces"] * dF)) <= 1e-9 * np.sum(np.abs(o["forces"] * dF))
+    present = sorted(set(cls.tolist()))
+    idx = []
+    for c in present:
+        b = SO.HEAD + c * SO.CLASS_SIZE
+        idx += [b + SO.O_W1 + 3, b + SO.O_W1 + 64 * 70 + 11, b + SO.O_B1 + 5, b + SO.O_W2 + 777, b + SO.O_B2 + 2,
+                b + SO.O_W3 + 4]
+    h = 1e-6
+    scale = np.max(np.abs(g))
+    for k in idx:
+        pp, pm = p.copy(), p.copy()
+        pp[k] += h
+        pm[k] -= h
+        fd = (L(pp) - L(pm)) / (2 * h)
+        assert abs(fd - g[k]) <= 1e-5 * max(1e-3 * scale, abs(g[k])), k
+
+
+def test_tangent_bf16_emulation_close_to_float64():
+    p = random_params(seed=28)
+    rng = np.random.default_rng(29)
+    X = rng.normal(0, 1, (60, SO.N_FEAT))
+    U = rng.normal(0, 1, (60, SO.N_FEAT))
+    cls = rng.integers(0, 3, 60)
+    w = rng.normal(0, 1, 60)
+    D64, g64 = SO.mlp_tangent(p, X, U, cls, w)
+    D16, g16 = SO.mlp_tangent_bf16(p, X, U, cls, w)
+    assert np.max(np.abs(D16 - D64)) <= 5e-2 * np.max(np.abs(D64))
+    assert np.max(np.abs(g16 - g64)) <= 5e-2 * np.max(np.abs(g64))
