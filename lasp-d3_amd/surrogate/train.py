@@ -8,8 +8,12 @@ library's own kernels (include/d3x_sur.h: d3x_sur_pack / d3x_sur_train_grad / d3
    so energies, forces and per-atom energies are those of the three-body term alone, in the mode
    that also returns the network's features.
 2. One network per element class (H; Li..Ne; heavier), fitted jointly to per-atom three-body
-   energies with Adam on random batches; inputs normalised by the training set's feature
-   statistics, outputs by each class's label mean and spread.
+   energies and to the frames' three-body forces with Adam: each step takes random batches of
+   atoms per class for the energy loss, and the force loss of one random training frame -- the
+   network's forces on it from the engine, dF = F_net - F_exact, u = J dF through the features
+   (d3x_sur_tangent) and the tangent-pass gradient of the network (d3x_sur_train_force_grad).
+   Inputs normalised by the training set's feature statistics, outputs by each class's label
+   mean and spread, forces by each kind's RMS exact force.
 3. Validation on held-out frames (every 20th): per-atom and per-frame energy errors, and through
    the engine: the judge's |dE| / atoms, force errors against the exact kernel, and the time of
    the three-body phase (exact kernel vs network).
@@ -18,7 +22,7 @@ Writes the parameter file (d3x_sur_load_file) and a JSON report.  PyTorch provid
 and the stream only.
 
 usage: python lasp-d3_amd/surrogate/train.py [--si-frames 1000] [--water-frames 1000]
-       [--water-bases 25] [--steps 6000] [--out FILE] [--report FILE]
+       [--water-bases 25] [--steps 6000] [--force-weight 1.0] [--out FILE] [--report FILE]
 """
 
 from __future__ import annotations
@@ -49,8 +53,8 @@ def log(*a):
     print("[train]", *a, flush=True)
 
 
-def make_handle(Z, n, mode=d3_lib.D3X_ATM_EXACT_FEATURES):
-    h = d3_lib.Handle(np.unique(Z), n, cutoff=RC2, cn_cutoff=RCCN, atm_cutoff=RC3, damping=1, functional=0,
+def make_handle(Z, n, mode=d3_lib.D3X_ATM_EXACT_FEATURES, rc2=RC2):
+    h = d3_lib.Handle(np.unique(Z), n, cutoff=rc2, cn_cutoff=RCCN, atm_cutoff=RC3, damping=1, functional=0,
                       precision=d3_lib.D3X_FP32)
     h.set_params(ATM_ONLY)
     h.set_atm_mode(mode)
@@ -97,8 +101,7 @@ def build_set(args):
         y[r:r + n] = ea
         Zall[r:r + n] = Z
         fid[r:r + n] = k
-        frames.append(dict(kind=kind, index=k, start=r, n=n, energy=e, forces=f if k % HOLDOUT == HOLDOUT - 1 else None,
-                           pos=pos if k % HOLDOUT == HOLDOUT - 1 else None, Z=Z, cell=cell))
+        frames.append(dict(kind=kind, index=k, start=r, n=n, energy=e, forces=f, pos=pos, Z=Z, cell=cell))
         r += n
         if len(frames) % 100 == 0:
             log(f"labelled {len(frames)} frames, {r} atoms, {time.time() - t0:.1f} s")
@@ -107,7 +110,78 @@ def build_set(args):
     return X, y, Zall, fid, frames, time.time() - t0
 
 
-def train(args, X, y, Zall, fid):
+class ForceLoss:
+    """Force-matching term of one frame per step.  The network's forces on the frame come from
+    an engine handle in network mode (two-body scale factors at zero and a two-body cutoff below
+    every bond, so the forces are the network's alone); u = J (F_net - F_exact) from
+    d3x_sur_tangent; the gradient of weight * sum |F_net - F_exact|^2 / (3 n rms_kind^2) goes
+    into grad through d3x_sur_train_force_grad, class by class."""
+
+    def __init__(self, args, frames, Xd, cls, lib, st, work, torch):
+        self.torch, self.lib, self.st, self.work = torch, lib, st, work
+        self.Xd, self.cls = Xd, cls
+        self.weight = args.force_weight
+        self.frames = [fr for fr in frames if fr["index"] % HOLDOUT != HOLDOUT - 1]
+        self.rms = {}
+        for kind in {fr["kind"] for fr in self.frames}:
+            f = np.concatenate([fr["forces"].ravel() for fr in self.frames if fr["kind"] == kind])
+            self.rms[kind] = float(np.sqrt(np.mean(f ** 2)))
+        self.handles = {}
+        self.rng = np.random.default_rng(args.seed + 1)
+        dev = torch.device("cuda")
+        nmax = max(fr["n"] for fr in self.frames)
+        self.D = torch.zeros(nmax, device=dev)
+        self.wbuf = torch.zeros(nmax, device=dev)
+        self.loss_sum, self.count = 0.0, 0
+
+    def handle(self, fr):
+        h = self.handles.get(fr["kind"])
+        if h is None:
+            h = make_handle(fr["Z"], fr["n"], d3_lib.D3X_ATM_EXACT, rc2=1.0)
+            self.handles[fr["kind"]] = h
+        return h
+
+    def step(self, P, pk):
+        torch, lib, vp = self.torch, self.lib, (lambda t: C.c_void_p(t.data_ptr()))  # noqa: E731
+        fr = self.frames[self.rng.integers(len(self.frames))]
+        h = self.handle(fr)
+        h.sur_load(P.cpu().numpy())
+        h.set_atm_mode(d3_lib.D3X_ATM_SURROGATE, 0, 1e9)
+        h.set_atoms(fr["pos"], fr["Z"])
+        h.set_cell(fr["cell"])
+        h.compute_device()  # not judged: the network alone
+        _, F, _ = h.fetch()
+        dF = F - fr["forces"]
+        n = fr["n"]
+        scale = self.weight / (3 * n * self.rms[fr["kind"]] ** 2)
+        self.loss_sum += scale * float(np.sum(dF ** 2))
+        self.count += 1
+        U = torch.from_numpy(h.sur_tangent(dF)).to(P.device)
+        a = fr["start"]
+        cf = self.cls[a:a + n]
+        chk = 0
+        for c in np.unique(cf):
+            idx = np.nonzero(cf == c)[0]
+            ti = torch.from_numpy(idx).to(P.device)
+            Xc = self.Xd[a:a + n].index_select(0, ti).contiguous()
+            Uc = U.index_select(0, ti).contiguous()
+            w = self.wbuf[:len(idx)]
+            w.fill_(-2.0 * scale)
+            chk |= lib.d3x_sur_train_force_grad(vp(P), vp(pk), int(c), len(idx), vp(Xc), vp(Uc), vp(w), vp(self.D),
+                                                vp(self.work), vp(self.grad), self.st)
+        return chk
+
+    def take_loss(self):
+        l = self.loss_sum / max(1, self.count)
+        self.loss_sum, self.count = 0.0, 0
+        return l
+
+    def close(self):
+        for h in self.handles.values():
+            h.close()
+
+
+def train(args, X, y, Zall, fid, frames):
     import torch
     lib = d3_lib.load()
     st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
@@ -134,7 +208,13 @@ def train(args, X, y, Zall, fid):
     B = args.batch
     grad, m1, m2 = torch.zeros_like(P), torch.zeros_like(P), torch.zeros_like(P)
     pk = torch.empty(lib.d3x_sur_packed_bytes(), dtype=torch.uint8, device=dev)
-    work = torch.empty(lib.d3x_sur_train_bytes(B), dtype=torch.uint8, device=dev)
+    nmax = max([B] + [fr["n"] for fr in frames])
+    work = torch.empty(lib.d3x_sur_train_bytes(nmax), dtype=torch.uint8, device=dev)
+    fl = None
+    if args.force_weight > 0:
+        fl = ForceLoss(args, frames, Xd, cls, lib, st, work, torch)
+        fl.grad = grad
+        log("force rms per kind (eV/A):", json.dumps(fl.rms))
     E = torch.zeros(B, device=dev)
     loss = torch.zeros(1, device=dev)
     w = {c: torch.full((B,), 1.0 / (B * scale[c] ** 2), device=dev) for c in present}
@@ -151,6 +231,8 @@ def train(args, X, y, Zall, fid):
             yb = yd.index_select(0, sel).contiguous()
             chk |= lib.d3x_sur_train_grad(vp(P), vp(pk), c, B, vp(Xb), vp(yb), vp(w[c]), vp(E), vp(work), vp(grad),
                                           vp(loss), st)
+        if fl is not None and step % args.force_every == 0:
+            chk |= fl.step(P, pk)
         frac = step / args.steps
         lr = args.lr * (0.5 * (1 + np.cos(np.pi * frac))) * min(1.0, step / 200) + 1e-6
         chk |= lib.d3x_sur_adam(vp(P), vp(grad), vp(m1), vp(m2), C.c_float(lr), C.c_float(0.9), C.c_float(0.999),
@@ -160,9 +242,12 @@ def train(args, X, y, Zall, fid):
         if step % 500 == 0 or step == 1:
             torch.cuda.synchronize()
             l = loss.item() / len(present)
-            curve.append((step, l))
-            log(f"step {step} loss {l:.4e} lr {lr:.2e} ({time.time() - t0:.1f} s)")
+            lf = fl.take_loss() if fl is not None else 0.0
+            curve.append((step, l, lf))
+            log(f"step {step} energy loss {l:.4e} force loss {lf:.4e} lr {lr:.2e} ({time.time() - t0:.1f} s)")
     torch.cuda.synchronize()
+    if fl is not None:
+        fl.close()
     p = P.cpu().numpy()
     # per-atom predictions for every row (held-out ones for validation)
     pred = np.empty(len(y), np.float32)
@@ -220,6 +305,9 @@ def main():
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--strain", type=float, default=0.03, help="isotropic strain range of the frames")
+    ap.add_argument("--force-weight", type=float, default=1.0,
+                    help="weight of the force loss (per kind normalised by the RMS exact force); 0: energies only")
+    ap.add_argument("--force-every", type=int, default=1, help="force-loss frame every this many steps")
     ap.add_argument("--eval-random", type=int, default=0,
                     help="also check a random H/C/N/O/Si/Cu cell of this many atoms (0.08 atoms/A^3)")
     ap.add_argument("--out", default=os.path.join(HERE, "atm_surrogate.d3sur"))
@@ -231,7 +319,7 @@ def main():
     torch.zeros(1, device="cuda")
     X, y, Zall, fid, frames, t_lab = build_set(args)
     log(f"training set: {len(y)} atoms from {len(frames)} frames in {t_lab:.1f} s")
-    p, pred, curve, t_train, stats = train(args, X, y, Zall, fid)
+    p, pred, curve, t_train, stats = train(args, X, y, Zall, fid, frames)
     d3_sur.save(args.out, p)
     log(f"wrote {args.out}")
     held = (fid % HOLDOUT) == HOLDOUT - 1
@@ -251,7 +339,7 @@ def main():
     frame_err = {k: float(max(e for kk, e in per_frame if kk == k)) for k in {k for k, _ in per_frame}}
     cases, seen = [], set()
     for fr in frames:  # one held-out frame of each kind
-        if fr["pos"] is not None and fr["kind"] not in seen:
+        if fr["index"] % HOLDOUT == HOLDOUT - 1 and fr["kind"] not in seen:
             seen.add(fr["kind"])
             cases.append((fr["kind"], fr["pos"], fr["Z"], fr["cell"]))
     if args.eval_random:
@@ -259,7 +347,8 @@ def main():
         cases.append((f"random{args.eval_random}", pos, Z, cell))
     checks = engine_checks(p, cases)
     report = dict(frames=len(frames), atoms=int(len(y)), label_seconds=t_lab, train_seconds=t_train,
-                  steps=args.steps, batch=args.batch, curve=curve, heldout_per_class=per_class,
+                  steps=args.steps, batch=args.batch, force_weight=args.force_weight, force_every=args.force_every,
+                  curve=curve, heldout_per_class=per_class,
                   heldout_max_frame_error_mev_per_atom=frame_err, engine=checks, stats=stats)
     if args.report:
         with open(args.report, "w") as f:

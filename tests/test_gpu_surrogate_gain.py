@@ -5,9 +5,10 @@ lasp-d3_amd/surrogate/train.py) gains over the method without it, on cells it wa
   1 meV/atom, while dropping the three-body term (the method without the network) misses by the
   whole ATM energy (~8-15 meV/atom here);
 * stress: the network removes most of the stress error of dropping the term;
-* forces (water): smaller error than dropping the term.  (The Si cell's three-body forces are
-  ~4e-5 eV/A; the network's force error there is larger than that, and the test does not claim a
-  gain for them.)
+* forces: the network is trained on the exact kernel's forces too (force matching through the
+  features' tangent), so its force error is well below that of dropping the term (which misses by
+  the whole three-body force): on Si (three-body forces ~4e-5 eV/A) about a fifth of it, on water
+  about 60 % of it;
 * time: the network's three-body phase is >= 5x faster than the exact kernel's;
 * out of distribution (a random H/C/N/O/Si/Cu cell; C, N, Cu are not in the training set) the
   judge sees more than 1 meV/atom, falls back and the exact results come back.
@@ -62,9 +63,9 @@ def test_network_gain_on_unseen_cells(kind):
     assert err_net <= 1e-3 and err_2b >= 5e-3 and err_net <= 0.05 * err_2b
     sn, s2 = np.max(np.abs(s_net - s_full)), np.max(np.abs(s_2b - s_full))
     assert sn <= 0.2 * s2
-    if kind == "water":
-        fn, f2 = np.sqrt(np.mean((f_net - f_full) ** 2)), np.sqrt(np.mean((f_2b - f_full) ** 2))
-        assert fn < f2
+    fn, f2 = np.sqrt(np.mean((f_net - f_full) ** 2)), np.sqrt(np.mean((f_2b - f_full) ** 2))
+    print(f"  force RMS error: network {fn:.3e} eV/A, without {f2:.3e} eV/A", flush=True)
+    assert fn <= (0.4 if kind == "si" else 0.8) * f2
     assert t_exact >= 5 * t_net
 
 
