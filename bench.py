@@ -196,24 +196,43 @@ def surrogate_leg():
     return dict(weights=os.path.relpath(SUR_WEIGHTS, ROOT), tolerance_mev_per_atom=1.0, cases=cases)
 
 
-def cpu_baseline(w, table, cn, g, target_s=15.0):
-    """FP64 oracle on a bounded random sample of centre atoms, all host threads of this job."""
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(w, table, cn, g, target_s=15.0, target_1t_s=5.0):
+    """FP64 oracle on a bounded random sample of centre atoms: all host threads of this job, then
+    one thread on a smaller sample."""
     import d3_oracle
     if w["kind"] != "single":
         return None
     n = len(w["Z"])
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
     rng = np.random.Generator(np.random.PCG64(1234))
-    kw = dict(cutoff=RC2, cn_cutoff=RCCN, atm_cutoff=RC3, damping=1, functional=0, nthreads=threads)
+    kw = dict(cutoff=RC2, cn_cutoff=RCCN, atm_cutoff=RC3, damping=1, functional=0)
     probe = rng.choice(n, size=min(n, threads), replace=False)
-    t1, _ = d3_oracle.time_centers(w["pos"], w["Z"], w["cell"], table, probe, cn, g, **kw)
+    t1, _ = d3_oracle.time_centers(w["pos"], w["Z"], w["cell"], table, probe, cn, g, nthreads=threads, **kw)
     reps = max(1, int(target_s / max(t1, 1e-6)))
     m = int(min(n, threads * reps))
     sample = rng.choice(n, size=m, replace=False)
-    t, _ = d3_oracle.time_centers(w["pos"], w["Z"], w["cell"], table, sample, cn, g, **kw)
+    t, _ = d3_oracle.time_centers(w["pos"], w["Z"], w["cell"], table, sample, cn, g, nthreads=threads, **kw)
+    # one thread: as many centres as take about target_1t_s at the per-centre cost just measured
+    m1 = int(max(1, min(n, target_1t_s * m / (t * threads))))
+    s1 = rng.choice(n, size=m1, replace=False)
+    ts, _ = d3_oracle.time_centers(w["pos"], w["Z"], w["cell"], table, s1, cn, g, nthreads=1, **kw)
+    print(f"[bench] cpu baseline: {threads} threads {m / t:.1f} atoms/s, 1 thread {m1 / ts:.2f} atoms/s",
+          file=sys.stderr, flush=True)
     return dict(value=m / t, unit="atoms/s", cores=threads, kind="port",
                 sample=f"{m} random centre atoms of {n} (all four passes per centre, FP64 oracle, "
-                       f"{t:.1f} s); throughput scales per atom because every atom's work is one centre's")
+                       f"{t:.1f} s); throughput scales per atom because every atom's work is one centre's",
+                value_1_thread=m1 / ts, sample_1_thread=f"{m1} random centre atoms, {ts:.1f} s",
+                host_cpus=os.cpu_count(), cpu_model=cpu_model())
 
 
 def main():
