@@ -201,8 +201,10 @@ Engine::Engine(const uint16_t *elements, uint64_t num_elements, uint64_t max_len
 
     max_len_ = max_length;
     // Bins about half the CN cutoff wide: the CN / chain-rule stencils then skip most corner
-    // bins, and the two-body stencil's bins stay small against its cutoff.
-    bin_width_ = std::max(4.0, 0.5 * std::min(rccn_, rc2_));
+    // bins, and the two-body stencil's bins stay small against its cutoff.  A two-body cutoff
+    // below the CN cutoff does not shrink them: the CN and chain-rule passes would then walk
+    // thousands of near-empty bins per centre while the two-body pass saves next to nothing.
+    bin_width_ = std::max(4.0, 0.5 * rccn_);
     if (const char *tc = std::getenv("D3_TUNE_CACHE")) tune_cache_ = tc;
     sys_.resize(1);
     HIPCHK(hipStreamCreate(&own_stream_));
