@@ -97,8 +97,11 @@ struct AtmLds {
     //   J0..J3  j tile: (dx, dy) (dz, a) (1/a, a^-1/2) (sqrt C6_ij, lam_j)
     //   Jp      j-side tile sums as pairs (A, Bx) (By, Bz) (H, G)       [3 rows]
     //   Jt      without DR: the j type x ne, one int per slot            [1 row]
-    //   Jyd     per type B of k: nref rows (y, dy) and, with DR, one damping row (FP32: lam_j +
-    //           (8/3) log2 R0_{A_j B}^2, FP64: lam_j R0_{A_j B}^2; 0)
+    //   Jyd     per type B of k: with DR, one damping row (FP32: lam_j + (8/3) log2
+    //           R0_{A_j B}^2, FP64: lam_j R0_{A_j B}^2; 0), then the type's reference count of
+    //           rows (y, dy); then zero padding rows.  A pair step reads NR rows from the start of
+    //           type(k)'s C6 rows: for a type with fewer references the extra rows are the next
+    //           type's (finite values), met by k weights that are exactly zero.
     T2<T> *J0, *J1, *J2, *J3, *Jp, *Jyd;
     int *Jt;
     T *Khi, *Klo;                    // [A_NF][64] k-side sums over the j tiles (two-sum pair in FP32; KL only)
@@ -106,12 +109,13 @@ struct AtmLds {
     T *Rt;                           // per type pair: FP32 (8/3) log2 R0^2, FP64 R0^2
     static constexpr int TI = sizeof(T2<T>) / sizeof(int);  // ints per slot
     __host__ __device__ static constexpr int fixed_rows(bool dr) { return dr ? 7 : 8; }
-    __host__ __device__ static size_t bytes(int ne, int nref, bool kl, bool dr) {
+    __host__ __device__ static int jy_rows(int ne, int yrows, bool dr) { return yrows + (dr ? ne : 0); }
+    __host__ __device__ static size_t bytes(int ne, int nref, int yrows, bool kl, bool dr) {
         const size_t ynr = (size_t)ne * nref;
-        return (fixed_rows(dr) + (size_t)ne * (nref + (dr ? 1 : 0))) * AW * sizeof(T2<T>) +
+        return (fixed_rows(dr) + (size_t)jy_rows(ne, yrows, dr)) * AW * sizeof(T2<T>) +
                (size_t)(kl ? 2 : 0) * A_NF * AW * sizeof(T) + 2 * ynr * sizeof(T) + (size_t)ne * ne * sizeof(T);
     }
-    __device__ AtmLds(unsigned char *s, int ne, int nref, bool kl, bool dr) {
+    __device__ AtmLds(unsigned char *s, int ne, int nref, int yrows, bool kl, bool dr) {
         const size_t ynr = (size_t)ne * nref;
         J0 = reinterpret_cast<T2<T> *>(s);
         J1 = J0 + AW;
@@ -120,7 +124,7 @@ struct AtmLds {
         Jp = J0 + 4 * AW;
         Jt = reinterpret_cast<int *>(J0 + 7 * AW);
         Jyd = J0 + fixed_rows(dr) * AW;
-        Khi = reinterpret_cast<T *>(Jyd + (size_t)ne * (nref + (dr ? 1 : 0)) * AW);
+        Khi = reinterpret_cast<T *>(Jyd + (size_t)jy_rows(ne, yrows, dr) * AW);
         Klo = Khi + A_NF * AW;
         Cy = kl ? Klo + A_NF * AW : Khi;
         Cdy = Cy + ynr;
@@ -146,10 +150,10 @@ template <class T, int NR, bool KL, bool DR>
 __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> P) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr bool F32 = sizeof(T) == 4;
-    constexpr int NRD = NR + (DR ? 1 : 0);  // rows per type B in the j tile
+    constexpr int DRO = DR ? 1 : 0;  // C6 rows of a type start after its damping row
     const int lane = threadIdx.x;
     const int ne = P.ne, ynr = ne * NR;
-    AtmLds<T> S(smem, ne, NR, KL, DR);
+    AtmLds<T> S(smem, ne, NR, P.atm_yrows, KL, DR);
 
     const int cap = P.atm_cap;
     int *Lj = P.atm_i32 + (size_t)blockIdx.x * cap * 2;
@@ -162,6 +166,12 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
         const T r0sq = P.r0[idx] * P.r0[idx];
         S.Rt[idx] = F32 ? T((8.0f / 3.0f) * log2f((float)r0sq)) : r0sq;
     }
+    // the padding rows are read (with zero weights) and never staged: zero every row once
+    for (int r = 0; r < AtmLds<T>::jy_rows(ne, P.atm_yrows, DR); ++r) S.Jyd[r * AW + lane] = T2<T>{T(0), T(0)};
+    // lane B (< ne <= 64): first C6 row and reference count of type B, read with readlane /
+    // a lane shuffle instead of dependent loads in the tile loops
+    const int roff_l = lane < ne ? P.atm_roff[lane] : 0;
+    const int rcnt_l = lane < ne ? P.atm_roff[lane + 1] - roff_l : 0;
 
     for (;;) {
         // Ticket: every lane takes part in the atomic (lane 0 adds 1, the others 0), and the value
@@ -302,7 +312,8 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
             const float kminx = uni(wmin(kval ? (float)kx : 3e38f)), kmaxx = uni(wmax(kval ? (float)kx : -3e38f));
             const float kminy = uni(wmin(kval ? (float)ky : 3e38f)), kmaxy = uni(wmax(kval ? (float)ky : -3e38f));
             const float kminz = uni(wmin(kval ? (float)kz : 3e38f)), kmaxz = uni(wmax(kval ? (float)kz : -3e38f));
-            const T2<T> *Yk = S.Jyd + tk * NRD * AW;  // rows of type(k) in the j tile
+            // rows of type(k) in the j tile: its damping row (DR), then its C6 rows
+            const T2<T> *Yk = S.Jyd + (__shfl(roff_l, tk, AW) + (DR ? tk : 0)) * AW;
             // k-side sums over all j tiles: lane-private LDS columns, folded to FP64 once per kb
             if constexpr (KL)
                 for (int q = 0; q < A_NF; ++q) S.Khi[q * AW + lane] = S.Klo[q * AW + lane] = T(0);
@@ -348,18 +359,22 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                     S.J3[lane] = T2<T>{sc6, lam};
                     if constexpr (!DR) S.Jt[lane * S.TI] = tj * ne;  // row of the per-type-pair table
                     for (int B = 0; B < ne; ++B) {
+                        const int r0 = __builtin_amdgcn_readlane(roff_l, B), nb = __builtin_amdgcn_readlane(rcnt_l, B);
+                        const int base = r0 + (DR ? B : 0);
+                        if constexpr (DR) {
+                            const T rt = S.Rt[tj * ne + B];
+                            S.Jyd[base * AW + lane] = T2<T>{F32 ? lam + rt : lam * rt, T(0)};
+                        }
+#pragma unroll
                         for (int bb = 0; bb < NR; ++bb) {
+                            if (bb >= nb) break;  // uniform
                             T sy = T(0), sdy = T(0);
                             for (int q = 0; q < NR; ++q) {
                                 const T c = P.c6[((tj * ne + B) * NR + q) * NR + bb];
                                 sy += c * wj[q];
                                 sdy += c * dwj[q];
                             }
-                            S.Jyd[(B * NRD + bb) * AW + lane] = T2<T>{sy, sdy};
-                        }
-                        if constexpr (DR) {
-                            const T rt = S.Rt[tj * ne + B];
-                            S.Jyd[(B * NRD + NR) * AW + lane] = T2<T>{F32 ? lam + rt : lam * rt, T(0)};
+                            S.Jyd[(base + DRO + bb) * AW + lane] = T2<T>{sy, sdy};
                         }
                     }
                     for (int q = 0; q < 3; ++q) S.Jp[q * AW + lane] = T2<T>{T(0), T(0)};
@@ -403,7 +418,7 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                     const T2<T> j2 = S.J2[jl], j3 = S.J3[jl];
                     T lamjk;  // damping term of the pair's types (with lam_j)
                     if constexpr (DR) {
-                        lamjk = Yk[NR * AW + jl].x;
+                        lamjk = Yk[jl].x;
                     } else {
                         const T rt = S.Rt[S.Jt[jl * S.TI] + tk];
                         lamjk = F32 ? j3.y + rt : j3.y * rt;
@@ -412,7 +427,7 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                     T c6jk = T(0), dk6 = T(0), dj6 = T(0);
 #pragma unroll
                     for (int bb = 0; bb < NR; ++bb) {
-                        const T2<T> yd = Yk[bb * AW + jl];
+                        const T2<T> yd = Yk[(DRO + bb) * AW + jl];
                         c6jk += yd.x * kw[bb];
                         dk6 += yd.x * kdw[bb];
                         dj6 += yd.y * kw[bb];
@@ -612,8 +627,8 @@ const void *atm_kernel(int nref, int var) {
     }
 }
 
-int blocks_per_cu(int ne, int nref, size_t ts, int var) {
-    const size_t lds = atm_lds_bytes(ne, nref, ts, var);
+int blocks_per_cu(int ne, int nref, int yrows, size_t ts, int var) {
+    const size_t lds = atm_lds_bytes(ne, nref, yrows, ts, var);
     const void *fn = ts == sizeof(double) ? atm_kernel<double>(nref, var) : atm_kernel<float>(nref, var);
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, AW, lds) != hipSuccess) per_cu = 0;
@@ -622,12 +637,13 @@ int blocks_per_cu(int ne, int nref, size_t ts, int var) {
 
 }  // namespace
 
-size_t atm_lds_bytes(int ne, int nref, size_t ts, int var) {
+size_t atm_lds_bytes(int ne, int nref, int yrows, size_t ts, int var) {
     const bool kl = var & ATM_KL, dr = var & ATM_DR;
-    return ts == sizeof(double) ? AtmLds<double>::bytes(ne, nref, kl, dr) : AtmLds<float>::bytes(ne, nref, kl, dr);
+    return ts == sizeof(double) ? AtmLds<double>::bytes(ne, nref, yrows, kl, dr)
+                                : AtmLds<float>::bytes(ne, nref, yrows, kl, dr);
 }
 
-int atm_choose_variant(int ne, int nref, size_t ts) {
+int atm_choose_variant(int ne, int nref, int yrows, size_t ts) {
     // The first of KL + DR, DR, KL, neither (most work saved first: DR per pair, KL per tile)
     // that keeps at least 90 % of the most resident blocks any of them reaches.  Measured: fcc Cu
     // (FP32, 3 references) runs KL + DR at 18 blocks per CU as fast as DR at 20, with a third less
@@ -635,7 +651,8 @@ int atm_choose_variant(int ne, int nref, size_t ts) {
     const int order[4] = {ATM_KL | ATM_DR, ATM_DR, ATM_KL, 0};
     int blocks[4], most = 0;
     for (int k = 0; k < 4; ++k) {
-        blocks[k] = atm_lds_bytes(ne, nref, ts, order[k]) > 65536 ? 0 : blocks_per_cu(ne, nref, ts, order[k]);
+        blocks[k] = atm_lds_bytes(ne, nref, yrows, ts, order[k]) > 65536 ? 0
+                                                                         : blocks_per_cu(ne, nref, yrows, ts, order[k]);
         most = blocks[k] > most ? blocks[k] : most;
     }
     for (int k = 0; k < 4; ++k)
@@ -643,8 +660,8 @@ int atm_choose_variant(int ne, int nref, size_t ts) {
     return 0;
 }
 
-int atm_resident_blocks(int ne, int nref, size_t ts, int var) {
-    const int per_cu = blocks_per_cu(ne, nref, ts, var);
+int atm_resident_blocks(int ne, int nref, int yrows, size_t ts, int var) {
+    const int per_cu = blocks_per_cu(ne, nref, yrows, ts, var);
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
@@ -658,7 +675,7 @@ void launch_atm(const Params<T> &P, int nblocks, hipStream_t s) {
     const size_t nz = (size_t)6 * P.n;
     if (nz) hipLaunchKernelGGL(k_fill_d, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, s, P.fa, nz, P.atm_next);
     if (P.n == 0 || !(P.rc_32 > T(0))) return;
-    const size_t lds = atm_lds_bytes(P.ne, P.nref, sizeof(T), P.atm_var);
+    const size_t lds = atm_lds_bytes(P.ne, P.nref, P.atm_yrows, sizeof(T), P.atm_var);
     const void *fn = atm_kernel<T>(P.nref, P.atm_var);
     if (!fn) return;  // not reached: a handle's reference count is 1..MAX_NREF
     Params<T> arg = P;

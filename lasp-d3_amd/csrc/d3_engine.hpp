@@ -87,6 +87,11 @@ struct Params {
     // take (zeroed before every launch).
     int atm_cap;
     int atm_var;  // three-body kernel variant: ATM_KL | ATM_DR bits (d3_atm.hip)
+    // C6 rows of the j tile per type B of k: atm_roff[B] = sum of the reference counts of the
+    // types before B (a type's count runs to its last valid reference), atm_roff[ne] = all of
+    // them, atm_roff[ne + 1] = atm_yrows = that plus padding for the last type's NR-row window
+    const int *atm_roff;
+    int atm_yrows;
     int *atm_i32;
     T *atm_T;
     double *atm_acc;
@@ -125,9 +130,9 @@ struct Launch {
 enum { ATM_KL = 1, ATM_DR = 2 };
 template <class T>
 void launch_atm(const Params<T> &P, int nblocks, hipStream_t s);
-size_t atm_lds_bytes(int ne, int nref, size_t tsize, int var);
-int atm_choose_variant(int ne, int nref, size_t tsize);
-int atm_resident_blocks(int ne, int nref, size_t tsize, int var);
+size_t atm_lds_bytes(int ne, int nref, int yrows, size_t tsize, int var);
+int atm_choose_variant(int ne, int nref, int yrows, size_t tsize);
+int atm_resident_blocks(int ne, int nref, int yrows, size_t tsize, int var);
 // P.f += fa, P.g += ga (what launch_atm ends with).
 template <class T>
 void launch_atm_merge(const Params<T> &P, hipStream_t s);

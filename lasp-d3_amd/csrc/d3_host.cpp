@@ -173,6 +173,17 @@ Engine::Engine(const uint16_t *elements, uint64_t num_elements, uint64_t max_len
         for (int a = 0; a < tnr; ++a)
             if (tcnref[A * tnr + a] > -1.0) nref_ = std::max(nref_, a + 1);
     const int nr = nref_;
+    // three-body j-tile rows: each type keeps its references up to its last valid one, plus
+    // padding so that the last type's window of nr rows stays inside the staged rows
+    atm_roff_.assign(ne + 2, 0);
+    for (int A = 0; A < ne; ++A) {
+        int nt = 1;
+        for (int a = 0; a < nr; ++a)
+            if (tcnref[A * tnr + a] > -1.0) nt = a + 1;
+        atm_roff_[A + 1] = atm_roff_[A] + nt;
+    }
+    atm_roff_[ne + 1] = atm_roff_[ne] + nr - (atm_roff_[ne] - atm_roff_[ne - 1]);
+    yrows_ = atm_roff_[ne + 1];
     std::vector<double> c6((size_t)ne * ne * nr * nr, 0.0), cnref((size_t)ne * nr, -1.0);
     for (int A = 0; A < ne; ++A)
         for (int a = 0; a < nr; ++a) cnref[A * nr + a] = tcnref[A * tnr + a];
@@ -188,7 +199,7 @@ Engine::Engine(const uint16_t *elements, uint64_t num_elements, uint64_t max_len
                         fail(D3X_ERR_INPUT, "parameter table CN references do not factorise per element");
                     if (valid) c6[(((size_t)A * ne + B) * nr + a) * nr + b] = tab.c6ab[ix + 0];
                 }
-    if (rc3_ > 0 && atm_lds_bytes(ne_, nref_, precision == D3X_FP64 ? 8 : 4, 0) > 65536)
+    if (rc3_ > 0 && atm_lds_bytes(ne_, nref_, yrows_, precision == D3X_FP64 ? 8 : 4, 0) > 65536)
         fail(D3X_ERR_INPUT, "too many distinct elements for the three-body kernel's LDS tile");
     // layout of h_tab_: c6 | cnref | rcov | r2r4 | r0ab  (q4 / rdamp derived at upload)
     h_tab_.clear();
@@ -216,6 +227,8 @@ Engine::Engine(const uint16_t *elements, uint64_t num_elements, uint64_t max_len
     for (int A = 0; A < ne_; ++A) cls[A] = sur::element_class(elems_[A]);
     d_type_cls = dalloc(ne_ * sizeof(int));
     HIPCHK(hipMemcpy(d_type_cls, cls.data(), ne_ * sizeof(int), hipMemcpyHostToDevice));
+    d_atm_roff = dalloc(atm_roff_.size() * sizeof(int));
+    HIPCHK(hipMemcpy(d_atm_roff, atm_roff_.data(), atm_roff_.size() * sizeof(int), hipMemcpyHostToDevice));
 }
 
 Engine::~Engine() {
@@ -226,7 +239,7 @@ Engine::~Engine() {
                     &d_npair, &d_out_f, &d_out_e, &d_out_cn, &d_out_g, &d_bin_count, &d_bin_start, &d_cursor,
                     &d_sys, &d_st_d, &d_st_D, &d_chunk_b, &d_chunk_e, &d_chunk_s, &d_sys_chunk, &d_partial,
                     &d_out_sys, &d_tab, &d_status, &d_atm_i32, &d_atm_T, &d_atm_acc, &d_atm_pa, &d_type_cls,
-                    &d_feat, &d_sur_G, &d_sur_E, &d_sur_cls, &d_sur_P, &d_sur_PK, &d_tan_dir, &d_tan_U};
+                    &d_feat, &d_sur_G, &d_sur_E, &d_sur_cls, &d_sur_P, &d_sur_PK, &d_tan_dir, &d_tan_U, &d_atm_roff};
     for (void **p : all) dfree(*p);
     for (auto &e : ev_)
         if (e) (void)hipEventDestroy(e);
@@ -578,6 +591,8 @@ Params<T> Engine::params() {
     P.r0 = tab + tab_off_[5];
     P.atm_cap = atm_cap_;
     P.atm_var = rc3_ > 0 ? atm_var() : 0;
+    P.atm_roff = (const int *)d_atm_roff;
+    P.atm_yrows = yrows_;
     P.atm_i32 = (int *)d_atm_i32;
     P.atm_T = (T *)d_atm_T;
     P.atm_acc = (double *)d_atm_acc;
@@ -773,7 +788,7 @@ void Engine::apply_tune_cache() {
         bin_width_ = w;
         atm_blocks_req_ = b;
         const size_t ts = precision_ == D3X_FP64 ? sizeof(double) : sizeof(float);
-        if (v >= 0 && v <= 3 && atm_lds_bytes(ne_, nref_, ts, v) <= 65536) set_atm_variant(v);
+        if (v >= 0 && v <= 3 && atm_lds_bytes(ne_, nref_, yrows_, ts, v) <= 65536) set_atm_variant(v);
     }
 }
 
@@ -834,7 +849,7 @@ double Engine::autotune(const std::string &cache_path) {
         int best_v = atm_var();
         best = 1e300;
         for (int v = 0; v < 4; ++v) {
-            if (atm_lds_bytes(ne_, nref_, ts, v) > 65536) continue;
+            if (atm_lds_bytes(ne_, nref_, yrows_, ts, v) > 65536) continue;
             set_atm_variant(v);
             atm_blocks_req_ = 0;
             const double t = timed_evaluation();

@@ -82,7 +82,8 @@ class Engine {
     // Three-body grid of the next evaluation: the request, else the device's resident count.
     int atm_blocks_effective() { return atm_blocks_req_ > 0 ? atm_blocks_req_ : atm_resident(); }
     int atm_var() {
-        if (atm_var_ < 0) atm_var_ = atm_choose_variant(ne_, nref_, precision_ == 0 ? sizeof(double) : sizeof(float));
+        if (atm_var_ < 0)
+            atm_var_ = atm_choose_variant(ne_, nref_, yrows_, precision_ == 0 ? sizeof(double) : sizeof(float));
         return atm_var_;
     }
 
@@ -129,11 +130,13 @@ class Engine {
     uint64_t sur_evals_ = 0;
     std::vector<float> h_sur_P_;
     void *d_type_cls = nullptr, *d_feat = nullptr, *d_sur_G = nullptr, *d_sur_E = nullptr, *d_sur_cls = nullptr,
-         *d_sur_P = nullptr, *d_sur_PK = nullptr, *d_tan_dir = nullptr, *d_tan_U = nullptr;
+         *d_sur_P = nullptr, *d_sur_PK = nullptr, *d_tan_dir = nullptr, *d_tan_U = nullptr, *d_atm_roff = nullptr;
 
     // configuration
     std::vector<uint16_t> elems_;  // sorted unique atomic numbers of the handle
     int ne_ = 0, nref_ = 0;
+    std::vector<int> atm_roff_;  // three-body j-tile row offsets per type (Params::atm_roff)
+    int yrows_ = 0;
     uint64_t max_len_ = 0;
     double rc2_, rccn_, rc3_;
     int damping_, functional_, precision_;
@@ -184,7 +187,8 @@ class Engine {
     }
     int atm_resident() {
         if (atm_resident_ <= 0)
-            atm_resident_ = atm_resident_blocks(ne_, nref_, precision_ == 0 ? sizeof(double) : sizeof(float), atm_var());
+            atm_resident_ =
+                atm_resident_blocks(ne_, nref_, yrows_, precision_ == 0 ? sizeof(double) : sizeof(float), atm_var());
         return atm_resident_;
     }
     size_t atm_alloc_entries_ = 0;  // allocated scratch entries (>= atm_blocks_ * atm_cap_)
