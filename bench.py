@@ -171,7 +171,8 @@ def surrogate_leg():
     water box (its training distribution; the box itself is not in the training set) and on the
     100,000-atom random H/C/N/O/Si/Cu cell at 0.08 atoms/A^3: three-body phase time (HIP events,
     average of 3 evaluations), whole-step time, the judge's |dE| / atoms against the exact kernel,
-    and whether it fell back."""
+    whether it fell back, and the RMS force error of the returned forces against the exact ones
+    (full D3 both: the difference is the three-body term's)."""
     cases = []
     for name, (pos, Z, cell) in (("water-4096H2O", S.water_box(4096, seed=0)),
                                   ("random-100000", S.random_cell(100000, seed=4242))):
@@ -181,18 +182,20 @@ def surrogate_leg():
                           precision=d3_lib.D3X_FP32)
         h.set_atoms(pos, Z)
         h.set_cell(cell)
-        e_x = h.compute()[0]
+        e_x, f_x, _ = h.compute()
         px = phase_profile(h, runs=3)
         h.sur_load_file(SUR_WEIGHTS)
         h.set_atm_mode(d3_lib.D3X_ATM_SURROGATE, 0, 1e-3)
-        e_n = h.compute()[0]  # judged against the exact kernel
+        e_n, f_n, _ = h.compute()  # judged against the exact kernel (its results after a fallback)
         mode, fb, dev = h.sur_state()
         pn = phase_profile(h, runs=3)
         h.close()
         cases.append(dict(workload=name, atoms=n, atm_ms_exact=px["atm"], atm_ms_network=pn["atm"],
                           atm_speedup=px["atm"] / max(pn["atm"], 1e-9), step_ms_exact=sum(px.values()),
                           step_ms_network=sum(pn.values()), judge_dev_mev_per_atom=1e3 * dev,
-                          fell_back=bool(fb), mode_after=mode, e_exact_ev=e_x, e_returned_ev=e_n))
+                          fell_back=bool(fb), mode_after=mode, e_exact_ev=e_x, e_returned_ev=e_n,
+                          force_rms_exact_ev_per_a=float(np.sqrt(np.mean(f_x ** 2))),
+                          force_rms_err_ev_per_a=float(np.sqrt(np.mean((f_n - f_x) ** 2)))))
     return dict(weights=os.path.relpath(SUR_WEIGHTS, ROOT), tolerance_mev_per_atom=1.0, cases=cases)
 
 
