@@ -343,3 +343,53 @@ def test_tangent_identity_full_water_box(precision):
     rhs = -float(np.sum(F * dirv))
     scale = float(np.sum(np.abs(F * dirv)))
     assert abs(lhs - rhs) <= 1e-4 * scale, (lhs, rhs, scale)
+
+
+def test_force_matching_loop_reduces_force_loss():
+    """The trainer's force step end to end on one Si 64 cell: network forces from the engine,
+    u = J (F - F_exact) by d3x_sur_tangent, d3x_sur_train_force_grad with w = -2 / sum F_exact^2,
+    Adam: 150 steps lower sum |F - F_exact|^2 at least 4x."""
+    import d3_structures as S
+    lib = d3_lib.load()
+    pos, Z, cell = S.si_diamond(2, jitter=0.05, seed=5)
+    n = len(Z)
+    print(f"case sur-force-fit si{n}", flush=True)
+    h = d3_lib.Handle(np.unique(Z), n, cutoff=1.0, cn_cutoff=40.0, atm_cutoff=40.0, damping=1, functional=0,
+                      precision=d3_lib.D3X_FP32)
+    h.set_params(ATM_ONLY)
+    h.set_atm_mode(d3_lib.D3X_ATM_EXACT_FEATURES)
+    h.set_atoms(pos, Z)
+    h.set_cell(cell)
+    _, F_ref, _ = h.compute()
+    X = h.features()
+    _, _, ea = h.atom_data()
+    mean, std = X.mean(0), X.std(0)
+    istd = 1.0 / np.maximum(std, 1e-3 * std.max())
+    sc = max(float(np.std(ea)), 1e-6)
+    p = SO.init_params(3, mean=mean, istd=istd, scale=[sc] * 3, shift=[float(ea.mean())] * 3).astype(np.float32)
+    P = dev(p)
+    m1, m2, grad = torch.zeros_like(P), torch.zeros_like(P), torch.zeros_like(P)
+    pk = torch.empty(lib.d3x_sur_packed_bytes(), dtype=torch.uint8, device="cuda")
+    work = torch.empty(lib.d3x_sur_train_bytes(n), dtype=torch.uint8, device="cuda")
+    D = torch.zeros(n, device="cuda")
+    Xd = dev(X)
+    cls = int(SO.element_class(Z)[0])
+    w = torch.full((n,), -2.0 / float(np.sum(F_ref ** 2)), device="cuda")
+    losses = []
+    for step in range(1, 151):
+        h.sur_load(P.cpu().numpy())
+        h.set_atm_mode(d3_lib.D3X_ATM_SURROGATE, 0, 1e9)
+        h.compute_device()
+        _, F, _ = h.fetch()
+        dF = F - F_ref
+        losses.append(float(np.sum(dF ** 2) / np.sum(F_ref ** 2)))
+        U = dev(h.sur_tangent(dF))
+        assert lib.d3x_sur_pack(vp(P), vp(pk), stream()) == 0
+        assert lib.d3x_sur_train_force_grad(vp(P), vp(pk), cls, n, vp(Xd), vp(U), vp(w), vp(D), vp(work), vp(grad),
+                                            stream()) == 0
+        assert lib.d3x_sur_adam(vp(P), vp(grad), vp(m1), vp(m2), C.c_float(2e-3), C.c_float(0.9), C.c_float(0.999),
+                                C.c_float(1e-8), step, stream()) == 0
+        torch.cuda.synchronize()
+    h.close()
+    print(f"  relative force loss {losses[0]:.3e} -> {losses[-1]:.3e}", flush=True)
+    assert losses[-1] < 0.25 * losses[0], (losses[0], losses[-1])
