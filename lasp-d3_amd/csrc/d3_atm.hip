@@ -99,9 +99,9 @@ struct AtmLds {
     //   Jt      without DR: the j type x ne, one int per slot            [1 row]
     //   Jyd     per type B of k: with DR, one damping row (FP32: lam_j + (8/3) log2
     //           R0_{A_j B}^2, FP64: lam_j R0_{A_j B}^2; 0), then the type's reference count of
-    //           rows (y, dy); then zero padding rows.  A pair step reads NR rows from the start of
+    //           rows (y, dy); then padding rows.  A pair step reads NR rows from the start of
     //           type(k)'s C6 rows: for a type with fewer references the extra rows are the next
-    //           type's (finite values), met by k weights that are exactly zero.
+    //           type's (finite values) or padding, met by k weights that are exactly zero.
     T2<T> *J0, *J1, *J2, *J3, *Jp, *Jyd;
     int *Jt;
     T *Khi, *Klo;                    // [A_NF][64] k-side sums over the j tiles (two-sum pair in FP32; KL only)
@@ -166,12 +166,9 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
         const T r0sq = P.r0[idx] * P.r0[idx];
         S.Rt[idx] = F32 ? T((8.0f / 3.0f) * log2f((float)r0sq)) : r0sq;
     }
-    // the padding rows are read (with zero weights) and never staged: zero every row once
-    for (int r = 0; r < AtmLds<T>::jy_rows(ne, P.atm_yrows, DR); ++r) S.Jyd[r * AW + lane] = T2<T>{T(0), T(0)};
-    // lane B (< ne <= 64): first C6 row and reference count of type B, read with readlane /
-    // a lane shuffle instead of dependent loads in the tile loops
+    // lane B (< ne <= 64): first C6 row of type B, read with readlane / a lane shuffle instead of
+    // dependent loads in the tile loops
     const int roff_l = lane < ne ? P.atm_roff[lane] : 0;
-    const int rcnt_l = lane < ne ? P.atm_roff[lane + 1] - roff_l : 0;
 
     for (;;) {
         // Ticket: every lane takes part in the atomic (lane 0 adds 1, the others 0), and the value
@@ -358,16 +355,18 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                     S.J2[lane] = T2<T>{ia, ra};
                     S.J3[lane] = T2<T>{sc6, lam};
                     if constexpr (!DR) S.Jt[lane * S.TI] = tj * ne;  // row of the per-type-pair table
+                    // Types in ascending order, NR rows each: the rows past a type's own count
+                    // (zero: its C6 entries there are 0) land on the next type's rows, which that
+                    // type then writes over; the last type's fill the padding.  Every row a pair
+                    // step can read is thus rewritten at each staging.
                     for (int B = 0; B < ne; ++B) {
-                        const int r0 = __builtin_amdgcn_readlane(roff_l, B), nb = __builtin_amdgcn_readlane(rcnt_l, B);
-                        const int base = r0 + (DR ? B : 0);
+                        const int base = __builtin_amdgcn_readlane(roff_l, B) + (DR ? B : 0);
                         if constexpr (DR) {
                             const T rt = S.Rt[tj * ne + B];
                             S.Jyd[base * AW + lane] = T2<T>{F32 ? lam + rt : lam * rt, T(0)};
                         }
 #pragma unroll
                         for (int bb = 0; bb < NR; ++bb) {
-                            if (bb >= nb) break;  // uniform
                             T sy = T(0), sdy = T(0);
                             for (int q = 0; q < NR; ++q) {
                                 const T c = P.c6[((tj * ne + B) * NR + q) * NR + bb];
