@@ -158,7 +158,8 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
     const int cap = P.atm_cap;
     int *Lj = P.atm_i32 + (size_t)blockIdx.x * cap * 2;
     int *Lt = Lj + cap;
-    T *LT = P.atm_T + (size_t)blockIdx.x * cap * ATM_NF;
+    T *LT = P.atm_T + (size_t)blockIdx.x * atm_T_stride(cap);
+    T *LB = LT + (size_t)ATM_NF * cap;  // per tile: min x, max x, min y, max y, min z, max z
     double *La = P.atm_acc + (size_t)blockIdx.x * atm_acc_stride(cap);
     const T rc32 = P.rc_32;
 
@@ -240,9 +241,12 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
         const int nf = count > cap ? 0 : count;
         __syncthreads();  // phase-1 slots were written by arbitrary lanes
 
-        // ---- phase 2: records; zero the accumulators.  Slot p is always touched by lane p % 64,
-        // here and below.
-        for (int p = lane; p < nf; p += AW) {
+        // ---- phase 2: records and each tile's bounding box (float, padding lanes excluded); zero
+        // the accumulators.  Slot p is always touched by lane p % 64, here and below.
+        for (int t = 0; t * AW < nf; ++t) {
+            const int p = t * AW + lane;
+            float bb[6] = {3e38f, -3e38f, 3e38f, -3e38f, 3e38f, -3e38f};
+            if (p < nf) {
             const int j = Lj[p], tj = P.type_s[j];
             T c6 = T(0), dci = T(0), dcj = T(0);
             for (int b = 0; b < NR; ++b) {
@@ -259,8 +263,24 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
             // never waits on them) and are read back with atomic loads (L2, not a stale L1 line)
             for (int q = 0; q < A_NF; ++q)
                 __hip_atomic_store(&La[q * cap + p], 0.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const float x = (float)LT[F_DX * cap + p], y = (float)LT[F_DY * cap + p], z = (float)LT[F_DZ * cap + p];
+            bb[0] = bb[1] = x;
+            bb[2] = bb[3] = y;
+            bb[4] = bb[5] = z;
+            }
+            // all lanes: one box per tile, computed once here instead of at every tile pair
+            const float v = wmin(bb[0]), w = wmax(bb[1]), v2 = wmin(bb[2]), w2 = wmax(bb[3]), v3 = wmin(bb[4]),
+                        w3 = wmax(bb[5]);
+            if (lane == 0) {
+                LB[8 * t + 0] = T(v);
+                LB[8 * t + 1] = T(w);
+                LB[8 * t + 2] = T(v2);
+                LB[8 * t + 3] = T(w2);
+                LB[8 * t + 4] = T(v3);
+                LB[8 * t + 5] = T(w3);
+            }
         }
-        __syncthreads();
+        __syncthreads();  // the boxes were written by lane 0
 
         // ---- phase 3: unordered pairs {j, k} of forward neighbours.  Per-pair terms are summed in T
         // over one 64 x 64 tile (at most 64 terms per sum); tile sums go to the FP64 per-neighbour
@@ -306,9 +326,9 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                 klam = F32 ? lam + T(L2K6_83) : T(K6) * lam;
             }
             // bounding box of the k wave (valid lanes), wave-uniform
-            const float kminx = uni(wmin(kval ? (float)kx : 3e38f)), kmaxx = uni(wmax(kval ? (float)kx : -3e38f));
-            const float kminy = uni(wmin(kval ? (float)ky : 3e38f)), kmaxy = uni(wmax(kval ? (float)ky : -3e38f));
-            const float kminz = uni(wmin(kval ? (float)kz : 3e38f)), kmaxz = uni(wmax(kval ? (float)kz : -3e38f));
+            const float kminx = uni((float)LB[8 * kb + 0]), kmaxx = uni((float)LB[8 * kb + 1]);
+            const float kminy = uni((float)LB[8 * kb + 2]), kmaxy = uni((float)LB[8 * kb + 3]);
+            const float kminz = uni((float)LB[8 * kb + 4]), kmaxz = uni((float)LB[8 * kb + 5]);
             // rows of type(k) in the j tile: its damping row (DR), then its C6 rows
             const T2<T> *Yk = S.Jyd + (__shfl(roff_l, tk, AW) + (DR ? tk : 0)) * AW;
             // k-side sums over all j tiles: lane-private LDS columns, folded to FP64 once per kb
@@ -318,16 +338,10 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
             for (int jt = 0; jt <= kb; ++jt) {
                 const int pj = jt * AW + lane;
                 const bool jval = pj < nf;
-                // bounding box of the j tile from the records (global scratch, same-lane slots)
-                T jxs = T(FAR), jys = T(FAR), jzs = T(FAR);
-                if (jval) {
-                    jxs = LT[F_DX * cap + pj];
-                    jys = LT[F_DY * cap + pj];
-                    jzs = LT[F_DZ * cap + pj];
-                }
-                const float jminx = uni(wmin(jval ? (float)jxs : 3e38f)), jmaxx = uni(wmax(jval ? (float)jxs : -3e38f));
-                const float jminy = uni(wmin(jval ? (float)jys : 3e38f)), jmaxy = uni(wmax(jval ? (float)jys : -3e38f));
-                const float jminz = uni(wmin(jval ? (float)jzs : 3e38f)), jmaxz = uni(wmax(jval ? (float)jzs : -3e38f));
+                // bounding box of the j tile (phase 2)
+                const float jminx = uni((float)LB[8 * jt + 0]), jmaxx = uni((float)LB[8 * jt + 1]);
+                const float jminy = uni((float)LB[8 * jt + 2]), jmaxy = uni((float)LB[8 * jt + 3]);
+                const float jminz = uni((float)LB[8 * jt + 4]), jmaxz = uni((float)LB[8 * jt + 5]);
                 const float gx = fmaxf(0.f, fmaxf(jminx - kmaxx, kminx - jmaxx));
                 const float gy = fmaxf(0.f, fmaxf(jminy - kmaxy, kminy - jmaxy));
                 const float gz = fmaxf(0.f, fmaxf(jminz - kmaxz, kminz - jmaxz));
@@ -337,11 +351,15 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                 // stage the j tile (padding lanes: far coordinates, zero rows)
                 wave_sync();
                 {
+                    T jxs = T(FAR), jys = T(FAR), jzs = T(FAR);
                     T a = T(1), ia = T(1), ra = T(1), sc6 = T(0), lam = T(0);
                     int tj = 0;
                     T wj[NR], dwj[NR];
                     for (int q = 0; q < NR; ++q) wj[q] = dwj[q] = T(0);
                     if (jval) {
+                        jxs = LT[F_DX * cap + pj];
+                        jys = LT[F_DY * cap + pj];
+                        jzs = LT[F_DZ * cap + pj];
                         const int jj = Lj[pj];
                         tj = Lt[pj];
                         for (int q = 0; q < NR; ++q) {

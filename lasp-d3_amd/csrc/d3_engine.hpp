@@ -79,7 +79,8 @@ struct Params {
     // r0[ne][ne] (ATM damping radii)
     const T *c6, *cnref, *rcov, *q4, *rdamp, *r0;
     // ATM: per-block scratch for one centre's forward neighbours (capacity atm_cap):
-    //   atm_i32: j, type (2 x cap); atm_T: record fields (ATM_NF x cap: offset, CN log-derivatives);
+    //   atm_i32: j, type (2 x cap); atm_T: record fields (ATM_NF x cap: offset, CN log-derivatives)
+    //   and per tile of 64 slots its bounding box (8 values: min, max per axis, 2 unused);
     //   atm_acc: per-neighbour accumulators (6 x cap, double)
     // and per-atom results (sorted order): fa[3N], ga[N], eas[N] filled with atomics (every
     // triangle's gradient, dE/dCN and e/3 for each vertex), eat[N] = sum of e over the
@@ -111,6 +112,9 @@ struct Params {
 constexpr int ATM_NF = 5;  // record fields per ATM neighbour (offset, CN log-derivatives)
 // doubles of ATM accumulator scratch per block: 6 per neighbour slot
 __host__ __device__ constexpr size_t atm_acc_stride(int cap) { return 6 * (size_t)cap; }
+// values of ATM record scratch per block (cap a multiple of 64): ATM_NF per neighbour slot, then
+// 8 per tile of 64 slots
+__host__ __device__ constexpr size_t atm_T_stride(int cap) { return ATM_NF * (size_t)cap + 8 * (size_t)(cap / 64); }
 
 // Host-side launchers (d3_kernels.hip).
 template <class T>

@@ -527,7 +527,7 @@ void Engine::ensure_atm_scratch() {
         atm_alloc_entries_ = (size_t)cap * nb;
         atm_alloc_blocks_ = nb;
         d_atm_i32 = dalloc(atm_alloc_entries_ * 2 * sizeof(int));
-        d_atm_T = dalloc(atm_alloc_entries_ * ATM_NF * ts);
+        d_atm_T = dalloc((size_t)nb * atm_T_stride(cap) * ts);
         d_atm_acc = dalloc((size_t)nb * atm_acc_stride(cap) * sizeof(double));
     }
 }
