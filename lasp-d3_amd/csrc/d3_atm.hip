@@ -247,37 +247,37 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
             const int p = t * AW + lane;
             float bb[6] = {3e38f, -3e38f, 3e38f, -3e38f, 3e38f, -3e38f};
             if (p < nf) {
-            const int j = Lj[p], tj = P.type_s[j];
-            T c6 = T(0), dci = T(0), dcj = T(0);
-            for (int b = 0; b < NR; ++b) {
-                const T wj = P.w[b * P.n + j], dwj = P.dw[b * P.n + j];
-                c6 += S.Cy[tj * NR + b] * wj;
-                dci += S.Cdy[tj * NR + b] * wj;
-                dcj += S.Cy[tj * NR + b] * dwj;
-            }
-            const T ic6 = c6 > T(0) ? T(1) / c6 : T(0);
-            Lt[p] = tj;
-            LT[F_DLI * cap + p] = dci * ic6;
-            LT[F_DLJ * cap + p] = dcj * ic6;
-            // the accumulators take FP64 atomic adds in phase 3 (no returned value, so the wave
-            // never waits on them) and are read back with atomic loads (L2, not a stale L1 line)
-            for (int q = 0; q < A_NF; ++q)
-                __hip_atomic_store(&La[q * cap + p], 0.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const float x = (float)LT[F_DX * cap + p], y = (float)LT[F_DY * cap + p], z = (float)LT[F_DZ * cap + p];
-            bb[0] = bb[1] = x;
-            bb[2] = bb[3] = y;
-            bb[4] = bb[5] = z;
+                const int j = Lj[p], tj = P.type_s[j];
+                T c6 = T(0), dci = T(0), dcj = T(0);
+                for (int b = 0; b < NR; ++b) {
+                    const T wj = P.w[b * P.n + j], dwj = P.dw[b * P.n + j];
+                    c6 += S.Cy[tj * NR + b] * wj;
+                    dci += S.Cdy[tj * NR + b] * wj;
+                    dcj += S.Cy[tj * NR + b] * dwj;
+                }
+                const T ic6 = c6 > T(0) ? T(1) / c6 : T(0);
+                Lt[p] = tj;
+                LT[F_DLI * cap + p] = dci * ic6;
+                LT[F_DLJ * cap + p] = dcj * ic6;
+                // the accumulators take FP64 atomic adds in phase 3 (no returned value, so the wave
+                // never waits on them) and are read back with atomic loads (L2, not a stale L1 line)
+                for (int q = 0; q < A_NF; ++q)
+                    __hip_atomic_store(&La[q * cap + p], 0.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const float x = (float)LT[F_DX * cap + p], y = (float)LT[F_DY * cap + p], z = (float)LT[F_DZ * cap + p];
+                bb[0] = bb[1] = x;
+                bb[2] = bb[3] = y;
+                bb[4] = bb[5] = z;
             }
             // all lanes: one box per tile, computed once here instead of at every tile pair
-            const float v = wmin(bb[0]), w = wmax(bb[1]), v2 = wmin(bb[2]), w2 = wmax(bb[3]), v3 = wmin(bb[4]),
-                        w3 = wmax(bb[5]);
+            const float xlo = wmin(bb[0]), xhi = wmax(bb[1]), ylo = wmin(bb[2]), yhi = wmax(bb[3]),
+                        zlo = wmin(bb[4]), zhi = wmax(bb[5]);
             if (lane == 0) {
-                LB[8 * t + 0] = T(v);
-                LB[8 * t + 1] = T(w);
-                LB[8 * t + 2] = T(v2);
-                LB[8 * t + 3] = T(w2);
-                LB[8 * t + 4] = T(v3);
-                LB[8 * t + 5] = T(w3);
+                LB[8 * t + 0] = T(xlo);
+                LB[8 * t + 1] = T(xhi);
+                LB[8 * t + 2] = T(ylo);
+                LB[8 * t + 3] = T(yhi);
+                LB[8 * t + 4] = T(zlo);
+                LB[8 * t + 5] = T(zhi);
             }
         }
         __syncthreads();  // the boxes were written by lane 0
