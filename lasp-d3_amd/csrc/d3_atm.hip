@@ -176,7 +176,8 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
         // is read back as a scalar.  No lane-dependent branch may sit in this loop: the compiler
         // may split a loop on such a branch into an inner loop for lanes 1..63 and an outer one
         // for lane 0, and lanes 1..63 would then run on without ever taking a ticket.
-        const int i = __builtin_amdgcn_readfirstlane(atomicAdd(P.atm_next, lane == 0 ? 1 : 0));
+        const int ticket = __builtin_amdgcn_readfirstlane(atomicAdd(P.atm_next, lane == 0 ? 1 : 0));
+        const int nch = P.atm_chunks, i = ticket / nch, chunk = ticket - i * nch;
         if (i >= P.n) break;
         Centre<T> C;
         C.load(P, i);
@@ -302,7 +303,11 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
             const T r0 = P.r0[ti * ne + tp];
             lam = F32 ? T((8.0f / 3.0f) * log2f((float)(r0 * r0 * ia))) : r0 * r0 * ia;
         };
-        for (int kb = 0; kb < ntile; ++kb) {
+        // this work item's k tiles: chunk c of nch takes [ntile sqrt(c / nch), ntile sqrt((c + 1) / nch)),
+        // about the same number of tile pairs each (tile kb pairs with jt = 0..kb)
+        const int kb_lo = chunk == 0 ? 0 : min(ntile, (int)(ntile * sqrtf((float)chunk / nch) + 0.5f));
+        const int kb_hi = chunk == nch - 1 ? ntile : min(ntile, (int)(ntile * sqrtf((float)(chunk + 1) / nch) + 0.5f));
+        for (int kb = kb_lo; kb < kb_hi; ++kb) {
             const int k = kb * AW + lane;
             const bool kval = k < nf;
             // padding lanes sit far away, so every pair with them fails the cutoff test
@@ -600,10 +605,16 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
             atomicAdd(&P.fa[3 * i + 2], F[2]);
             atomicAdd(&P.ga[i], G);
             atomicAdd(&P.eas[i], E * (1.0 / 3.0));
-            P.eat[i] = E;
             const double iv = C.S->inv_vol;
-            for (int q = 0; q < 6; ++q) P.vir[6 * i + q] += W[q] * iv;
-            P.npair[3 * i + 2] = (float)cnt;
+            if (nch == 1) {
+                P.eat[i] = E;
+                for (int q = 0; q < 6; ++q) P.vir[6 * i + q] += W[q] * iv;
+                P.npair[3 * i + 2] = (float)cnt;
+            } else {
+                P.atm_eatc[(size_t)chunk * P.n + i] = E;
+                for (int q = 0; q < 6; ++q) atomicAdd(&P.vir[6 * i + q], W[q] * iv);
+                atomicAdd(&P.npair[3 * i + 2], (float)cnt);  // whole numbers below 2^24: exact
+            }
         }
     }
 }
@@ -613,6 +624,16 @@ __global__ void k_fill_d(double *p, size_t n, int *counter) {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) p[t] = 0.0;
     if (t == 0) *counter = 0;
+}
+
+// Per-centre three-body energy from the work items' shares, in chunk order (deterministic).
+template <class T>
+__global__ void k_eat_chunks(Params<T> P) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    double e = 0.0;
+    for (int c = 0; c < P.atm_chunks; ++c) e += P.atm_eatc[(size_t)c * P.n + i];
+    P.eat[i] = e;
 }
 
 // P.g += ga, P.f += fa (after the ATM kernel, before the chain rule).
@@ -698,6 +719,7 @@ void launch_atm(const Params<T> &P, int nblocks, hipStream_t s) {
     Params<T> arg = P;
     void *args[] = {&arg};
     (void)hipLaunchKernel(fn, dim3(nblocks), dim3(AW), args, lds, s);
+    if (P.atm_chunks > 1) hipLaunchKernelGGL(k_eat_chunks<T>, dim3((P.n + 255) / 256), dim3(256), 0, s, P);
     hipLaunchKernelGGL(k_merge<T>, dim3((P.n + 255) / 256), dim3(256), 0, s, P);
 }
 

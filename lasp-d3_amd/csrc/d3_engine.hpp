@@ -98,6 +98,12 @@ struct Params {
     double *atm_acc;
     double *fa, *ga, *eas, *eat;
     int *atm_next;
+    // Work items per centre (1..8): with atm_chunks > 1 item t is centre t / atm_chunks and a
+    // contiguous range of its k tiles; each item writes its share of the centre's energy to
+    // atm_eatc[chunk * n + centre] (summed in chunk order after the launch) and adds its virial
+    // and triangle count atomically.  More items than centres fill the GPU on small cells.
+    int atm_chunks;
+    double *atm_eatc;
     // three-body stand-in (d3_sur.hip): class of each element type, features [N][64], the
     // network's dE/dx [N][64] and energies (eV), class per sorted atom, fp32 parameters and
     // packed bf16 weights

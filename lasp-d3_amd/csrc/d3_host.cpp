@@ -239,7 +239,7 @@ Engine::~Engine() {
                     &d_npair, &d_out_f, &d_out_e, &d_out_cn, &d_out_g, &d_bin_count, &d_bin_start, &d_cursor,
                     &d_sys, &d_st_d, &d_st_D, &d_chunk_b, &d_chunk_e, &d_chunk_s, &d_sys_chunk, &d_partial,
                     &d_out_sys, &d_tab, &d_status, &d_atm_i32, &d_atm_T, &d_atm_acc, &d_atm_pa, &d_type_cls,
-                    &d_feat, &d_sur_G, &d_sur_E, &d_sur_cls, &d_sur_P, &d_sur_PK, &d_tan_dir, &d_tan_U, &d_atm_roff};
+                    &d_feat, &d_sur_G, &d_sur_E, &d_sur_cls, &d_sur_P, &d_sur_PK, &d_tan_dir, &d_tan_U, &d_atm_roff, &d_atm_eatc};
     for (void **p : all) dfree(*p);
     for (auto &e : ev_)
         if (e) (void)hipEventDestroy(e);
@@ -515,7 +515,17 @@ void Engine::ensure_atm_scratch() {
     int cap = (int)std::min(262144.0, 1.5 * mean + 1024.0);
     cap = (std::max(cap, atm_cap_) + 63) / 64 * 64;
     int nb = atm_blocks_req_ > 0 ? atm_blocks_req_ : atm_resident();
-    nb = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)nb, n_));
+    // Work items per centre: small cells would leave most resident waves without a centre (512
+    // centres on 1024 SIMDs), and a few centres per wave leave a long tail; split each centre's k
+    // tiles into up to 8 items until there are about 8 items per wave.
+    const uint64_t ncen = std::max<uint64_t>(n_, 1);
+    atm_chunks_ = (int)std::min<uint64_t>(8, std::max<uint64_t>(1, (8 * (uint64_t)nb + ncen - 1) / ncen));
+    nb = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)nb, ncen * atm_chunks_));
+    if (atm_chunks_ > 1 && (size_t)atm_chunks_ * n_ > atm_eatc_alloc_) {
+        dfree(d_atm_eatc);
+        atm_eatc_alloc_ = (size_t)8 * max_len_;
+        d_atm_eatc = dalloc(atm_eatc_alloc_ * sizeof(double));
+    }
     atm_cap_ = cap;
     atm_blocks_ = nb;
     // the kernel indexes scratch as blockIdx.x * atm_cap_: reallocate whenever either grows
@@ -591,6 +601,8 @@ Params<T> Engine::params() {
     P.r0 = tab + tab_off_[5];
     P.atm_cap = atm_cap_;
     P.atm_var = rc3_ > 0 ? atm_var() : 0;
+    P.atm_chunks = atm_chunks_;
+    P.atm_eatc = (double *)d_atm_eatc;
     P.atm_roff = (const int *)d_atm_roff;
     P.atm_yrows = yrows_;
     P.atm_i32 = (int *)d_atm_i32;

@@ -130,7 +130,8 @@ class Engine {
     uint64_t sur_evals_ = 0;
     std::vector<float> h_sur_P_;
     void *d_type_cls = nullptr, *d_feat = nullptr, *d_sur_G = nullptr, *d_sur_E = nullptr, *d_sur_cls = nullptr,
-         *d_sur_P = nullptr, *d_sur_PK = nullptr, *d_tan_dir = nullptr, *d_tan_U = nullptr, *d_atm_roff = nullptr;
+         *d_sur_P = nullptr, *d_sur_PK = nullptr, *d_tan_dir = nullptr, *d_tan_U = nullptr, *d_atm_roff = nullptr,
+         *d_atm_eatc = nullptr;
 
     // configuration
     std::vector<uint16_t> elems_;  // sorted unique atomic numbers of the handle
@@ -193,6 +194,8 @@ class Engine {
     }
     size_t atm_alloc_entries_ = 0;  // allocated scratch entries (>= atm_blocks_ * atm_cap_)
     int atm_alloc_blocks_ = 0;      // blocks the scratch was allocated for (>= atm_blocks_)
+    int atm_chunks_ = 1;            // three-body work items per centre (Params::atm_chunks)
+    size_t atm_eatc_alloc_ = 0;     // doubles in d_atm_eatc (per-item centre energies)
 
     bool tuning_explicit_ = false;  // set_tuning / autotune fixed the parameters
     std::string tune_cache_;        // cache consulted at geometry build (empty: none)
