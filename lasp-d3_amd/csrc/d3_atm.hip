@@ -240,11 +240,19 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
         // loop in the simple shape described at the ticket.)
         if (count > cap && lane == 0) atomicOr(P.status, 2u);
         const int nf = count > cap ? 0 : count;
+        const int ntile = (nf + AW - 1) / AW;
+        // this work item's k tiles: chunk c of nch takes [ntile sqrt(c / nch), ntile sqrt((c + 1) / nch)),
+        // about the same number of tile pairs each (tile kb pairs with jt = 0..kb).  It touches
+        // the slots of tiles 0..kb_hi-1 only: phases 2 and 4 stop there (the other slots' sums
+        // are zero).
+        const int kb_lo = chunk == 0 ? 0 : min(ntile, (int)(ntile * sqrtf((float)chunk / nch) + 0.5f));
+        const int kb_hi = chunk == nch - 1 ? ntile : min(ntile, (int)(ntile * sqrtf((float)(chunk + 1) / nch) + 0.5f));
+        const int np = min(nf, kb_hi * AW);
         __syncthreads();  // phase-1 slots were written by arbitrary lanes
 
         // ---- phase 2: records and each tile's bounding box (float, padding lanes excluded); zero
         // the accumulators.  Slot p is always touched by lane p % 64, here and below.
-        for (int t = 0; t * AW < nf; ++t) {
+        for (int t = 0; t < kb_hi; ++t) {
             const int p = t * AW + lane;
             float bb[6] = {3e38f, -3e38f, 3e38f, -3e38f, 3e38f, -3e38f};
             if (p < nf) {
@@ -289,7 +297,6 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
         // adds to one slot reach L2 in issue order, so the sums stay bitwise repeatable, and
         // they cost no read latency (a read-modify-write here took 7.6 % of the kernel on Cu).
         uint64_t cnt = 0;
-        const int ntile = (nf + AW - 1) / AW;
         // Pair (i, p) terms of a forward neighbour p from its offset, type and weights, the same
         // expressions for the k and the j side: r^2, 1/r^2, 1/r, sqrt C6_ip, damping term
         // (FP32: (8/3) log2(R0^2 / r^2), FP64: R0^2 / r^2).
@@ -303,10 +310,6 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
             const T r0 = P.r0[ti * ne + tp];
             lam = F32 ? T((8.0f / 3.0f) * log2f((float)(r0 * r0 * ia))) : r0 * r0 * ia;
         };
-        // this work item's k tiles: chunk c of nch takes [ntile sqrt(c / nch), ntile sqrt((c + 1) / nch)),
-        // about the same number of tile pairs each (tile kb pairs with jt = 0..kb)
-        const int kb_lo = chunk == 0 ? 0 : min(ntile, (int)(ntile * sqrtf((float)chunk / nch) + 0.5f));
-        const int kb_hi = chunk == nch - 1 ? ntile : min(ntile, (int)(ntile * sqrtf((float)(chunk + 1) / nch) + 0.5f));
         for (int kb = kb_lo; kb < kb_hi; ++kb) {
             const int k = kb * AW + lane;
             const bool kval = k < nf;
@@ -563,7 +566,7 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
         aG.init();
         for (int q = 0; q < 3; ++q) aF[q].init();
         for (int q = 0; q < 9; ++q) aW[q].init();
-        for (int p = lane; p < nf; p += AW) {
+        for (int p = lane; p < np; p += AW) {
             const int j = Lj[p];
             const double dx = (double)LT[F_DX * cap + p], dy = (double)LT[F_DY * cap + p],
                          dz = (double)LT[F_DZ * cap + p];
