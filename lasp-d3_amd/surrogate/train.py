@@ -117,8 +117,8 @@ class ForceLoss:
     d3x_sur_tangent; the gradient of weight * sum |F_net - F_exact|^2 / (3 n rms_kind^2) goes
     into grad through d3x_sur_train_force_grad, class by class."""
 
-    def __init__(self, args, frames, Xd, cls, lib, st, work, torch):
-        self.torch, self.lib, self.st, self.work = torch, lib, st, work
+    def __init__(self, args, frames, Xd, cls, lib, st, work, grad, torch):
+        self.torch, self.lib, self.st, self.work, self.grad = torch, lib, st, work, grad
         self.Xd, self.cls = Xd, cls
         self.weight = args.force_weight
         self.frames = [fr for fr in frames if fr["index"] % HOLDOUT != HOLDOUT - 1]
@@ -212,8 +212,7 @@ def train(args, X, y, Zall, fid, frames):
     work = torch.empty(lib.d3x_sur_train_bytes(nmax), dtype=torch.uint8, device=dev)
     fl = None
     if args.force_weight > 0:
-        fl = ForceLoss(args, frames, Xd, cls, lib, st, work, torch)
-        fl.grad = grad
+        fl = ForceLoss(args, frames, Xd, cls, lib, st, work, grad, torch)
         log("force rms per kind (eV/A):", json.dumps(fl.rms))
     E = torch.zeros(B, device=dev)
     loss = torch.zeros(1, device=dev)
