@@ -22,7 +22,7 @@ Writes the parameter file (d3x_sur_load_file) and a JSON report.  PyTorch provid
 and the stream only.
 
 usage: python lasp-d3_amd/surrogate/train.py [--si-frames 1000] [--water-frames 1000]
-       [--water-bases 25] [--steps 6000] [--force-weight 1.0] [--out FILE] [--report FILE]
+       [--water-bases 25] [--steps 20000] [--force-weight 1.0] [--out FILE] [--report FILE]
 """
 
 from __future__ import annotations
@@ -299,7 +299,7 @@ def main():
     ap.add_argument("--si-frames", type=int, default=1000)
     ap.add_argument("--water-frames", type=int, default=1000)
     ap.add_argument("--water-bases", type=int, default=25)
-    ap.add_argument("--steps", type=int, default=6000)
+    ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--batch", type=int, default=8192)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--seed", type=int, default=0)
