@@ -104,6 +104,11 @@ struct Params {
     // and triangle count atomically.  More items than centres fill the GPU on small cells.
     int atm_chunks;
     double *atm_eatc;
+    // Two-body waves per centre (1..8): with tb_parts > 1 wave w handles centre w / tb_parts over
+    // a contiguous share of its stencil bins and writes its sums to tb_part[(part n + centre) *
+    // TB_PART_W] (energy, dE/dCN, force, virial, pair count), added in part order afterwards.
+    int tb_parts;
+    double *tb_part;
     // three-body stand-in (d3_sur.hip): class of each element type, features [N][64], the
     // network's dE/dx [N][64] and energies (eV), class per sorted atom, fp32 parameters and
     // packed bf16 weights
@@ -116,6 +121,7 @@ struct Params {
 };
 
 constexpr int ATM_NF = 5;  // record fields per ATM neighbour (offset, CN log-derivatives)
+constexpr int TB_PART_W = 12;  // doubles per two-body part sum: E, dE/dCN, F[3], virial[6], pairs
 // doubles of ATM accumulator scratch per block: 6 per neighbour slot
 __host__ __device__ constexpr size_t atm_acc_stride(int cap) { return 6 * (size_t)cap; }
 // values of ATM record scratch per block (cap a multiple of 64): ATM_NF per neighbour slot, then

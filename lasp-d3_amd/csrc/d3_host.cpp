@@ -239,7 +239,7 @@ Engine::~Engine() {
                     &d_npair, &d_out_f, &d_out_e, &d_out_cn, &d_out_g, &d_bin_count, &d_bin_start, &d_cursor,
                     &d_sys, &d_st_d, &d_st_D, &d_chunk_b, &d_chunk_e, &d_chunk_s, &d_sys_chunk, &d_partial,
                     &d_out_sys, &d_tab, &d_status, &d_atm_i32, &d_atm_T, &d_atm_acc, &d_atm_pa, &d_type_cls,
-                    &d_feat, &d_sur_G, &d_sur_E, &d_sur_cls, &d_sur_P, &d_sur_PK, &d_tan_dir, &d_tan_U, &d_atm_roff, &d_atm_eatc};
+                    &d_feat, &d_sur_G, &d_sur_E, &d_sur_cls, &d_sur_P, &d_sur_PK, &d_tan_dir, &d_tan_U, &d_atm_roff, &d_atm_eatc, &d_tb_part};
     for (void **p : all) dfree(*p);
     for (auto &e : ev_)
         if (e) (void)hipEventDestroy(e);
@@ -504,6 +504,19 @@ void Engine::upload_geometry() {
     HIPCHK(hipStreamSynchronize(stream_));
 }
 
+void Engine::ensure_tb_scratch() {
+    // Two-body waves per centre: about 4096 waves in all for small cells (512 centres alone leave
+    // most SIMDs idle); parts x n stays below 8192 whenever parts > 1.
+    const uint64_t ncen = std::max<uint64_t>(n_, 1);
+    tb_parts_ = (int)std::min<uint64_t>(8, std::max<uint64_t>(1, (4096 + ncen - 1) / ncen));
+    const size_t need = (size_t)tb_parts_ * n_ * TB_PART_W;
+    if (tb_parts_ > 1 && need > tb_part_alloc_) {
+        dfree(d_tb_part);
+        tb_part_alloc_ = std::max<size_t>(need, (size_t)8192 * TB_PART_W);
+        d_tb_part = dalloc(tb_part_alloc_ * sizeof(double));
+    }
+}
+
 void Engine::ensure_atm_scratch() {
     if (rc3_ <= 0) return;
     // forward-neighbour capacity: 1.5 x the mean number of forward neighbours (half of all
@@ -603,6 +616,8 @@ Params<T> Engine::params() {
     P.atm_var = rc3_ > 0 ? atm_var() : 0;
     P.atm_chunks = atm_chunks_;
     P.atm_eatc = (double *)d_atm_eatc;
+    P.tb_parts = tb_parts_;
+    P.tb_part = (double *)d_tb_part;
     P.atm_roff = (const int *)d_atm_roff;
     P.atm_yrows = yrows_;
     P.atm_i32 = (int *)d_atm_i32;
@@ -659,6 +674,7 @@ void Engine::run() {
         tables_dirty_ = false;
     }
     ensure_atm_scratch();
+    ensure_tb_scratch();
     ensure_surrogate();
     Params<T> P = params<T>();
     feat_valid_ = mode_now() != D3X_ATM_EXACT;

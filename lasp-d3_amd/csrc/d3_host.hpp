@@ -120,6 +120,7 @@ class Engine {
     void upload_geometry();
     void alloc_atoms();
     void ensure_atm_scratch();
+    void ensure_tb_scratch();
     void ensure_surrogate();
     int type_of(uint16_t Z) const;
 
@@ -131,7 +132,7 @@ class Engine {
     std::vector<float> h_sur_P_;
     void *d_type_cls = nullptr, *d_feat = nullptr, *d_sur_G = nullptr, *d_sur_E = nullptr, *d_sur_cls = nullptr,
          *d_sur_P = nullptr, *d_sur_PK = nullptr, *d_tan_dir = nullptr, *d_tan_U = nullptr, *d_atm_roff = nullptr,
-         *d_atm_eatc = nullptr;
+         *d_atm_eatc = nullptr, *d_tb_part = nullptr;
 
     // configuration
     std::vector<uint16_t> elems_;  // sorted unique atomic numbers of the handle
@@ -196,6 +197,8 @@ class Engine {
     int atm_alloc_blocks_ = 0;      // blocks the scratch was allocated for (>= atm_blocks_)
     int atm_chunks_ = 1;            // three-body work items per centre (Params::atm_chunks)
     size_t atm_eatc_alloc_ = 0;     // doubles in d_atm_eatc (per-item centre energies)
+    int tb_parts_ = 1;              // two-body waves per centre (Params::tb_parts)
+    size_t tb_part_alloc_ = 0;      // doubles in d_tb_part
 
     bool tuning_explicit_ = false;  // set_tuning / autotune fixed the parameters
     std::string tune_cache_;        // cache consulted at geometry build (empty: none)

@@ -346,7 +346,7 @@ template <class T, int NR>
 __global__ __launch_bounds__(BLOCK) void k_2b(Params<T> P) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int wv = threadIdx.x >> 6, lane = lane_id();
-    const int i = blockIdx.x * WPB + wv;
+    const int w = blockIdx.x * WPB + wv, nparts = P.tb_parts, i = w / nparts, part = w - i * nparts;
     if (i >= P.n) return;
     const int ynr = P.ne * NR;
     unsigned char *base = smem + wv * twobody_wave_lds<T>(P.ne, NR);
@@ -390,7 +390,9 @@ __global__ __launch_bounds__(BLOCK) void k_2b(Params<T> P) {
     A.init();
     int qn = 0;
     float cnt = 0.0f;
-    const int s0 = C.S->st_off[CUT_2B], s1 = s0 + C.S->st_cnt[CUT_2B];
+    // this wave's share of the stencil (all of it with one part per centre)
+    const int sa = C.S->st_off[CUT_2B], sz = C.S->st_cnt[CUT_2B], per = (sz + nparts - 1) / nparts;
+    const int s0 = sa + min(sz, part * per), s1 = sa + min(sz, (part + 1) * per);
     for (int sb = s0; sb < s1; sb += WAVE) {
         const auto Bn = C.bins(P, sb, s1);
         const int nsb = s1 - sb < WAVE ? s1 - sb : WAVE;
@@ -458,12 +460,39 @@ __global__ __launch_bounds__(BLOCK) void k_2b(Params<T> P) {
     cnt = wave_sumf(cnt);
     if (lane == 0) {
         const double iv = C.S->inv_vol;
-        P.e[i] = E;
-        P.g[i] = G;
-        for (int k = 0; k < 3; ++k) P.f[3 * i + k] = F[k];
-        for (int k = 0; k < 6; ++k) P.vir[6 * i + k] = V[k] * iv;
-        P.npair[3 * i + 0] = cnt;
+        if (nparts == 1) {
+            P.e[i] = E;
+            P.g[i] = G;
+            for (int k = 0; k < 3; ++k) P.f[3 * i + k] = F[k];
+            for (int k = 0; k < 6; ++k) P.vir[6 * i + k] = V[k] * iv;
+            P.npair[3 * i + 0] = cnt;
+        } else {
+            double *o = P.tb_part + ((size_t)part * P.n + i) * TB_PART_W;
+            o[0] = E;
+            o[1] = G;
+            for (int k = 0; k < 3; ++k) o[2 + k] = F[k];
+            for (int k = 0; k < 6; ++k) o[5 + k] = V[k] * iv;
+            o[11] = cnt;
+        }
     }
+}
+
+// Per-centre two-body sums from the parts, in part order (bitwise reproducible).
+template <class T>
+__global__ void k_2b_parts(Params<T> P) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    double acc[TB_PART_W];
+    for (int k = 0; k < TB_PART_W; ++k) acc[k] = 0.0;
+    for (int c = 0; c < P.tb_parts; ++c) {
+        const double *o = P.tb_part + ((size_t)c * P.n + i) * TB_PART_W;
+        for (int k = 0; k < TB_PART_W; ++k) acc[k] += o[k];
+    }
+    P.e[i] = acc[0];
+    P.g[i] = acc[1];
+    for (int k = 0; k < 3; ++k) P.f[3 * i + k] = acc[2 + k];
+    for (int k = 0; k < 6; ++k) P.vir[6 * i + k] = acc[5 + k];
+    P.npair[3 * i + 0] = (float)acc[11];
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -561,7 +590,9 @@ void Launch<T>::twobody(const Params<T> &P, hipStream_t s) {
     if (!fn) return;  // not reached: a handle's reference count is 1..MAX_NREF
     Params<T> arg = P;
     void *args[] = {&arg};
-    (void)hipLaunchKernel(fn, dim3((P.n + WPB - 1) / WPB), dim3(BLOCK), args, lds, s);
+    const size_t waves = (size_t)P.n * P.tb_parts;
+    (void)hipLaunchKernel(fn, dim3((unsigned)((waves + WPB - 1) / WPB)), dim3(BLOCK), args, lds, s);
+    if (P.tb_parts > 1) hipLaunchKernelGGL(k_2b_parts<T>, dim3((P.n + 255) / 256), dim3(256), 0, s, P);
 }
 
 template <class T>
