@@ -118,3 +118,15 @@ def test_every_three_body_variant(tmp_path):
                 assert np.max(np.abs(f - ref[1])) <= (1e-9 if fp64 else 2e-5 * np.max(np.abs(ref[1]))), (prec, v)
     finally:
         del os.environ["D3_TUNE_CACHE"]
+
+
+def test_default_bin_width_follows_cn_cutoff():
+    """Bins half the CN cutoff wide by default, also when the two-body cutoff is smaller (a
+    1-bohr two-body cutoff once shrank them to 4 bohr, and the CN, chain-rule and feature
+    passes then walked thousands of near-empty bins per centre)."""
+    s = systems.si64(seed=3)
+    for rc2, rccn in ((94.87, 40.0), (1.0, 40.0), (50.0, 30.0)):
+        h = d3_lib.Handle(s["Z"], len(s["Z"]), cutoff=rc2, cn_cutoff=rccn, atm_cutoff=0.0, damping=1, functional=0)
+        w, _ = h.tuning()
+        h.close()
+        assert w == pytest.approx(0.5 * rccn), (rc2, rccn, w)
