@@ -75,7 +75,8 @@ int d3x_phase_times(d3_handle_t *handle, double ms[6]);
  * reference-compatible init functions. */
 int d3x_set_graph(d3_handle_t *handle, int enable);
 
-/* Tuning parameters: bin width of the cell lists (bohr; default half the CN cutoff) and the
+/* Tuning parameters: bin width of the cell lists (bohr; default half the CN cutoff, 3/8 of it on
+ * cells less than 6 CN cutoffs across) and the
  * number of persistent blocks of the three-body kernel (default: as many as the device holds at
  * once).  Values <= 0 keep the current setting.  d3x_autotune times full evaluations of the
  * handle's current input over a small candidate set -- bin widths, the three-body kernel's

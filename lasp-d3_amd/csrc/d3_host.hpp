@@ -121,6 +121,7 @@ class Engine {
     void alloc_atoms();
     void ensure_atm_scratch();
     void ensure_tb_scratch();
+    double default_bin_width() const;
     void ensure_surrogate();
     int type_of(uint16_t Z) const;
 

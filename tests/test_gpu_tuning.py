@@ -121,12 +121,18 @@ def test_every_three_body_variant(tmp_path):
 
 
 def test_default_bin_width_follows_cn_cutoff():
-    """Bins half the CN cutoff wide by default, also when the two-body cutoff is smaller (a
+    """Default bins follow the CN cutoff alone, also when the two-body cutoff is smaller (a
     1-bohr two-body cutoff once shrank them to 4 bohr, and the CN, chain-rule and feature
-    passes then walked thousands of near-empty bins per centre)."""
+    passes then walked thousands of near-empty bins per centre): half of it before any cell is
+    known and on cells at least 6 CN cutoffs across, 3/8 of it on smaller cells such as this one."""
     s = systems.si64(seed=3)
     for rc2, rccn in ((94.87, 40.0), (1.0, 40.0), (50.0, 30.0)):
         h = d3_lib.Handle(s["Z"], len(s["Z"]), cutoff=rc2, cn_cutoff=rccn, atm_cutoff=0.0, damping=1, functional=0)
         w, _ = h.tuning()
-        h.close()
         assert w == pytest.approx(0.5 * rccn), (rc2, rccn, w)
+        h.set_atoms(s["pos"], s["Z"])
+        h.set_cell(s["cell"])
+        h.compute()
+        w, _ = h.tuning()
+        h.close()
+        assert w == pytest.approx(0.375 * rccn), (rc2, rccn, w)
