@@ -686,9 +686,11 @@ size_t atm_lds_bytes(int ne, int nref, int yrows, size_t ts, int var) {
 
 int atm_choose_variant(int ne, int nref, int yrows, size_t ts) {
     // The first of KL + DR, DR, KL, neither (most work saved first: DR per pair, KL per tile)
-    // that keeps at least 90 % of the most resident blocks any of them reaches.  Measured: fcc Cu
-    // (FP32, 3 references) runs KL + DR at 18 blocks per CU as fast as DR at 20, with a third less
-    // HBM traffic; the six-element batch loses 8 % with KL at 7 blocks instead of 8.
+    // that keeps at least 90 % of the most resident blocks any of them reaches; at 10 blocks per
+    // CU or fewer (2.5 waves per SIMD) the first that keeps all of them.  Measured: fcc Cu (FP32,
+    // 3 references) runs KL + DR at 18 blocks per CU as fast as DR at 20, with a third less HBM
+    // traffic; the six-element batch runs neither at 10 blocks 3 % faster than DR at 9 and 22 %
+    // faster than KL at 7.
     const int order[4] = {ATM_KL | ATM_DR, ATM_DR, ATM_KL, 0};
     int blocks[4], most = 0;
     for (int k = 0; k < 4; ++k) {
@@ -696,8 +698,9 @@ int atm_choose_variant(int ne, int nref, int yrows, size_t ts) {
                                                                          : blocks_per_cu(ne, nref, yrows, ts, order[k]);
         most = blocks[k] > most ? blocks[k] : most;
     }
+    const bool low = most <= 10;
     for (int k = 0; k < 4; ++k)
-        if (blocks[k] > 0 && 10 * blocks[k] >= 9 * most) return order[k];
+        if (blocks[k] > 0 && (low ? blocks[k] == most : 10 * blocks[k] >= 9 * most)) return order[k];
     return 0;
 }
 
