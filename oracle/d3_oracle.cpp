@@ -11,7 +11,9 @@
 // has a bug, the oracle follows the physics (decisions listed per function below).
 //
 // Parity pins (tests/test_oracle.py): closed forms (dimer energy/force/stress, isolated-pair CN,
-// single-reference C6, BJ r->0 limit, equilateral ATM angle factor), central finite differences
+// single-reference C6, BJ r->0 limit, equilateral ATM angle factor, and mixed-element clusters
+// whose atoms have different CNs: per-pair 25-term C6 sums, dE/dCN, a heteronuclear ATM
+// triangle's energy and dE/dCN), central finite differences
 // of forces and of stress under homogeneous strain, translation/permutation/supercell
 // invariance, sum(F) = 0, and the reference's seven property tests (test/d3_test.cpp:73-341).
 //

@@ -3,7 +3,10 @@ a dropped term, a wrong sign or index, or a transposed operand fails at least on
 
 Closed forms use single-reference elements (C: Z=6, B: Z=5 in the synthetic table), for which the
 C6 interpolation returns the reference C6 exactly and dC6/dCN = 0, and a box so large that only
-the direct pair lies within the cutoffs.
+the direct pair lies within the cutoffs.  The mixed-element closed forms (an O-H-H cluster and an
+O-N-H triangle, multi-reference elements with three different CNs) write out every CN, the
+25-term C6 sums and their CN derivatives, so that an interpolation pairing an atom's CN with the
+other element's references, or a three-body C6_jk taken at swapped CNs, fails.
 """
 
 import numpy as np
@@ -138,6 +141,149 @@ def test_atm_closed_form(table, geom):
     f = 1 / (1 + 6 * (4.0 / 3.0 * R0 / np.prod(edges) ** (1 / 3)) ** 16)
     assert dE == pytest.approx(-HARTREE * C9 * A * f, rel=1e-12)
     assert o["ntri"] == 3  # one triangle, counted once per vertex
+
+
+# Mixed-element closed forms.  Every quantity below is written out from the definitions in
+# src/d3_kernel.cu (CN :544-574, C6 :709-800, two-body :902-986, ATM :1199-1509) with each
+# element's CN references read from its homonuclear diagonal entry c6ab[Z][Z][a][a][1], so a
+# C6 interpolation that pairs CN_A with element B's references (or takes C6_jk at swapped CNs)
+# fails: the clusters are shaped so that every atom has a different CN.
+
+def _cn_refs(table, Z):
+    """CN references of element Z (one per valid reference) from the homonuclear entries."""
+    c = table["c6ab"]
+    return [(a, float(c[Z, Z, a, a, 1])) for a in range(table["nref"]) if c[Z, Z, a, a, 1] > -1]
+
+
+def _pair_cn(table, Zi, Zj, r_bohr):
+    R = float(table["rcov"][Zi]) + float(table["rcov"][Zj])
+    return 1.0 / (1.0 + np.exp(-16.0 * (R / r_bohr - 1.0)))
+
+
+def _c6_closed(table, Zi, Zj, cni, cnj):
+    """C6(CN_i, CN_j) = sum_ab C6ref_ab L_ab / sum_ab L_ab with
+    L_ab = exp(-4 (CN_i - CNref_i,a)^2 - 4 (CN_j - CNref_j,b)^2), and dC6/dCN_i, dC6/dCN_j."""
+    num = den = dnum_i = dden_i = dnum_j = dden_j = 0.0
+    for a, ra in _cn_refs(table, Zi):
+        for b, rb in _cn_refs(table, Zj):
+            L = np.exp(-4.0 * (cni - ra) ** 2 - 4.0 * (cnj - rb) ** 2)
+            cref = float(table["c6ab"][Zi, Zj, a, b, 0])
+            num += cref * L
+            den += L
+            dnum_i += cref * (-8.0 * (cni - ra)) * L
+            dden_i += -8.0 * (cni - ra) * L
+            dnum_j += cref * (-8.0 * (cnj - rb)) * L
+            dden_j += -8.0 * (cnj - rb) * L
+    c6 = num / den
+    return c6, (dnum_i - c6 * dden_i) / den, (dnum_j - c6 * dden_j) / den
+
+
+def test_mixed_element_table_invariants(table):
+    """The entries [1] and [2] of c6ab[A][B][a][b] are CNref_A,a and CNref_B,b, and C6ref is
+    symmetric under (A, a) <-> (B, b): the layout the closed forms below rely on."""
+    c = table["c6ab"]
+    for A, B in [(8, 1), (1, 8), (7, 8), (1, 7)]:
+        ra, rb = dict(_cn_refs(table, A)), dict(_cn_refs(table, B))
+        for a in ra:
+            for b in rb:
+                assert c[A, B, a, b, 1] == ra[a] and c[A, B, a, b, 2] == rb[b]
+                assert c[A, B, a, b, 0] == c[B, A, b, a, 0]
+
+
+@pytest.mark.parametrize("damping", [0, 1])
+def test_mixed_element_cluster_closed_form(table, damping):
+    """Isolated O-H-H cluster (two multi-reference elements, CN_O != CN_H1 != CN_H2): CN, C6 of
+    each pair by the 25-term sum, energy, per-atom energy and dE/dCN written out."""
+    pos = np.array([[0.0, 0.0, 0.0], [1.60, 0.0, 0.0], [-0.55, 1.78, 0.30]])
+    Z = [8, 1, 1]
+    P, cell = iso(pos)
+    o = run(P, Z, cell, table, cutoff=40.0, cn_cutoff=40.0, damping=damping, functional=0)
+    s6, s8z, s8b, sr6, sr8, a1, a2 = PBE
+    n = 3
+    rb = {(i, j): np.linalg.norm(pos[i] - pos[j]) / BOHR for i in range(n) for j in range(n) if i != j}
+    cn = np.array([sum(_pair_cn(table, Z[i], Z[j], rb[i, j]) for j in range(n) if j != i) for i in range(n)])
+    np.testing.assert_allclose(o["cn"], cn, rtol=1e-12)
+    assert min(abs(cn[0] - cn[1]), abs(cn[0] - cn[2]), abs(cn[1] - cn[2])) > 0.1
+    E = 0.0
+    ea = np.zeros(n)
+    g = np.zeros(n)
+    for i in range(n):
+        for j in range(i + 1, n):
+            r = rb[i, j]
+            c6, dc6i, dc6j = _c6_closed(table, Z[i], Z[j], cn[i], cn[j])
+            q = float(table["r2r4"][Z[i]]) * float(table["r2r4"][Z[j]])
+            if damping == 0:
+                R0 = float(table["r0ab"][Z[i], Z[j]])
+                f6 = 1 / (1 + 6 * (sr6 * R0 / r) ** 14)
+                f8 = 1 / (1 + 6 * (sr8 * R0 / r) ** 16)
+                s8 = s8z
+            else:
+                a = a1 * np.sqrt(3 * q) + a2
+                f6 = r ** 6 / (r ** 6 + a ** 6)
+                f8 = r ** 8 / (r ** 8 + a ** 8)
+                s8 = s8b
+            t = s6 * f6 / r ** 6 + s8 * 3 * q * f8 / r ** 8  # E+ of the pair per unit C6
+            E += c6 * t
+            ea[i] += 0.5 * c6 * t
+            ea[j] += 0.5 * c6 * t
+            g[i] += dc6i * t
+            g[j] += dc6j * t
+    assert o["energy"] == pytest.approx(-HARTREE * E, rel=1e-13)
+    np.testing.assert_allclose(o["atom_energy"], -HARTREE * ea, rtol=1e-13)
+    np.testing.assert_allclose(o["dedcn"], g, rtol=1e-11, atol=1e-16)
+
+
+def test_mixed_element_c6_derivative_closed_form_by_differences(table):
+    """The closed form's dC6/dCN against central differences of its own C6 (so the dE/dCN pin
+    above does not rest on an unchecked derivative)."""
+    for Zi, Zj, ci, cj in [(8, 1, 0.83, 0.41), (7, 8, 1.9, 2.7), (1, 7, 0.2, 1.3)]:
+        c6, di, dj = _c6_closed(table, Zi, Zj, ci, cj)
+        h = 1e-6
+        fi = (_c6_closed(table, Zi, Zj, ci + h, cj)[0] - _c6_closed(table, Zi, Zj, ci - h, cj)[0]) / (2 * h)
+        fj = (_c6_closed(table, Zi, Zj, ci, cj + h)[0] - _c6_closed(table, Zi, Zj, ci, cj - h)[0]) / (2 * h)
+        assert di == pytest.approx(fi, rel=1e-7, abs=1e-9 * abs(c6))
+        assert dj == pytest.approx(fj, rel=1e-7, abs=1e-9 * abs(c6))
+
+
+def test_mixed_element_atm_closed_form(table):
+    """Isolated O-N-H triangle (three multi-reference elements, three different CNs): the ATM
+    energy, its per-atom thirds and its dE/dCN from C9 = -sqrt(C6_ab C6_ac C6_bc) at the CNs,
+    the angle factor from the triangle's cosines and the damping from r0ab."""
+    pos = np.array([[0.0, 0.0, 0.0], [2.05, 0.0, 0.0], [-0.5, 1.2, 0.3]])
+    Z = [8, 7, 1]
+    P, cell = iso(pos)
+    kw = dict(cutoff=40.0, cn_cutoff=40.0, damping=1, functional=0)
+    with_atm = run(P, Z, cell, table, atm_cutoff=40.0, **kw)
+    without = run(P, Z, cell, table, **kw)
+    n = 3
+    r = {(i, j): np.linalg.norm(pos[i] - pos[j]) / BOHR for i in range(n) for j in range(n) if i != j}
+    cn = np.array([sum(_pair_cn(table, Z[i], Z[j], r[i, j]) for j in range(n) if j != i) for i in range(n)])
+    np.testing.assert_allclose(with_atm["cn"], cn, rtol=1e-12)
+    assert min(abs(cn[0] - cn[1]), abs(cn[0] - cn[2]), abs(cn[1] - cn[2])) > 0.1
+    pairs = [(0, 1), (0, 2), (1, 2)]
+    c6 = {}
+    for i, j in pairs:
+        c, di, dj = _c6_closed(table, Z[i], Z[j], cn[i], cn[j])
+        c6[i, j] = (c, {i: di, j: dj})
+    C9 = -np.sqrt(c6[0, 1][0] * c6[0, 2][0] * c6[1, 2][0])
+    # cosines of the three interior angles from the position vectors
+    cos = []
+    for v in range(3):
+        u, w = [pos[x] - pos[v] for x in range(3) if x != v]
+        cos.append(np.dot(u, w) / (np.linalg.norm(u) * np.linalg.norm(w)))
+    rp = r[0, 1] * r[0, 2] * r[1, 2]
+    A = (1 + 3 * cos[0] * cos[1] * cos[2]) / rp ** 3
+    R0 = (float(table["r0ab"][Z[0], Z[1]]) * float(table["r0ab"][Z[0], Z[2]]) * float(table["r0ab"][Z[1], Z[2]])) ** (1 / 3)
+    f = 1 / (1 + 6 * (4.0 / 3.0 * R0 / rp ** (1 / 3)) ** 16)
+    e = C9 * A * f  # E+ of the triangle
+    assert with_atm["energy"] - without["energy"] == pytest.approx(-HARTREE * e, rel=1e-11)
+    np.testing.assert_allclose(with_atm["atom_energy"] - without["atom_energy"], [-HARTREE * e / 3] * 3, rtol=1e-10)
+    # dE+/dCN_x = (C9 / 2) sum over the two edges at x of (dC6_xy/dCN_x) / C6_xy, times A f
+    gx = np.zeros(n)
+    for (i, j), (c, d) in c6.items():
+        for x in (i, j):
+            gx[x] += 0.5 * C9 * d[x] / c * A * f
+    np.testing.assert_allclose(with_atm["dedcn"] - without["dedcn"], gx, rtol=1e-9, atol=1e-18)
 
 
 # ---------------------------------------------------------------------- finite differences
