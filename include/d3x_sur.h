@@ -30,8 +30,9 @@ extern "C" {
 
 /* Three-body mode of a handle.  EXACT: the ATM kernel (default).  SURROGATE: the network; every
  * judge_every-th evaluation (and the first after atoms or cell change) also runs the exact kernel
- * and, when |E_net - E_exact| / atoms > tol_ev_per_atom, returns the exact result and keeps the
- * handle on the exact kernel from then on (the fallback; d3x_sur_state reports it).
+ * and, when |E_net - E_exact| / atoms > tol_ev_per_atom (or the force criterion of
+ * d3x_sur_set_force_tol fails), returns the exact result and keeps the handle on the exact kernel
+ * from then on (the fallback; d3x_sur_state reports it).
  * judge_every = 0 judges only the first evaluation after a geometry change.
  * EXACT_FEATURES: exact kernel, and the network's features are computed too (d3x_sur_features),
  * for building training sets. */
@@ -47,6 +48,12 @@ int d3x_set_atm_mode(d3_handle_t *handle, int mode, int judge_every, double tol_
 /* mode in use (after fallbacks), fallback count, |E_net - E_exact| / atoms of the last judged
  * evaluation (eV; negative before any judgement).  Any pointer may be NULL. */
 int d3x_sur_state(d3_handle_t *handle, int *mode, int *fallbacks, double *last_dev_ev_per_atom);
+/* Force criterion of the judge: a judged evaluation also falls back when, in any cell, the RMS
+ * over the cell's force components of F_net - F_exact exceeds tol_rms_ev_per_A (eV/A; default
+ * 5e-3; <= 0 judges energies only).  d3x_sur_last_force_dev: that RMS deviation of the last judged
+ * evaluation (largest over cells; negative before any judgement or for a NULL handle). */
+int d3x_sur_set_force_tol(d3_handle_t *handle, double tol_rms_ev_per_A);
+double d3x_sur_last_force_dev(d3_handle_t *handle);
 /* Features of the last evaluation (SURROGATE or EXACT_FEATURES), caller order, atoms x 64. */
 int d3x_sur_features(d3_handle_t *handle, float *features);
 /* Push-forward of a direction field through the features, for force matching: U[i] =

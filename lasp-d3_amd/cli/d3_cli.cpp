@@ -5,9 +5,9 @@
 // compute_dispersion_energy_from_handle, and prints the reference's report lines (structure
 // count, cell, elapsed time, energy, per-atom forces, force sum, stress).
 //
-// Usage: d3_cpp [--use-atm] [--bj] [--fp32] [--cutoff R] [--cn-cutoff R] [--atm-cutoff R] FILE...
+// Usage: d3_cpp [--use-atm] [--bj] [--fp32 | --fp64] [--cutoff R] [--cn-cutoff R] [--atm-cutoff R] FILE...
 // Defaults as in the reference's driver: cutoff 60 bohr, CN and ATM cutoffs 40 bohr, zero
-// damping, PBE, FP64 arithmetic unless --fp32 (or D3_PRECISION=fp32).
+// damping, PBE, the library's default FP32 arithmetic unless --fp64 (or D3_PRECISION=fp64).
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -80,8 +80,11 @@ bool read_arc(const std::string &path, std::vector<Structure> &out) {
         try {
             if (t.size() == 10 && atomic_number(t[0]) > 0) {
                 if (open) {
+                    // parse all three coordinates before storing anything, so that a bad number
+                    // leaves the structure's Z and pos the same length
+                    const float x = std::stof(t[1]), y = std::stof(t[2]), z = std::stof(t[3]);
                     out.back().Z.push_back((uint16_t)atomic_number(t[0]));
-                    for (int k = 1; k <= 3; ++k) out.back().pos.push_back(std::stof(t[k]));
+                    out.back().pos.insert(out.back().pos.end(), {x, y, z});
                 }
                 continue;
             }
@@ -162,8 +165,15 @@ bool read_extxyz(const std::string &path, std::vector<Structure> &out) {
                 std::cerr << "Element not found: " << t[col_species] << std::endl;
                 return false;
             }
+            float x[3];
+            try {
+                for (int k = 0; k < 3; ++k) x[k] = std::stof(t[col_pos + k]);
+            } catch (const std::exception &) {
+                std::cerr << "Invalid line: " << line << std::endl;
+                return false;
+            }
             s.Z.push_back((uint16_t)z);
-            for (int k = 0; k < 3; ++k) s.pos.push_back(std::stof(t[col_pos + k]));
+            s.pos.insert(s.pos.end(), x, x + 3);
         }
         out.push_back(std::move(s));
     }
@@ -181,7 +191,8 @@ void usage() {
     std::cout << "  -h, --help        Display this help message" << std::endl;
     std::cout << "  --use-atm         Include the ATM three-body term" << std::endl;
     std::cout << "  --bj              Becke-Johnson damping (default: zero damping)" << std::endl;
-    std::cout << "  --fp32            FP32 arithmetic (default FP64)" << std::endl;
+    std::cout << "  --fp32            FP32 arithmetic (default)" << std::endl;
+    std::cout << "  --fp64            FP64 arithmetic" << std::endl;
     std::cout << "  --cutoff R        two-body cutoff in bohr (default 60)" << std::endl;
     std::cout << "  --cn-cutoff R     CN cutoff in bohr (default 40)" << std::endl;
     std::cout << "  --atm-cutoff R    ATM cutoff in bohr (default 40)" << std::endl;
@@ -208,6 +219,8 @@ int main(int argc, char **argv) {
             bj = true;
         } else if (a == "--fp32") {
             setenv("D3_PRECISION", "fp32", 1);
+        } else if (a == "--fp64") {
+            setenv("D3_PRECISION", "fp64", 1);
         } else if ((a == "--cutoff" || a == "--cn-cutoff" || a == "--atm-cutoff") && i + 1 < argc) {
             const float v = std::strtof(argv[++i], nullptr);
             (a == "--cutoff" ? rc : a == "--cn-cutoff" ? rccn : rc3) = v;

@@ -2,6 +2,7 @@
 // extensions, on top of d3::Engine.  Errors never escape as C++ exceptions: they are printed to
 // stderr (as the reference does), recorded for d3x_last_error / d3x_last_status, and turned into
 // NULL / non-zero returns.
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -27,11 +28,12 @@ struct Handle {
 
 Handle *H(d3_handle_t *h) { return static_cast<Handle *>(h); }
 
-// Default arithmetic of handles made through the reference-compatible init functions:
-// FP64 unless D3_PRECISION=fp32 is set in the environment.
+// Default arithmetic of handles made through the reference-compatible init functions: FP32
+// (pair and triplet math in float with compensated sums, as the reference computes in float,
+// src/d3_types.h:7) unless D3_PRECISION=fp64 is set in the environment.
 int default_precision() {
     const char *p = std::getenv("D3_PRECISION");
-    return (p && (std::strcmp(p, "fp32") == 0 || std::strcmp(p, "FP32") == 0)) ? D3X_FP32 : D3X_FP64;
+    return (p && (std::strcmp(p, "fp64") == 0 || std::strcmp(p, "FP64") == 0)) ? D3X_FP64 : D3X_FP32;
 }
 
 void report(const char *where, const std::string &msg) {
@@ -62,9 +64,10 @@ uint32_t guarded(Handle *h, const char *where, F &&fn) {
 void evaluate_once(Engine &e, double *energies, double *forces, double *stresses);
 
 // Full evaluation with results copied to the host.  With the three-body stand-in active, a
-// judged evaluation runs the network and the exact kernel: on agreement (per-cell |dE| / atoms
-// within the tolerance) the network's results are returned, otherwise the exact ones, and the
-// handle stays on the exact kernel.
+// judged evaluation runs the exact kernel and then the network: on agreement (per cell, |dE| /
+// atoms and the RMS force deviation within their tolerances) the network's results are returned,
+// otherwise the exact kernel runs once more (so the device holds its per-atom data), its results
+// are returned, and the handle stays on the exact kernel.
 void evaluate(Engine &e, double *energies, double *forces, double *stresses) {
     if (e.mode_now() != D3X_ATM_SURROGATE) {
         evaluate_once(e, energies, forces, stresses);
@@ -76,21 +79,23 @@ void evaluate(Engine &e, double *energies, double *forces, double *stresses) {
         evaluate_once(e, energies, forces, stresses);
         return;
     }
-    const size_t ns = (size_t)e.num_sys();
-    std::vector<double> e_net(ns), e_exact(ns);
-    evaluate_once(e, e_net.data(), nullptr, nullptr);
+    const size_t ns = (size_t)e.num_sys(), n3 = 3 * (size_t)e.num_atoms();
+    std::vector<double> e_net(ns), e_exact(ns), f_net(n3), f_exact(n3), s_net(9 * ns);
     e.set_force_exact(true);
     try {
-        evaluate_once(e, e_exact.data(), forces, stresses);
+        evaluate_once(e, e_exact.data(), f_exact.data(), nullptr);
     } catch (...) {
         e.set_force_exact(false);
         throw;
     }
     e.set_force_exact(false);
-    if (e.record_judgement(e_net.data(), e_exact.data())) {
-        evaluate_once(e, energies, forces, stresses);  // the network's results
-    } else if (energies) {
-        std::copy(e_exact.begin(), e_exact.end(), energies);  // fallen back: exact results stand
+    evaluate_once(e, e_net.data(), f_net.data(), s_net.data());
+    if (e.record_judgement(e_net.data(), e_exact.data(), f_net.data(), f_exact.data())) {
+        if (energies) std::copy(e_net.begin(), e_net.end(), energies);
+        if (forces) std::copy(f_net.begin(), f_net.end(), forces);
+        if (stresses) std::copy(s_net.begin(), s_net.end(), stresses);
+    } else {
+        evaluate_once(e, energies, forces, stresses);  // fallen back: the exact kernel's results
     }
 }
 
@@ -203,6 +208,7 @@ uint16_t compute_dispersion_energy_from_handle_status(d3_handle_t *handle, real_
     Handle *h = H(handle);
     uint32_t st = guarded(h, "compute_dispersion_energy_from_handle", [&] {
         Engine &e = *h->eng;
+        if (e.num_sys() != 1) throw Error{D3X_ERR_GENERIC | D3X_ERR_STATE, "handle holds a batch; call set_atoms first"};
         const uint64_t n = e.num_atoms();
         std::vector<double> f(3 * n);
         double E, S[9];
@@ -369,6 +375,16 @@ int d3x_sur_state(d3_handle_t *handle, int *mode, int *fallbacks, double *last_d
     if (last_dev_ev_per_atom) *last_dev_ev_per_atom = e.last_deviation();
     return 0;
 }
+
+int d3x_sur_set_force_tol(d3_handle_t *handle, double tol_rms_ev_per_A) {
+    if (!handle) return (int)(D3X_ERR_GENERIC | D3X_ERR_INPUT);
+    return (int)guarded(H(handle), "d3x_sur_set_force_tol", [&] {
+        if (!std::isfinite(tol_rms_ev_per_A)) throw Error{D3X_ERR_GENERIC | D3X_ERR_INPUT, "tolerance must be finite"};
+        H(handle)->eng->set_judge_force_tol(tol_rms_ev_per_A);
+    });
+}
+
+double d3x_sur_last_force_dev(d3_handle_t *handle) { return handle ? H(handle)->eng->last_force_deviation() : -1.0; }
 
 int d3x_sur_features(d3_handle_t *handle, float *features) {
     if (!handle || !features) return (int)(D3X_ERR_GENERIC | D3X_ERR_INPUT);

@@ -331,7 +331,13 @@ void Engine::set_cell(const double cell_A[9]) {
         H[k] = cell_A[k] / BOHR_A;
     }
     if (!(std::fabs(det3(H)) > 1e-12)) fail(D3X_ERR_INPUT, "singular cell");
-    if (sys_.size() != 1) sys_.assign(1, Cell{});
+    if (sys_.size() != 1) {
+        // after a batch the device holds the batch's atoms and their cell ids: one cell over
+        // them would be wrong, so the atoms must be set again
+        sys_.assign(1, Cell{});
+        n_ = 0;
+        atoms_set_ = false;
+    }
     std::memcpy(sys_[0].H, H, sizeof(H));
     sys_[0].set = true;
     sys_[0].n = n_;
@@ -983,12 +989,23 @@ bool Engine::judge_due() const {
     return mode_now() == D3X_ATM_SURROGATE && (sur_evals_ == 0 || (judge_every_ > 0 && sur_evals_ % judge_every_ == 0));
 }
 
-bool Engine::record_judgement(const double *e_net, const double *e_exact) {
-    double dev = 0;
-    for (size_t s = 0; s < sys_.size(); ++s)
-        if (sys_[s].n) dev = std::max(dev, std::fabs(e_net[s] - e_exact[s]) / (double)sys_[s].n);
+bool Engine::record_judgement(const double *e_net, const double *e_exact, const double *f_net,
+                              const double *f_exact) {
+    double dev = 0, fdev = 0;
+    size_t a = 0;
+    for (size_t s = 0; s < sys_.size(); ++s) {
+        const size_t n = sys_[s].n;
+        if (n) dev = std::max(dev, std::fabs(e_net[s] - e_exact[s]) / (double)n);
+        if (n && f_net && f_exact) {
+            double ss = 0;
+            for (size_t k = 3 * a; k < 3 * (a + n); ++k) ss += (f_net[k] - f_exact[k]) * (f_net[k] - f_exact[k]);
+            fdev = std::max(fdev, std::sqrt(ss / (3.0 * (double)n)));
+        }
+        a += n;
+    }
     last_dev_ = dev;
-    if (dev > judge_tol_) {
+    last_fdev_ = (f_net && f_exact) ? fdev : -1.0;
+    if (dev > judge_tol_ || (judge_ftol_ > 0 && f_net && f_exact && fdev > judge_ftol_)) {
         fallen_back_ = true;
         ++fallbacks_;
         return false;

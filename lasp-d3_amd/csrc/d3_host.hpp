@@ -96,7 +96,12 @@ class Engine {
     int mode_now() const { return (atm_mode_ == 1 && (fallen_back_ || force_exact_)) ? 0 : atm_mode_; }
     bool judge_due() const;
     void set_force_exact(bool on) { force_exact_ = on; }
-    bool record_judgement(const double *e_net, const double *e_exact);
+    // per cell: |E_net - E_exact| / atoms against the energy tolerance and, when forces are given
+    // (concatenated in cell order), the RMS over the cell's force components of F_net - F_exact
+    // against the force tolerance (<= 0: not judged)
+    bool record_judgement(const double *e_net, const double *e_exact, const double *f_net, const double *f_exact);
+    void set_judge_force_tol(double tol) { judge_ftol_ = tol; }
+    double last_force_deviation() const { return last_fdev_; }
     void count_surrogate_eval() { ++sur_evals_; }
     int fallbacks() const { return fallbacks_; }
     double last_deviation() const { return last_dev_; }
@@ -127,7 +132,7 @@ class Engine {
 
     // three-body stand-in state
     int atm_mode_ = 0, judge_every_ = 0, fallbacks_ = 0;
-    double judge_tol_ = 1e-3, last_dev_ = -1.0;
+    double judge_tol_ = 1e-3, last_dev_ = -1.0, judge_ftol_ = 5e-3, last_fdev_ = -1.0;
     bool fallen_back_ = false, force_exact_ = false, sur_dirty_ = false, feat_valid_ = false;
     uint64_t sur_evals_ = 0;
     std::vector<float> h_sur_P_;

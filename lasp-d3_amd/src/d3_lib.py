@@ -83,6 +83,8 @@ SIGNATURES = {
     "d3x_sur_load_file": (C.c_int, [_vp, C.c_char_p]),
     "d3x_set_atm_mode": (C.c_int, [_vp, C.c_int, C.c_int, C.c_double]),
     "d3x_sur_state": (C.c_int, [_vp, _ip, _ip, _dp]),
+    "d3x_sur_set_force_tol": (C.c_int, [_vp, C.c_double]),
+    "d3x_sur_last_force_dev": (C.c_double, [_vp]),
     "d3x_sur_features": (C.c_int, [_vp, _fp]),
     "d3x_sur_tangent": (C.c_int, [_vp, _dp, _fp]),
     "d3x_sur_packed_bytes": (C.c_size_t, []),
@@ -298,6 +300,14 @@ class Handle:
         m, f, d = C.c_int(), C.c_int(), C.c_double()
         self._check(self.lib.d3x_sur_state(self.h, C.byref(m), C.byref(f), C.byref(d)), "d3x_sur_state")
         return m.value, f.value, d.value
+
+    def set_judge_force_tol(self, tol_rms_ev_per_A: float):
+        """Force criterion of the judge (RMS of F_net - F_exact per cell, eV/A; <= 0: energies only)."""
+        self._check(self.lib.d3x_sur_set_force_tol(self.h, float(tol_rms_ev_per_A)), "d3x_sur_set_force_tol")
+
+    def last_force_dev(self) -> float:
+        """RMS force deviation of the last judged evaluation (eV/A; negative before any)."""
+        return float(self.lib.d3x_sur_last_force_dev(self.h))
 
     def features(self) -> np.ndarray:
         x = np.zeros((self.n, SUR_NFEAT), np.float32)

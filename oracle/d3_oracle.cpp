@@ -460,6 +460,45 @@ double d3o_time_centers(const double *pos_A, const int32_t *Z, int64_t n, const 
     return t1 - t0;
 }
 
+// Per-centre results for checks of large cells: for the given centre atoms, CN from the CN pass
+// (out_cn), and with the CN of every atom taken from cn_in and its total dE/dCN from g_in, the
+// two-body + three-body per-atom energy (eV, out_e), their dE/dCN (Hartree, out_g, before the
+// chain rule), and the force on the centre (eV/Angstrom, out_f: two-body, three-body and chain
+// rule).  Each is what a full evaluation gives for that atom when cn_in and g_in are its CN and
+// dE/dCN; the cost is the centres' share of a full evaluation.
+int d3o_centers(const double *pos_A, const int32_t *Z, int64_t n, const double *cell_A, const float *c6ab,
+                const float *r0ab, const float *rcov, const float *r2r4, int32_t nel, int32_t nref, int32_t zoff,
+                double rc2, double rccn, double rc3, int32_t damping, const double *params, const int64_t *centers,
+                int64_t nc, const double *cn_in, const double *g_in, double *out_cn, double *out_e, double *out_g,
+                double *out_f) {
+    Tab t{c6ab, r0ab, rcov, r2r4, nel, nref, zoff};
+    Sys S;
+    int rc = setup(S, pos_A, Z, n, cell_A, t, rc2, rccn, rc3, damping, params);
+    if (rc) return rc;
+    for (int64_t q = 0; q < nc; ++q)
+        if (centers[q] < 0 || centers[q] >= n) return 4;
+    Out o;
+    o.cn.assign(n, 0);
+    o.g.assign(n, 0);
+    o.e.assign(n, 0);
+    o.f.assign(n * 3, 0);
+    o.sig.assign(n * 9, 0);
+    pass_cn(S, t, o, centers, nc);
+    for (int64_t q = 0; q < nc; ++q) out_cn[q] = o.cn[centers[q]];
+    o.cn.assign(cn_in, cn_in + n);
+    if (rc3 > 0) pass_atm(S, t, o, centers, nc);
+    pass_2b(S, t, o, centers, nc);
+    for (int64_t q = 0; q < nc; ++q) out_g[q] = o.g[centers[q]];
+    o.g.assign(g_in, g_in + n);
+    pass_chain(S, t, o, centers, nc);
+    for (int64_t q = 0; q < nc; ++q) {
+        const int64_t i = centers[q];
+        out_e[q] = -HARTREE_EV * o.e[i];
+        for (int a = 0; a < 3; ++a) out_f[3 * q + a] = o.f[3 * i + a] * HARTREE_EV / BOHR;
+    }
+    return 0;
+}
+
 int d3o_max_threads(void) { return omp_get_max_threads(); }
 
 }  // extern "C"

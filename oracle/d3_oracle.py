@@ -49,6 +49,10 @@ def load():
         lib.d3o_time_centers.argtypes = [dp, i32p, C.c_int64, dp, fp, fp, fp, fp, C.c_int32, C.c_int32, C.c_int32,
                                          C.c_double, C.c_double, C.c_double, C.c_int32, dp, i64p, C.c_int64, dp,
                                          dp, C.c_int32, u64p]
+        lib.d3o_centers.restype = C.c_int
+        lib.d3o_centers.argtypes = [dp, i32p, C.c_int64, dp, fp, fp, fp, fp, C.c_int32, C.c_int32, C.c_int32,
+                                    C.c_double, C.c_double, C.c_double, C.c_int32, dp, i64p, C.c_int64, dp, dp,
+                                    dp, dp, dp, dp]
         lib.d3o_max_threads.restype = C.c_int
         _lib = lib
     return _lib
@@ -105,6 +109,33 @@ def time_centers(coords, Z, cell, table: dict, centers, cn, dedcn, cutoff=94.868
                              _p(cen, C.c_int64), len(cen), _p(cnv, C.c_double), _p(gv, C.c_double), int(nthreads),
                              _p(counts, C.c_uint64))
     return t, counts
+
+
+def centers(coords, Z, cell, table: dict, centers, cn, dedcn, cutoff=94.8683298, cn_cutoff=40.0,
+            atm_cutoff=0.0, damping=1, functional=0) -> dict:
+    """Per-centre results for the given centre atoms of a large cell: cn from the CN pass; with the
+    CN and total dE/dCN of every atom taken from cn / dedcn, the per-atom energy (eV), the
+    two-body + three-body dE/dCN (Hartree) and the force (eV/Angstrom) of each centre."""
+    lib = load()
+    x = np.ascontiguousarray(coords, dtype=np.float64).reshape(-1, 3)
+    z = np.ascontiguousarray(Z, dtype=np.int32)
+    cl = np.ascontiguousarray(cell, dtype=np.float64).reshape(9)
+    prm = functional_params(functional)
+    tb = {k: np.ascontiguousarray(table[k], dtype=np.float32) for k in ("c6ab", "r0ab", "rcov", "r2r4")}
+    cen = np.ascontiguousarray(centers, dtype=np.int64)
+    cnv = np.ascontiguousarray(cn, dtype=np.float64)
+    gv = np.ascontiguousarray(dedcn, dtype=np.float64)
+    nc = len(cen)
+    ocn, oe, og, of = np.zeros(nc), np.zeros(nc), np.zeros(nc), np.zeros((nc, 3))
+    rc = lib.d3o_centers(_p(x, C.c_double), _p(z, C.c_int32), len(z), _p(cl, C.c_double), _p(tb["c6ab"], C.c_float),
+                         _p(tb["r0ab"], C.c_float), _p(tb["rcov"], C.c_float), _p(tb["r2r4"], C.c_float),
+                         table["nel"], table["nref"], table["zoff"], float(cutoff), float(cn_cutoff),
+                         float(atm_cutoff), int(damping), _p(prm, C.c_double), _p(cen, C.c_int64), nc,
+                         _p(cnv, C.c_double), _p(gv, C.c_double), _p(ocn, C.c_double), _p(oe, C.c_double),
+                         _p(og, C.c_double), _p(of, C.c_double))
+    if rc:
+        raise ValueError(f"oracle rejected the input (code {rc})")
+    return dict(cn=ocn, atom_energy=oe, dedcn=og, forces=of)
 
 
 def max_threads() -> int:

@@ -29,7 +29,7 @@ def declared_functions():
 
 def test_every_declared_symbol_is_exported():
     names = declared_functions()
-    assert len(names) == 48  # 9 reference entry points + 26 extensions + 13 of the three-body stand-in
+    assert len(names) == 50  # 9 reference entry points + 26 extensions + 15 of the three-body stand-in
     out = subprocess.run(["nm", "-D", "--defined-only", d3_lib.LIB_PATH], capture_output=True, text=True,
                          check=True).stdout
     exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
@@ -127,6 +127,27 @@ def test_table_file_round_trip(tmp_path):
     finally:
         d3_lib.load_table(None)
     assert d3_lib.table()["name"] == "synthetic-original"
+
+
+def test_non_factorising_table_is_rejected(tmp_path):
+    """The kernels hold one set of CN references per element (entry [1] of c6ab[A][B][a][b] equal
+    for every partner B, as in the D3 reference tables); a table that breaks this is refused when
+    a handle over the affected elements is made, before any device use."""
+    import pack_table
+    t0 = d3_lib.table()
+    c6 = t0["c6ab"].copy()
+    c6[6, 8, 0, 0, 1] += 0.25  # C's first CN reference, seen from a C-O pair only
+    path = str(tmp_path / "nonfact.d3tab")
+    pack_table.write_table(path, c6, t0["r0ab"], t0["rcov"], t0["r2r4"], zoff=0, zmax=94)
+    lib = d3_lib.load()
+    try:
+        d3_lib.load_table(path)
+        el = np.array([6, 8], np.uint16)
+        h = lib.d3x_init(d3_lib.ptr(el, C.c_uint16), 2, 16, 40.0, 40.0, 0.0, 1, 0, 0)
+        assert not h
+        assert "factorise" in d3_lib.last_error()
+    finally:
+        d3_lib.load_table(None)
 
 
 def test_header_initialiser_parsing(tmp_path):

@@ -5,6 +5,8 @@ centre with more forward three-body neighbours than the scratch capacity (grown 
 again).  The FP64 engine is held to the oracle at the parity bar wherever there is something to
 compare."""
 
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -126,3 +128,37 @@ def test_three_body_capacity_overflow_is_retried(table):
     h.close()
     assert cap > 1100
     _assert_parity(out, o)
+
+
+def test_single_cell_calls_after_a_batch(table):
+    """A handle that evaluated a batch: the reference's compute entry refuses it (one energy and
+    one stress would not hold the batch's results), and set_cell alone does not turn the batch's
+    atoms into one cell -- set_atoms is required again, after which the handle is a one-cell
+    handle at the parity bar."""
+    print("case single-cell calls after a batch", flush=True)
+    rng = np.random.Generator(np.random.PCG64(31))
+    cells, pos, Z, counts = [], [], [], []
+    for n, edge in ((6, 8.0), (9, 9.0)):
+        cell = np.eye(3) * edge + rng.uniform(-0.4, 0.4, (3, 3))
+        p, z = S.random_packing(n, cell, rng, elements=np.array([1, 6, 8], np.uint16))
+        cells.append(cell)
+        pos.append(p)
+        Z.append(z)
+        counts.append(n)
+    h = d3_lib.Handle([1, 6, 8], sum(counts), **KW)
+    h.compute_batch(counts, np.concatenate(pos), np.concatenate(Z), np.array(cells))
+    lib = d3_lib.load()
+    E = C.c_float(0.0)
+    F = np.zeros((sum(counts), 3), np.float32)
+    Sg = np.zeros(9, np.float32)
+    st = lib.compute_dispersion_energy_from_handle_status(h.h, C.byref(E), d3_lib.ptr(F, C.c_float),
+                                                          d3_lib.ptr(Sg, C.c_float))
+    assert st == 1 and "batch" in d3_lib.last_error()
+    h.set_cell(cells[1])
+    with pytest.raises(d3_lib.D3Error) as err:
+        h.compute()
+    assert err.value.code & d3_lib.D3X_ERR_STATE
+    h.set_atoms(pos[1], Z[1])
+    out = h.compute()
+    h.close()
+    _assert_parity(out, d3_oracle.compute(pos[1], Z[1], cells[1], table, **KW))

@@ -6,7 +6,12 @@
 * the 256-cell triclinic batch: per-cell results of the batched evaluation equal single-cell runs
   for a sample of cells, sum of forces per cell, FP32 against FP64 per cell;
 * the 1,000,188-atom fcc Cu cell without jitter: per-atom energy, CN, dE/dCN and stress equal the
-  4-atom cell's (oracle), forces vanish.
+  4-atom cell's (oracle), forces vanish;
+* the jittered 1,000,188-atom Cu cell and the 100,000-atom random H/C/N/O/Si/Cu cell that the
+  benchmark times: at random centre atoms the oracle's per-centre passes (CN from scratch; energy,
+  dE/dCN and force with the other atoms' CN and dE/dCN from the engine) against the engine's
+  per-atom results, FP64 at the parity bar and FP32 at its own; the random cell also with the
+  three-body network in place of the exact term (judged against the exact kernel).
 """
 
 import numpy as np
@@ -134,3 +139,88 @@ def test_perfect_fcc_cu_million_atoms(table):
     print(f"case perfect fcc Cu: {n} atoms fp32", flush=True)
     e32 = run(pos, Z, cell, d3_lib.D3X_FP32)[0]
     assert abs(e32 - e) <= 1e-6 * abs(e)
+
+
+def _centre_parity(pos, Z, cell, table, precision, n_centres, seed, kw=RC):
+    """Engine per-atom results at n_centres random atoms against the oracle's per-centre passes."""
+    n = len(Z)
+    h = handle(Z, n, precision, **{k: v for k, v in kw.items() if k not in RC or RC[k] != v})
+    h.set_atoms(pos, Z)
+    h.set_cell(cell)
+    e, f, st = h.compute()
+    cn, g, ea = h.atom_data()
+    h.close()
+    cen = np.sort(np.random.Generator(np.random.PCG64(seed)).choice(n, n_centres, replace=False))
+    o = d3_oracle.centers(pos, Z, cell, table, cen, cn, g, **kw)
+    d = dict(cn=np.max(np.abs(cn[cen] - o["cn"]) / np.maximum(np.abs(o["cn"]), 1.0)),
+             e=np.max(np.abs(ea[cen] - o["atom_energy"]) / np.abs(o["atom_energy"])),
+             # dE/dCN against its own scale or, where it nearly vanishes (one reference dominating
+             # every weight, as on Cu), against the per-atom energy in Hartree
+             g=np.max(np.abs(g[cen] - o["dedcn"])) / max(np.max(np.abs(o["dedcn"])),
+                                                         np.max(np.abs(o["atom_energy"])) / 27.211396641308),
+             f=np.max(np.abs(f[cen] - o["forces"])), fmax=np.max(np.abs(o["forces"])))
+    print("  max deviation: " + ", ".join(f"{k} {v:.2e}" for k, v in d.items()), flush=True)
+    return d, (e, f, st)
+
+
+@pytest.fixture(scope="module")
+def jittered_cu():
+    return S.fcc_cu(63, jitter=0.05, seed=0)  # the benchmark's input
+
+
+@pytest.mark.parametrize("precision", [d3_lib.D3X_FP64, d3_lib.D3X_FP32])
+def test_jittered_cu_million_atoms_at_random_centres(table, jittered_cu, precision):
+    pos, Z, cell = jittered_cu
+    print(f"case jittered fcc Cu {len(Z)} atoms precision={precision}: 64 centres", flush=True)
+    d, _ = _centre_parity(pos, Z, cell, table, precision, 64, seed=101)
+    if precision == d3_lib.D3X_FP64:
+        assert d["cn"] <= 1e-12 and d["e"] <= 1e-11 and d["g"] <= 1e-10 and d["f"] <= 1e-9
+    else:
+        assert d["cn"] <= 2e-6 and d["e"] <= 2e-6 and d["g"] <= 2e-5 and d["f"] <= 5e-6
+
+
+@pytest.fixture(scope="module")
+def random100k():
+    return S.random_cell(100000, seed=4242)  # the benchmark's network workload
+
+
+@pytest.mark.parametrize("precision", [d3_lib.D3X_FP64, d3_lib.D3X_FP32])
+def test_random_cell_100k_exact_at_random_centres(table, random100k, precision):
+    pos, Z, cell = random100k
+    print(f"case random cell {len(Z)} atoms precision={precision}: 96 centres", flush=True)
+    d, _ = _centre_parity(pos, Z, cell, table, precision, 96, seed=102)
+    if precision == d3_lib.D3X_FP64:
+        assert d["cn"] <= 1e-12 and d["e"] <= 1e-11 and d["g"] <= 1e-10 and d["f"] <= 1e-9
+    else:
+        assert d["cn"] <= 2e-6 and d["e"] <= 2e-6 and d["g"] <= 2e-5 and d["f"] <= 2e-5 * d["fmax"] + 1e-6
+
+
+def test_random_cell_100k_network_mode(random100k):
+    """The benchmark's network leg as a test: the judged first evaluation either accepts the network
+    (energy within 1 meV/atom and forces within the judge's RMS tolerance of the exact kernel's) or
+    falls back and returns the exact kernel's results."""
+    import os
+    pos, Z, cell = random100k
+    n = len(Z)
+    weights = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "lasp-d3_amd", "surrogate",
+                           "atm_surrogate.d3sur")
+    print(f"case random cell {n} atoms: network mode", flush=True)
+    h = handle(Z, n, d3_lib.D3X_FP32)
+    h.set_atoms(pos, Z)
+    h.set_cell(cell)
+    e_x, f_x, s_x = h.compute()
+    h.sur_load_file(weights)
+    h.set_atm_mode(d3_lib.D3X_ATM_SURROGATE, 0, 1e-3)
+    e_n, f_n, s_n = h.compute()
+    mode, fb, dev = h.sur_state()
+    fdev = h.last_force_dev()
+    h.close()
+    print(f"  judge: mode {mode}, |dE|/atom {1e3 * dev:.4f} meV, force RMS dev {fdev:.3e} eV/A", flush=True)
+    assert dev >= 0 and fdev >= 0
+    if mode == d3_lib.D3X_ATM_SURROGATE:
+        assert fb == 0 and dev <= 1e-3 and fdev <= 5e-3
+        assert abs(e_n - e_x) / n == pytest.approx(dev, rel=1e-6, abs=1e-9)
+        assert float(np.sqrt(np.mean((f_n - f_x) ** 2))) == pytest.approx(fdev, rel=1e-3)
+    else:
+        assert fb == 1 and (dev > 1e-3 or fdev > 5e-3)
+        assert abs(e_n - e_x) <= 1e-6 * abs(e_x) and np.max(np.abs(f_n - f_x)) <= 1e-5

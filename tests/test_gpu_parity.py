@@ -100,9 +100,15 @@ def test_batch_equals_single(table):
         off += n
 
 
-def test_drop_in_float_api(table):
-    """The reference-compatible float API (d3_cffi.D3Calculator) returns the FP64 results rounded to float."""
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_drop_in_float_api(table, precision, monkeypatch):
+    """The reference-compatible float API (d3_cffi.D3Calculator): by default the FP32 path (within
+    its 2e-6 / 2e-5 bar of the oracle), with D3_PRECISION=fp64 the FP64 results rounded to float."""
     import d3_cffi
+    if precision == "fp64":
+        monkeypatch.setenv("D3_PRECISION", "fp64")
+    else:
+        monkeypatch.delenv("D3_PRECISION", raising=False)
     s = systems.SMALL
     calc = d3_cffi.D3Calculator(list(s["Z"]), len(s["Z"]), cutoff_radius=40.0, cn_cutoff_radius=40.0,
                                 damping_type=0, functional_type=1)
@@ -112,9 +118,45 @@ def test_drop_in_float_api(table):
     calc.clear()
     o = oracle_result(s, table)
     # float32 rounding of Angstrom inputs (the test coordinates are float32 values already)
-    assert abs(e - o["energy"]) <= 1e-6 * abs(o["energy"])
-    assert np.max(np.abs(f - o["forces"])) <= 1e-6 * np.max(np.abs(o["forces"])) + 1e-9
+    te, tf = (1e-6, 1e-6) if precision == "fp64" else (2e-6, 2e-5)
+    assert abs(e - o["energy"]) <= te * abs(o["energy"])
+    assert np.max(np.abs(f - o["forces"])) <= tf * np.max(np.abs(o["forces"])) + 1e-9
     assert f.dtype == np.float32 and f.shape == (10, 3) and st.shape == (3, 3)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_drop_in_md_loop(table, precision, monkeypatch):
+    """The reference's MD pattern (src/test_d3_cffi.py:79-123): one D3Calculator with the
+    three-body term, then per step set_atoms -> set_cell -> compute -> clear on coordinates
+    displaced by U(-0.01, 0.01) A per component and a slowly strained cell; every step against the
+    oracle on the same float inputs (FP64 at the float output's resolution, FP32 at its bar)."""
+    import d3_cffi
+    if precision == "fp64":
+        monkeypatch.setenv("D3_PRECISION", "fp64")
+    else:
+        monkeypatch.delenv("D3_PRECISION", raising=False)
+    s = systems.triclinic_mixed(n=36, seed=41)
+    Z = s["Z"]
+    kw = dict(cutoff=50.0, cn_cutoff=30.0, atm_cutoff=25.0, damping=1, functional=0)
+    print(f"case md-loop {precision} atoms={len(Z)}", flush=True)
+    calc = d3_cffi.D3Calculator(list(Z), len(Z), cutoff_radius=kw["cutoff"], cn_cutoff_radius=kw["cn_cutoff"],
+                                damping_type=1, functional_type=0, atm_cutoff=kw["atm_cutoff"])
+    rng = np.random.Generator(np.random.PCG64(7))
+    pos = s["pos"].copy()
+    te, tf = (1e-6, 1e-6) if precision == "fp64" else (2e-6, 2e-5)
+    for step in range(10):
+        pos = pos + rng.uniform(-0.01, 0.01, pos.shape)
+        cell = s["cell"] @ (np.eye(3) + 1e-3 * step * np.array([[1, 0.2, 0], [0, -0.5, 0], [0.1, 0, 0.3]]))
+        p32, c32 = pos.astype(np.float32), cell.astype(np.float32)
+        calc.set_atoms(p32, list(Z))
+        calc.set_cell(c32)
+        e, f, st = calc.compute()
+        calc.clear()
+        o = d3_oracle.compute(p32.astype(np.float64), Z, c32.astype(np.float64), table, **kw)
+        assert abs(e - o["energy"]) <= te * abs(o["energy"]), step
+        assert np.max(np.abs(f - o["forces"])) <= tf * np.max(np.abs(o["forces"])) + 1e-9, step
+        assert np.max(np.abs(st - o["stress"])) <= tf * np.max(np.abs(o["stress"])) + 1e-9, step
+    del calc
 
 
 @pytest.mark.parametrize("atm", [0.0, 40.0])

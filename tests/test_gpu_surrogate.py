@@ -149,6 +149,7 @@ def test_surrogate_energy_forces_stress_match_oracle(table):
     h = engine(Z, len(Z))
     h.sur_load(p)
     h.set_atm_mode(d3_lib.D3X_ATM_SURROGATE, 0, 1e9)  # the judge always accepts
+    h.set_judge_force_tol(0.0)
     h.set_atoms(pos, Z)
     h.set_cell(cell)
     e, f, s = h.compute()
@@ -184,13 +185,56 @@ def test_judge_accepts_or_falls_back():
     assert np.max(np.abs(f1 - f_exact)) <= 1e-10
     e2, _, _ = h.compute()  # stays on the exact kernel
     assert abs(e2 - e_exact) <= 1e-12 * abs(e_exact) and h.sur_state()[1] == 1
-    # a generous tolerance: the network's results come back, judged every second evaluation
+    # a generous energy tolerance, forces not judged: the network's results come back, judged
+    # every second evaluation
     h.set_atm_mode(d3_lib.D3X_ATM_SURROGATE, 2, 1e9)
+    h.set_judge_force_tol(0.0)
     e3, _, _ = h.compute()
     e4, _, _ = h.compute()
     assert h.sur_state()[0] == d3_lib.D3X_ATM_SURROGATE
     assert abs(e3 - e4) <= 1e-9 * abs(e3) and abs(e3 - e_exact) > 1e-9 * abs(e_exact)
     assert abs(h.sur_state()[2] - abs(e3 - e_exact) / len(Z)) <= 1e-6 * abs(e3 - e_exact) / len(Z) + 1e-12
+    h.close()
+
+
+def test_judge_force_criterion():
+    """The judge's force criterion: the RMS of F_net - F_exact over the cell's force components is
+    reported, and a force tolerance below it falls back to the exact kernel even when the energy
+    tolerance accepts."""
+    pos, Z, cell = mixed_cell(seed=24)
+    p = random_params(seed=15)
+    print("case sur-judge-forces", flush=True)
+    h = engine(Z, len(Z))
+    h.set_atoms(pos, Z)
+    h.set_cell(cell)
+    e_exact, f_exact, s_exact = h.compute()
+    assert h.last_force_dev() < 0
+    h.sur_load(p)
+    h.set_atm_mode(d3_lib.D3X_ATM_SURROGATE, 0, 1e9)
+    h.set_judge_force_tol(0.0)  # energies only: accepted
+    e1, f1, _ = h.compute()
+    assert h.sur_state()[:2] == (d3_lib.D3X_ATM_SURROGATE, 0)
+    fdev = h.last_force_dev()
+    assert fdev > 0
+    assert fdev == pytest.approx(float(np.sqrt(np.mean((f1 - f_exact) ** 2))), rel=1e-6)
+    # the same network, judged again with a force tolerance below its deviation
+    h.set_atm_mode(d3_lib.D3X_ATM_SURROGATE, 0, 1e9)
+    h.set_judge_force_tol(0.5 * fdev)
+    e2, f2, s2 = h.compute()
+    mode, fb, _ = h.sur_state()
+    assert mode == d3_lib.D3X_ATM_EXACT and fb == 1
+    assert abs(e2 - e_exact) <= 1e-12 * abs(e_exact)
+    assert np.max(np.abs(f2 - f_exact)) <= 1e-10
+    np.testing.assert_allclose(s2, s_exact, rtol=1e-10, atol=1e-14)
+    # per-atom data of the handle are the exact kernel's after the fallback
+    cn, g, ea = h.atom_data()
+    assert abs(np.sum(ea) - e_exact) <= 1e-10 * abs(e_exact)
+    # and a tolerance above it accepts
+    h.set_atm_mode(d3_lib.D3X_ATM_SURROGATE, 0, 1e9)
+    h.set_judge_force_tol(2.0 * fdev)
+    e3, f3, _ = h.compute()
+    assert h.sur_state()[0] == d3_lib.D3X_ATM_SURROGATE
+    assert abs(e3 - e1) <= 1e-9 * abs(e1) and np.max(np.abs(f3 - f1)) <= 1e-9
     h.close()
 
 

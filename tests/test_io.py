@@ -63,6 +63,16 @@ def test_cli_arguments():
     assert r.returncode == 1 and "does not exist" in r.stderr
 
 
+def test_cli_rejects_malformed_extxyz(tmp_path):
+    """A coordinate that is not a number ends the parse with an error (no evaluation)."""
+    if not os.path.exists(CLI):
+        pytest.skip("d3_cpp not built")
+    bad = tmp_path / "bad.extxyz"
+    bad.write_text('2\nLattice="10 0 0 0 10 0 0 0 10" Properties=species:S:1:pos:R:3\nC 0 0 0\nO 1.0 abc 0\n')
+    r = subprocess.run([CLI, str(bad)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "Invalid line" in r.stderr and "Error parsing file" in r.stderr
+
+
 def test_cli_declares_only_the_reference_api():
     """The driver is a caller of the reference's C API only (include/d3.h), like the reference's."""
     src = open(os.path.join(ROOT, "lasp-d3_amd", "cli", "d3_cli.cpp")).read()

@@ -390,6 +390,20 @@ def test_supercell_extensive(table):
     np.testing.assert_allclose(s["stress"], o["stress"], atol=1e-11)
 
 
+def test_per_centre_entry_equals_full_evaluation(table):
+    """d3o_centers (per-centre passes for checks of large cells) gives, with the full evaluation's
+    CN and dE/dCN as the other atoms' values, exactly the full evaluation's per-atom CN, energy,
+    dE/dCN and force at the chosen centres."""
+    pos, Z, cell, kw = small_triclinic(True)
+    o = run(pos, Z, cell, table, **kw)
+    cen = np.array([0, 2, 5, 6])
+    c = d3_oracle.centers(pos, Z, cell, table, cen, o["cn"], o["dedcn"], **kw)
+    np.testing.assert_allclose(c["cn"], o["cn"][cen], rtol=1e-14)
+    np.testing.assert_allclose(c["atom_energy"], o["atom_energy"][cen], rtol=1e-13)
+    np.testing.assert_allclose(c["dedcn"], o["dedcn"][cen], rtol=1e-12)
+    np.testing.assert_allclose(c["forces"], o["forces"][cen], rtol=1e-12, atol=1e-15)
+
+
 def test_left_handed_cell_flips_stress_sign(table):
     """V is the signed determinant (src/d3_kernel.cu:90-96): swapping two lattice vectors keeps
     the energy and forces and negates the stress (reference behaviour, quirk Q14)."""
