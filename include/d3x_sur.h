@@ -79,6 +79,15 @@ int d3x_sur_infer(const float *params, const void *packed, uint64_t n, const flo
  * grad (D3X_SUR_NPARAMS floats, accumulated) += dloss/dparams */
 int d3x_sur_train_grad(const float *params, const void *packed, int cls, uint64_t n, const float *X, const float *y,
                        const float *w, float *E, void *work, float *grad, float *loss, void *stream);
+/* No counterpart in the reference: d3x_sur_train_grad for callers that cannot allocate device
+ * memory (code under HIP graph capture, training loops that forbid allocation).  Same arguments
+ * and results without the workspace: the kernels' scratch is a static device array of the
+ * library (12 MiB per device, held until the process ends), so n is at most
+ * D3X_SUR_NOALLOC_MAX_ROWS (larger n returns D3X_ERR_INPUT without launching).  Calls share that
+ * array: calls that overlap on one device (other streams, other threads) race on it. */
+#define D3X_SUR_NOALLOC_MAX_ROWS 16384
+int d3x_sur_train_grad_noalloc(const float *params, const void *packed, int cls, uint64_t n, const float *X,
+                               const float *y, const float *w, float *E, float *grad, float *loss, void *stream);
 /* one force-matching gradient on n rows of class cls: D[n] (out) = dE/dx . U of each row (the
  * network's derivative along U, eV per unit of U) and grad += d(sum_rows w D)/dparams.  Since
  * F_net = -J^T dE/dx, the force loss L = sum |F_net - F_ref|^2 has dL/dparams = -2 sum_rows

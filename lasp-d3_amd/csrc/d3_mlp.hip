@@ -14,8 +14,17 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
+#include <vector>
 
 #include "d3_sur.hpp"
+
+// Fragment scratch of the training gradient for callers that cannot allocate
+// (d3x_sur_train_grad_noalloc): STATIC_TRAIN_ROWS rows, 16 rows per tile of 24 x 64 fragments.
+// One copy per device, held until the process ends; it keeps the last call's contents, which
+// need no clearing (k_train writes every fragment of its tiles before k_wgrad reads them).  Host
+// code never uses this name as an address: static_train_scratch() gives the device's address.
+static __device__ __align__(256) unsigned char scratch[(size_t)d3::sur::STATIC_TRAIN_ROWS * d3::sur::TRAIN_ROW_BYTES];
 
 namespace d3 {
 namespace sur {
@@ -636,6 +645,27 @@ void launch_train(const float *P, const void *PK, int c, int n, const float *X, 
     const int tpc = 64;
     hipLaunchKernelGGL(k_wgrad, dim3(96, (ntile + tpc - 1) / tpc), dim3(64), 0, s, P, c, n, X, (const v4s *)work,
                        tpc, grad);
+}
+
+namespace {
+std::mutex g_scratch_mtx;
+std::vector<void *> g_scratch_ptr;  // address of `scratch` per device (nullptr: not looked up yet)
+}  // namespace
+
+hipError_t static_train_scratch(void **ptr) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> lock(g_scratch_mtx);
+    if ((size_t)dev >= g_scratch_ptr.size()) g_scratch_ptr.resize((size_t)dev + 1, nullptr);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));  // with the device current
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = p;
+    }
+    *ptr = g_scratch_ptr[dev];
+    return hipSuccess;
 }
 
 void launch_train_force(const float *P, const void *PK, int c, int n, const float *X, const float *U, const float *Wt,

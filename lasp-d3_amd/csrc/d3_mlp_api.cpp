@@ -65,6 +65,25 @@ int d3x_sur_train_grad(const float *params, const void *packed, int cls, uint64_
     return launched("d3x_sur_train_grad");
 }
 
+int d3x_sur_train_grad_noalloc(const float *params, const void *packed, int cls, uint64_t n, const float *X,
+                               const float *y, const float *w, float *E, float *grad, float *loss, void *stream) {
+    static_assert(D3X_SUR_NOALLOC_MAX_ROWS == STATIC_TRAIN_ROWS, "static scratch rows");
+    if (n == 0) return 0;
+    if (!params || !packed || !X || !y || !w || !E || !grad || !loss)
+        return bad("d3x_sur_train_grad_noalloc", "NULL buffer");
+    if (cls < 0 || cls >= NCLASS) return bad("d3x_sur_train_grad_noalloc", "class out of range");
+    if (n > (uint64_t)STATIC_TRAIN_ROWS)
+        return bad("d3x_sur_train_grad_noalloc", "more rows than the static scratch holds (at most 16384 per call)");
+    void *work = nullptr;
+    const hipError_t e = static_train_scratch(&work);
+    if (e != hipSuccess) {
+        d3::set_last_error(std::string("d3x_sur_train_grad_noalloc: static scratch: ") + hipGetErrorString(e));
+        return (int)(D3X_ERR_GENERIC | D3X_ERR_DEVICE);
+    }
+    launch_train(params, packed, cls, (int)n, X, y, w, E, work, grad, loss, (hipStream_t)stream);
+    return launched("d3x_sur_train_grad_noalloc");
+}
+
 int d3x_sur_train_force_grad(const float *params, const void *packed, int cls, uint64_t n, const float *X,
                              const float *U, const float *w, float *D, void *work, float *grad, void *stream) {
     if (n == 0) return 0;

@@ -46,6 +46,13 @@ inline int element_class(int Z) { return Z <= 2 ? 0 : (Z <= 10 ? 1 : 2); }
 // loss; of h1, h1', dz2, dt2, dz1, dt1 for the force loss (tangent pass).
 constexpr size_t TRAIN_ROW_BYTES = 3 * NHID * 2, TRAIN_FORCE_ROW_BYTES = 6 * NHID * 2;
 
+// Rows of one call of the training gradient with static scratch (d3x_sur_train_grad_noalloc):
+// the fragment scratch is a file-scope device array of STATIC_TRAIN_ROWS * TRAIN_ROW_BYTES
+// (12 MiB), one copy per device for the life of the process.
+constexpr int STATIC_TRAIN_ROWS = 16384;
+// Device address of that array on the current device (looked up once per device).
+hipError_t static_train_scratch(void **ptr);
+
 // Launchers (d3_mlp.hip).  P: fp32 parameters, PK: packed bf16 fragments (PACKED_BYTES).
 void launch_pack(const float *P, void *PK, hipStream_t s);
 void launch_infer(const float *P, const void *PK, int n, const float *X, const int *cls, float *E, float *G,

@@ -92,6 +92,7 @@ SIGNATURES = {
     "d3x_sur_pack": (C.c_int, [_vp, _vp, _vp]),
     "d3x_sur_infer": (C.c_int, [_vp, _vp, _u64, _vp, _vp, _vp, _vp, _vp]),
     "d3x_sur_train_grad": (C.c_int, [_vp, _vp, C.c_int, _u64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "d3x_sur_train_grad_noalloc": (C.c_int, [_vp, _vp, C.c_int, _u64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "d3x_sur_train_force_grad": (C.c_int, [_vp, _vp, C.c_int, _u64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "d3x_sur_adam": (C.c_int, [_vp, _vp, _vp, _vp, _f, _f, _f, _f, C.c_int, _vp]),
 }
