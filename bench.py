@@ -133,6 +133,27 @@ def time_steps(h, ev, steps, warmup):
     return ms / steps, wall
 
 
+def host_to_host(w, runs=5):
+    """The timed region of a caller with host buffers: positions, types and cell in host memory ->
+    energy, forces and stress in host memory (d3x_set_atoms_f64, d3x_set_cell_f64, d3x_compute_f64 on
+    an FP32 handle), wall clock, one warm-up then the median of `runs`."""
+    n = len(w["Z"])
+    h = d3_lib.Handle(np.unique(w["Z"]), n, cutoff=RC2, cn_cutoff=RCCN, atm_cutoff=RC3, damping=1, functional=0,
+                      precision=d3_lib.D3X_FP32)
+    times = []
+    for k in range(runs + 1):
+        t0 = time.perf_counter()
+        h.set_atoms(w["pos"], w["Z"])
+        h.set_cell(w["cell"])
+        h.compute()
+        if k:
+            times.append(time.perf_counter() - t0)
+    h.close()
+    ms = 1e3 * float(np.median(times))
+    return dict(ms_per_step=ms, atoms_per_s=n / (ms * 1e-3), runs=runs,
+                note="host positions in, host E/F/stress out, PCIe and host preparation included")
+
+
 def phase_profile(h, runs=2):
     h.set_timing(True)
     for _ in range(runs):
@@ -216,7 +237,10 @@ def cpu_baseline(w, table, cn, g, target_s=15.0, target_1t_s=5.0):
     if w["kind"] != "single":
         return None
     n = len(w["Z"])
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    # every CPU this job may run on (the box sets OMP_NUM_THREADS to its CPU share), threads kept
+    # close to their cores
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
     rng = np.random.Generator(np.random.PCG64(1234))
     kw = dict(cutoff=RC2, cn_cutoff=RCCN, atm_cutoff=RC3, damping=1, functional=0)
     probe = rng.choice(n, size=min(n, threads), replace=False)
@@ -315,6 +339,9 @@ def main():
                note="hipMemGetInfo deltas: the FP32 handle after its first evaluation, then with the FP64 handle too")
     h32.close()
     h64.close()
+    h2h = host_to_host(w) if w["kind"] == "single" else None
+    if h2h:
+        print(f"[bench] host to host fp32 {h2h['ms_per_step']:.2f} ms/step", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu and w["kind"] == "single":
@@ -341,6 +368,7 @@ def main():
                    "table": d3_lib.table()["name"], "hip_graph": bool(args.graph), "autotuned": bool(args.autotune),
                    "bin_width_bohr": tuning32[0], "atm_blocks": tuning32[1], "atm_variant": variant32},
         "fp64_plain": {"atoms_per_s": n / (ms64 * 1e-3), "ms_per_step": ms64, "steps": k64},
+        "host_to_host_fp32": h2h,
         "speedup_fp32_vs_fp64": ms64 / ms32,
         "fp32_error_vs_fp64": err,
         "phase_ms_fp32": prof32,
