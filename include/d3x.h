@@ -92,6 +92,24 @@ int d3x_set_tuning(d3_handle_t *handle, double bin_width_bohr, int atm_blocks);
 int d3x_get_tuning(d3_handle_t *handle, double *bin_width_bohr, int *atm_blocks);
 int d3x_get_atm_variant(d3_handle_t *handle);
 
+/* Neighbour state across evaluations, for MD loops that call set_atoms with slightly moved atoms
+ * (no counterpart in the reference, which sorts its cell list again at every compute,
+ * src/d3.cu:141-143).  skin_bohr > 0: every stencil covers its cutoff plus the skin, and an
+ * evaluation after set_atoms with the same atoms (count and elements) in the same cell keeps the
+ * bin order and the three-body forward-neighbour lists of the last rebuild while no atom has moved
+ * more than skin_bohr / 2 from its position then (compared on the host at set_atoms); otherwise it
+ * rebuilds them.  Results equal those of a rebuild up to summation order.  0 (default) rebuilds
+ * at every evaluation.  Batches always rebuild.  D3_SKIN=<bohr> in the environment sets it for
+ * handles made by the reference-compatible init functions.
+ * d3x_set_neighbour_lists: -1 (default) = three-body forward lists when a skin is set or centres
+ * are split into more three-body work items than four per resident block, 0 = never (phase 1
+ * walks the stencil), 1 = always; lists need atoms x capacity x 8 bytes and are not used beyond
+ * 4 GiB.
+ * d3x_neighbour_stats: evaluations so far that rebuilt / reused the neighbour state. */
+int d3x_set_skin(d3_handle_t *handle, double skin_bohr);
+int d3x_set_neighbour_lists(d3_handle_t *handle, int mode);
+int d3x_neighbour_stats(d3_handle_t *handle, int *rebuilds, int *reuses);
+
 /* Forward-neighbour capacity per centre of the three-body scratch (0 before the first
  * evaluation).  A centre with more forward neighbours within the ATM cutoff makes the
  * evaluation grow it (x4) and run again; the result is unaffected. */

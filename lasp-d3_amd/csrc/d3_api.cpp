@@ -128,10 +128,20 @@ d3_handle_t *make_handle(const uint16_t *elements, uint64_t ne, uint64_t max_len
     return st ? nullptr : h;
 }
 
-// D3_GRAPH=1: reference-compatible handles evaluate through a captured hipGraph.
+// Options of reference-compatible handles from the environment: D3_GRAPH=1 evaluates through a
+// captured hipGraph, D3_SKIN=<bohr> keeps the neighbour state across MD steps.
 d3_handle_t *with_env_graph(d3_handle_t *h) {
+    if (!h) return h;
     const char *g = std::getenv("D3_GRAPH");
-    if (h && g && std::strcmp(g, "1") == 0) H(h)->eng->set_graph(true);
+    if (g && std::strcmp(g, "1") == 0) H(h)->eng->set_graph(true);
+    const char *sk = std::getenv("D3_SKIN");
+    if (sk && *sk) {
+        const double v = std::atof(sk);
+        if (v > 0 && guarded(H(h), "D3_SKIN", [&] { H(h)->eng->set_skin(v); })) {
+            delete H(h);
+            return nullptr;
+        }
+    }
     return h;
 }
 
@@ -315,6 +325,25 @@ int d3x_get_tuning(d3_handle_t *handle, double *bin_width_bohr, int *atm_blocks)
     if (!handle) return (int)(D3X_ERR_GENERIC | D3X_ERR_INPUT);
     if (bin_width_bohr) *bin_width_bohr = H(handle)->eng->bin_width();
     if (atm_blocks) *atm_blocks = H(handle)->eng->atm_blocks_effective();
+    return 0;
+}
+
+int d3x_set_skin(d3_handle_t *handle, double skin_bohr) {
+    if (!handle) return (int)(D3X_ERR_GENERIC | D3X_ERR_INPUT);
+    return (int)guarded(H(handle), "d3x_set_skin", [&] { H(handle)->eng->set_skin(skin_bohr); });
+}
+
+int d3x_set_neighbour_lists(d3_handle_t *handle, int mode) {
+    if (!handle) return (int)(D3X_ERR_GENERIC | D3X_ERR_INPUT);
+    return (int)guarded(H(handle), "d3x_set_neighbour_lists", [&] { H(handle)->eng->set_neighbour_lists(mode); });
+}
+
+int d3x_neighbour_stats(d3_handle_t *handle, int *rebuilds, int *reuses) {
+    if (!handle) return (int)(D3X_ERR_GENERIC | D3X_ERR_INPUT);
+    int a = 0, b = 0;
+    H(handle)->eng->neighbour_stats(a, b);
+    if (rebuilds) *rebuilds = a;
+    if (reuses) *reuses = b;
     return 0;
 }
 

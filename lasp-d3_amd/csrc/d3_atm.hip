@@ -196,42 +196,68 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
         }
         __syncthreads();
 
-        // ---- phase 1: forward neighbours within rc3 (1e-12 <= r^2 <= rc3^2, src/d3_kernel.cu:1268)
+        // ---- phase 1: forward neighbours within rc3 (1e-12 <= r^2 <= rc3^2, src/d3_kernel.cu:1268),
+        // compacted in stencil-walk order: from the centre's forward list when the handle keeps one
+        // (the same candidates in the same order, so the same compaction), else by the walk.
         int count = 0;
-        const int s0 = C.S->st_off[CUT_ATM], s1 = s0 + C.S->st_cnt[CUT_ATM];
-        for (int sb = s0; sb < s1; sb += AW) {
-            const auto Bn = C.bins(P, sb, s1);
-            const int nsb = s1 - sb < AW ? s1 - sb : AW;
-            for (int sk = 0; sk < nsb; ++sk) {
-                int j0, j1;
-                T ox, oy, oz;
-                Bn.get(sk, j0, j1, ox, oy, oz);
-                if (j1 <= i) continue;  // bins are sorted: every atom here has index < i
-                const StEntry d = P.st_d[sb + sk];
-                const bool dlex = d.dx > 0 || (d.dx == 0 && (d.dy > 0 || (d.dy == 0 && d.dz > 0)));
-                for (int jb = j0; jb < j1; jb += AW) {
-                    const int j = jb + lane;
-                    bool in = false;
-                    T dx = T(0), dy = T(0), dz = T(0), r2 = T(0);
-                    if (j < j1 && (j > i || (j == i && dlex))) {
-                        dx = ox - P.lx[j];
-                        dy = oy - P.ly[j];
-                        dz = oz - P.lz[j];
-                        r2 = dx * dx + dy * dy + dz * dz;
-                        in = r2 <= rc32 && r2 >= T(1e-12);
-                    }
-                    const uint64_t m = ballot64(in);
-                    if (m == 0) continue;
-                    if (in) {
-                        const int pos = count + mbcnt64(m);
-                        if (pos < cap) {
-                            Lj[pos] = j;
-                            LT[F_DX * cap + pos] = dx;
-                            LT[F_DY * cap + pos] = dy;
-                            LT[F_DZ * cap + pos] = dz;
+        auto take = [&](int j, T dx, T dy, T dz, bool in) {
+            const uint64_t m = ballot64(in);
+            if (m == 0) return;
+            if (in) {
+                const int pos = count + mbcnt64(m);
+                if (pos < cap) {
+                    Lj[pos] = j;
+                    LT[F_DX * cap + pos] = dx;
+                    LT[F_DY * cap + pos] = dy;
+                    LT[F_DZ * cap + pos] = dz;
+                }
+            }
+            count += __popcll(m);
+        };
+        if (P.atm_fl) {
+            const FwdEntry *FL = P.atm_fl + (size_t)i * P.atm_fl_cap;
+            const int nl = min(P.atm_fl_n[i], P.atm_fl_cap);
+            for (int fb = 0; fb < nl; fb += AW) {
+                const int f = fb + lane;
+                int j = 0;
+                bool in = false;
+                T dx = T(0), dy = T(0), dz = T(0);
+                if (f < nl) {
+                    const FwdEntry e = FL[f];
+                    j = e.j;
+                    dx = (C.x + P.st_D[4 * e.s + 0]) - P.lx[j];
+                    dy = (C.y + P.st_D[4 * e.s + 1]) - P.ly[j];
+                    dz = (C.z + P.st_D[4 * e.s + 2]) - P.lz[j];
+                    const T r2 = dx * dx + dy * dy + dz * dz;
+                    in = r2 <= rc32 && r2 >= T(1e-12);
+                }
+                take(j, dx, dy, dz, in);
+            }
+        } else {
+            const int s0 = C.S->st_off[CUT_ATM], s1 = s0 + C.S->st_cnt[CUT_ATM];
+            for (int sb = s0; sb < s1; sb += AW) {
+                const auto Bn = C.bins(P, sb, s1);
+                const int nsb = s1 - sb < AW ? s1 - sb : AW;
+                for (int sk = 0; sk < nsb; ++sk) {
+                    int j0, j1;
+                    T ox, oy, oz;
+                    Bn.get(sk, j0, j1, ox, oy, oz);
+                    if (j1 <= i) continue;  // bins are sorted: every atom here has index < i
+                    const StEntry d = P.st_d[sb + sk];
+                    const bool dlex = d.dx > 0 || (d.dx == 0 && (d.dy > 0 || (d.dy == 0 && d.dz > 0)));
+                    for (int jb = j0; jb < j1; jb += AW) {
+                        const int j = jb + lane;
+                        bool in = false;
+                        T dx = T(0), dy = T(0), dz = T(0);
+                        if (j < j1 && (j > i || (j == i && dlex))) {
+                            dx = ox - P.lx[j];
+                            dy = oy - P.ly[j];
+                            dz = oz - P.lz[j];
+                            const T r2 = dx * dx + dy * dy + dz * dz;
+                            in = r2 <= rc32 && r2 >= T(1e-12);
                         }
+                        take(j, dx, dy, dz, in);
                     }
-                    count += __popcll(m);
                 }
             }
         }
@@ -622,6 +648,54 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
     }
 }
 
+// Forward-neighbour list of each centre (one wave per centre): the three-body stencil walk of
+// phase 1 with the list cutoff, keeping (j, stencil entry) of every forward neighbour within it in
+// walk order.  With a skin the list stays valid while no atom moves more than half the skin.
+template <class T>
+__global__ __launch_bounds__(256) void k_atm_fwd(Params<T> P) {
+    const int lane = threadIdx.x & (AW - 1);
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= P.n) return;  // whole wave
+    Centre<T> C;
+    C.load(P, i);
+    const int cap = P.atm_fl_cap;
+    FwdEntry *FL = P.atm_fl + (size_t)i * cap;
+    const T rcf = P.rc_fl2;
+    int count = 0;
+    const int s0 = C.S->st_off[CUT_ATM], s1 = s0 + C.S->st_cnt[CUT_ATM];
+    for (int sb = s0; sb < s1; sb += AW) {
+        const auto Bn = C.bins(P, sb, s1);
+        const int nsb = s1 - sb < AW ? s1 - sb : AW;
+        for (int sk = 0; sk < nsb; ++sk) {
+            int j0, j1;
+            T ox, oy, oz;
+            Bn.get(sk, j0, j1, ox, oy, oz);
+            if (j1 <= i) continue;
+            const StEntry d = P.st_d[sb + sk];
+            const bool dlex = d.dx > 0 || (d.dx == 0 && (d.dy > 0 || (d.dy == 0 && d.dz > 0)));
+            for (int jb = j0; jb < j1; jb += AW) {
+                const int j = jb + lane;
+                bool in = false;
+                if (j < j1 && (j > i || (j == i && dlex))) {
+                    const T dx = ox - P.lx[j], dy = oy - P.ly[j], dz = oz - P.lz[j];
+                    in = dx * dx + dy * dy + dz * dz <= rcf;
+                }
+                const uint64_t m = ballot64(in);
+                if (m == 0) continue;
+                if (in) {
+                    const int pos = count + mbcnt64(m);
+                    if (pos < cap) FL[pos] = FwdEntry{j, sb + sk};
+                }
+                count += __popcll(m);
+            }
+        }
+    }
+    if (lane == 0) {
+        P.atm_fl_n[i] = count;
+        if (count > cap) atomicOr(P.status, 2u);  // the host grows the capacity and runs again
+    }
+}
+
 // Zeroes fa | ga | eas | eat (6 N doubles) and the centre counter.
 __global__ void k_fill_d(double *p, size_t n, int *counter) {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -728,6 +802,14 @@ void launch_atm(const Params<T> &P, int nblocks, hipStream_t s) {
     if (P.atm_chunks > 1) hipLaunchKernelGGL(k_eat_chunks<T>, dim3((P.n + 255) / 256), dim3(256), 0, s, P);
     hipLaunchKernelGGL(k_merge<T>, dim3((P.n + 255) / 256), dim3(256), 0, s, P);
 }
+
+template <class T>
+void launch_atm_lists(const Params<T> &P, hipStream_t s) {
+    if (P.n == 0 || !P.atm_fl || !(P.rc_32 > T(0))) return;
+    hipLaunchKernelGGL(k_atm_fwd<T>, dim3((P.n + 3) / 4), dim3(256), 0, s, P);
+}
+template void launch_atm_lists<double>(const Params<double> &, hipStream_t);
+template void launch_atm_lists<float>(const Params<float> &, hipStream_t);
 
 template void launch_atm<double>(const Params<double> &, int, hipStream_t);
 template void launch_atm<float>(const Params<float> &, int, hipStream_t);

@@ -32,6 +32,12 @@ struct StEntry {
     int dx, dy, dz, pad;
 };
 
+// One forward three-body neighbour of a centre: sorted atom index and the stencil entry (bin
+// offset and image vector) it was found through.
+struct FwdEntry {
+    int j, s;
+};
+
 struct SysDev {
     double H[9];     // rows = lattice vectors (bohr)
     double Hinv[9];  // H^-1
@@ -98,6 +104,18 @@ struct Params {
     double *atm_acc;
     double *fa, *ga, *eas, *eat;
     int *atm_next;
+    // Forward-neighbour lists (null: phase 1 walks the three-body stencil).  atm_fl[i * atm_fl_cap
+    // + p], p < atm_fl_n[i]: centre i's forward neighbours within sqrt(rc_fl2) (the three-body
+    // cutoff plus the skin) in stencil-walk order, built by launch_atm_lists; phase 1 then tests
+    // only these.  A centre with more than atm_fl_cap sets status bit 2 like a scratch overflow.
+    FwdEntry *atm_fl;
+    int *atm_fl_n;
+    int atm_fl_cap;
+    T rc_fl2;
+    // Lattice shifts of the wrap per caller atom (3 doubles), written by the bin sort when the
+    // handle keeps its neighbour state (null otherwise) and used to gather moved positions into
+    // the kept bin order.
+    double *shift;
     // Work items per centre (1..8): with atm_chunks > 1 item t is centre t / atm_chunks and a
     // contiguous range of its k tiles; each item writes its share of the centre's energy to
     // atm_eatc[chunk * n + centre] (summed in chunk order after the launch) and adds its virial
@@ -132,6 +150,8 @@ __host__ __device__ constexpr size_t atm_T_stride(int cap) { return ATM_NF * (si
 template <class T>
 struct Launch {
     static void bin_sort(const Params<T> &P, int nbins, int *bin_start_mut, hipStream_t s);
+    // Moved positions into the bin order of the last sort (wrap shifts and bins kept).
+    static void regather(const Params<T> &P, hipStream_t s);
     static void cn(const Params<T> &P, hipStream_t s);
     static void twobody(const Params<T> &P, hipStream_t s);
     static void chain(const Params<T> &P, hipStream_t s);
@@ -146,6 +166,9 @@ struct Launch {
 enum { ATM_KL = 1, ATM_DR = 2 };
 template <class T>
 void launch_atm(const Params<T> &P, int nblocks, hipStream_t s);
+// Forward-neighbour lists of every centre (P.atm_fl, P.atm_fl_n), one wave per centre.
+template <class T>
+void launch_atm_lists(const Params<T> &P, hipStream_t s);
 size_t atm_lds_bytes(int ne, int nref, int yrows, size_t tsize, int var);
 int atm_choose_variant(int ne, int nref, int yrows, size_t tsize);
 int atm_resident_blocks(int ne, int nref, int yrows, size_t tsize, int var);

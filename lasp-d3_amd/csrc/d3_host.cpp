@@ -239,7 +239,8 @@ Engine::~Engine() {
                     &d_npair, &d_out_f, &d_out_e, &d_out_cn, &d_out_g, &d_bin_count, &d_bin_start, &d_cursor,
                     &d_sys, &d_st_d, &d_st_D, &d_chunk_b, &d_chunk_e, &d_chunk_s, &d_sys_chunk, &d_partial,
                     &d_out_sys, &d_tab, &d_status, &d_atm_i32, &d_atm_T, &d_atm_acc, &d_atm_pa, &d_type_cls,
-                    &d_feat, &d_sur_G, &d_sur_E, &d_sur_cls, &d_sur_P, &d_sur_PK, &d_tan_dir, &d_tan_U, &d_atm_roff, &d_atm_eatc, &d_tb_part};
+                    &d_feat, &d_sur_G, &d_sur_E, &d_sur_cls, &d_sur_P, &d_sur_PK, &d_tan_dir, &d_tan_U, &d_atm_roff, &d_atm_eatc, &d_tb_part,
+                    &d_fl, &d_fl_n, &d_shift};
     for (void **p : all) dfree(*p);
     for (auto &e : ev_)
         if (e) (void)hipEventDestroy(e);
@@ -307,9 +308,8 @@ void Engine::set_atoms(const double *coords_A, const uint16_t *Z, uint64_t n) {
             pos[3 * i + c] = v / BOHR_A;  // Angstrom -> bohr, src/d3.cu:50-56
         }
     }
-    if (sys_.size() != 1) {
-        sys_.assign(1, Cell{});
-    }
+    const bool was_batch = sys_.size() != 1;
+    if (was_batch) sys_.assign(1, Cell{});
     std::vector<int> sid(n, 0);
     if (n) {
         HIPCHK(hipMemcpyAsync(d_pos_in, pos.data(), pos.size() * sizeof(double), hipMemcpyHostToDevice, stream_));
@@ -317,10 +317,27 @@ void Engine::set_atoms(const double *coords_A, const uint16_t *Z, uint64_t n) {
         HIPCHK(hipMemcpyAsync(d_sys_in, sid.data(), n * sizeof(int), hipMemcpyHostToDevice, stream_));
         HIPCHK(hipStreamSynchronize(stream_));
     }
+    // the bins and stencils depend on the cell and the atom count only
+    if (was_batch || n != n_ || !atoms_set_) geom_dirty_ = true;
+    // the kept neighbour state serves while the same atoms have moved less than half the skin
+    nbr_moved_far_ = true;
+    if (skin_ > 0 && nbr_valid_ && !was_batch && types == nbr_typ_) {
+        double m2 = 0;
+        for (uint64_t i = 0; i < n; ++i) {
+            double d2 = 0;
+            for (int c = 0; c < 3; ++c) {
+                const double d = pos[3 * i + c] - nbr_pos_[3 * i + c];
+                d2 += d * d;
+            }
+            m2 = std::max(m2, d2);
+        }
+        nbr_moved_far_ = !(m2 <= 0.25 * skin_ * skin_);
+    }
+    h_pos_ = std::move(pos);
+    h_typ_ = std::move(types);
     sys_[0].n = n;
     n_ = n;
     atoms_set_ = true;
-    geom_dirty_ = true;
     feat_valid_ = false;
 }
 
@@ -331,6 +348,7 @@ void Engine::set_cell(const double cell_A[9]) {
         H[k] = cell_A[k] / BOHR_A;
     }
     if (!(std::fabs(det3(H)) > 1e-12)) fail(D3X_ERR_INPUT, "singular cell");
+    const bool same = sys_.size() == 1 && sys_[0].set && std::memcmp(sys_[0].H, H, sizeof(H)) == 0;
     if (sys_.size() != 1) {
         // after a batch the device holds the batch's atoms and their cell ids: one cell over
         // them would be wrong, so the atoms must be set again
@@ -341,7 +359,10 @@ void Engine::set_cell(const double cell_A[9]) {
     std::memcpy(sys_[0].H, H, sizeof(H));
     sys_[0].set = true;
     sys_[0].n = n_;
-    geom_dirty_ = true;
+    if (!same) {  // the same cell again (an MD step at constant volume) keeps bins and stencils
+        geom_dirty_ = true;
+        nbr_valid_ = false;
+    }
     feat_valid_ = false;
 }
 
@@ -387,7 +408,25 @@ void Engine::set_batch(uint64_t nsys, const uint64_t *counts, const double *coor
     n_ = tot;
     atoms_set_ = true;
     geom_dirty_ = true;
+    nbr_valid_ = false;
+    nbr_moved_far_ = true;
     feat_valid_ = false;
+}
+
+void Engine::set_skin(double skin) {
+    if (!(skin >= 0) || !std::isfinite(skin)) fail(D3X_ERR_INPUT, "skin must be finite and >= 0");
+    if (skin != skin_) {
+        skin_ = skin;
+        geom_dirty_ = true;  // stencils cover cutoff + skin
+        nbr_valid_ = false;
+    }
+    if (skin_ > 0 && !d_shift) d_shift = dalloc(3 * max_len_ * sizeof(double));
+}
+
+void Engine::set_neighbour_lists(int mode) {
+    if (mode < -1 || mode > 1) fail(D3X_ERR_INPUT, "neighbour list mode must be -1, 0 or 1");
+    fl_mode_ = mode;
+    nbr_valid_ = false;
 }
 
 void Engine::clear() {
@@ -432,10 +471,12 @@ void Engine::build_geometry() {
         d.atom_base = atom_base;
         d.n_atoms = (int)C.n;
         // the stand-in's feature stencil is built only when a mode needs it
+        // with a skin every stencil covers cutoff + skin: atoms binned at the last rebuild may
+        // have moved by up to half of it each
         const double rcs[NCUT] = {rccn_, rc2_, rc3_, atm_mode_ != 0 ? sur::RCF : 0.0};
         for (int cut = 0; cut < NCUT; ++cut) {
             d.st_off[cut] = (int)st_d_.size();
-            if (rcs[cut] > 0) make_stencil(C.H, d.nb, rcs[cut], st_d_, st_D_);
+            if (rcs[cut] > 0) make_stencil(C.H, d.nb, rcs[cut] + skin_, st_d_, st_D_);
             d.st_cnt[cut] = (int)st_d_.size() - d.st_off[cut];
         }
         for (int a = 0; a < (int)C.n; a += CHUNK) {
@@ -547,6 +588,22 @@ void Engine::ensure_atm_scratch() {
     }
     atm_cap_ = cap;
     atm_blocks_ = nb;
+    // forward lists: with a skin (kept across evaluations), or when centres are split into work
+    // items that keep every resident block busy several times over (each item would walk the
+    // stencil again: water box 62.1 -> 60.9 ms), or always when asked; within 4 GiB of entries.
+    // With fewer items the walks run side by side and a list pass before them only adds its own
+    // latency (512-atom Si: 1.95 -> 2.01 ms).
+    const bool many = atm_chunks_ > 1 && (uint64_t)n_ * atm_chunks_ >= 4 * (uint64_t)nb;
+    const bool want = fl_mode_ == 1 || (fl_mode_ < 0 && (skin_ > 0 || many));
+    const size_t fl_need = (size_t)std::max<uint64_t>(n_, 1) * (size_t)cap;
+    fl_use_ = want && fl_need * sizeof(FwdEntry) <= ((size_t)4 << 30);
+    if (fl_use_ && (fl_need > fl_alloc_ || !d_fl_n)) {
+        dfree(d_fl);
+        dfree(d_fl_n);
+        fl_alloc_ = fl_need;
+        d_fl = dalloc(fl_alloc_ * sizeof(FwdEntry));
+        d_fl_n = dalloc(max_len_ * sizeof(int));
+    }
     // the kernel indexes scratch as blockIdx.x * atm_cap_: reallocate whenever either grows
     if ((size_t)cap * nb > atm_alloc_entries_ || nb > atm_alloc_blocks_ || !d_atm_i32) {
         dfree(d_atm_i32);
@@ -626,6 +683,11 @@ Params<T> Engine::params() {
     P.tb_part = (double *)d_tb_part;
     P.atm_roff = (const int *)d_atm_roff;
     P.atm_yrows = yrows_;
+    P.atm_fl = fl_use_ ? (FwdEntry *)d_fl : nullptr;
+    P.atm_fl_n = (int *)d_fl_n;
+    P.atm_fl_cap = atm_cap_;
+    P.rc_fl2 = (T)((rc3_ + skin_) * (rc3_ + skin_));
+    P.shift = skin_ > 0 ? (double *)d_shift : nullptr;
     P.atm_i32 = (int *)d_atm_i32;
     P.atm_T = (T *)d_atm_T;
     P.atm_acc = (double *)d_atm_acc;
@@ -682,21 +744,39 @@ void Engine::run() {
     ensure_atm_scratch();
     ensure_tb_scratch();
     ensure_surrogate();
+    if (skin_ > 0 && !d_shift) d_shift = dalloc(3 * max_len_ * sizeof(double));
     Params<T> P = params<T>();
     feat_valid_ = mode_now() != D3X_ATM_EXACT;
+    // keep the neighbour state of the last rebuild when the atoms stayed within half the skin
+    // and the lists this evaluation needs were built then, with this capacity
+    const bool need_lists = fl_use_ && rc3_ > 0 && mode_now() != D3X_ATM_SURROGATE;
+    const bool reuse = reuse_now() && (!need_lists || nbr_fl_) && atm_cap_ == nbr_cap_;
+    if (reuse) {
+        ++nbr_reuses_;
+    } else {
+        ++nbr_rebuilds_;
+        if (skin_ > 0 && sys_.size() == 1) {
+            nbr_valid_ = true;
+            nbr_moved_far_ = false;
+            nbr_pos_ = h_pos_;
+            nbr_typ_ = h_typ_;
+            nbr_fl_ = need_lists;
+            nbr_cap_ = atm_cap_;
+        }
+    }
     if (graph_mode_ && !timing_) {
         // The graph holds every kernel argument and launch shape; re-capture when any changes.
-        std::vector<unsigned char> sig(sizeof(P) + 7 * sizeof(int64_t));
+        std::vector<unsigned char> sig(sizeof(P) + 8 * sizeof(int64_t));
         std::memcpy(sig.data(), &P, sizeof(P));
-        const int64_t extra[7] = {nbins_, atm_blocks_, (int64_t)chunk_b_.size(), (int64_t)sysdev_.size(),
-                                  (int64_t)(intptr_t)stream_, (int64_t)(intptr_t)d_tab, mode_now()};
+        const int64_t extra[8] = {nbins_, atm_blocks_, (int64_t)chunk_b_.size(), (int64_t)sysdev_.size(),
+                                  (int64_t)(intptr_t)stream_, (int64_t)(intptr_t)d_tab, mode_now(), reuse};
         std::memcpy(sig.data() + sizeof(P), extra, sizeof(extra));
         if (!gexec_ || sig != gsig_) {
             if (gexec_) (void)hipGraphExecDestroy(gexec_);
             gexec_ = nullptr;
             hipGraph_t g = nullptr;
             HIPCHK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-            enqueue<T>(P);
+            enqueue<T>(P, reuse);
             HIPCHK(hipStreamEndCapture(stream_, &g));
             const hipError_t ie = hipGraphInstantiate(&gexec_, g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
@@ -708,15 +788,18 @@ void Engine::run() {
         return;
     }
     if (timing_) harvest();
-    enqueue<T>(P);
+    enqueue<T>(P, reuse);
     if (timing_) timing_pending_ = true;
 }
 
 template <class T>
-void Engine::enqueue(const Params<T> &P) {
+void Engine::enqueue(const Params<T> &P, bool reuse) {
     launch_fill_u32((unsigned *)d_status, 1, 0u, stream_);
     mark(0);
-    Launch<T>::bin_sort(P, nbins_, (int *)d_bin_start, stream_);
+    if (reuse)
+        Launch<T>::regather(P, stream_);  // bins and order of the last rebuild
+    else
+        Launch<T>::bin_sort(P, nbins_, (int *)d_bin_start, stream_);
     mark(1);
     Launch<T>::cn(P, stream_);
     mark(2);
@@ -725,6 +808,7 @@ void Engine::enqueue(const Params<T> &P) {
     if (mode_now() == D3X_ATM_SURROGATE) {
         launch_surrogate<T>(P, stream_);
     } else {
+        if (P.atm_fl && !reuse) launch_atm_lists<T>(P, stream_);
         launch_atm<T>(P, atm_blocks_, stream_);  // zeroes the ATM arrays; kernels only when rc3 > 0
         if (mode_now() == D3X_ATM_EXACT_FEATURES) launch_sur_features<T>(P, stream_);
     }
@@ -843,8 +927,10 @@ void Engine::apply_tune_cache() {
 }
 
 double Engine::timed_evaluation() {
-    compute_device();  // builds geometry, sizes scratch
+    nbr_valid_ = false;  // every candidate timed with its neighbour search
+    compute_device();    // builds geometry, sizes scratch
     set_timing(true);
+    nbr_valid_ = false;
     compute_device();
     double ms[NPHASE];
     phase_times(ms);
@@ -1101,6 +1187,7 @@ void Engine::compute_device() {
     for (const Cell &c : sys_)
         if (!c.set) fail(D3X_ERR_STATE, "set_cell has not been called");
     if (geom_dirty_) {
+        nbr_valid_ = false;  // new bins or stencils
         if (!tuning_explicit_) {
             bin_width_ = default_bin_width();
             apply_tune_cache();

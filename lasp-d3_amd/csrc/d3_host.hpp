@@ -53,8 +53,27 @@ class Engine {
     void set_bin_width(double w) { bin_width_ = w; geom_dirty_ = true; }
     double bin_width() const { return bin_width_; }
     void set_atm_blocks(int nb) { atm_blocks_req_ = nb; }
-    // Quadruples the three-body neighbour capacity (after a D3X_ERR_OVERFLOW).
-    void grow_atm_cap() { atm_cap_ = std::max(atm_cap_, 1024) * 4; }
+    // Quadruples the three-body neighbour capacity (after a D3X_ERR_OVERFLOW); kept neighbour
+    // lists of the old capacity are rebuilt.
+    void grow_atm_cap() {
+        atm_cap_ = std::max(atm_cap_, 1024) * 4;
+        nbr_valid_ = false;
+    }
+
+    // Neighbour state across evaluations (MD steps).  skin > 0 (bohr): stencils and three-body
+    // forward lists cover every cutoff plus the skin, and an evaluation after set_atoms with the
+    // same atoms and cell reuses the last bin order and lists while no atom has moved more than
+    // skin / 2 from where it was at the last rebuild.  lists: -1 = forward lists when the skin is
+    // set or centres are split into more work items than four per resident block, 0 = never,
+    // 1 = always (within a 4 GiB budget).  Counters: evaluations that rebuilt / reused the neighbour state.
+    void set_skin(double skin);
+    double skin() const { return skin_; }
+    void set_neighbour_lists(int mode);
+    int neighbour_lists() const { return fl_mode_; }
+    void neighbour_stats(int &rebuilds, int &reuses) const {
+        rebuilds = nbr_rebuilds_;
+        reuses = nbr_reuses_;
+    }
     int atm_cap() const { return atm_cap_; }
     const std::vector<SysDev> &sysdev() const { return sysdev_; }
     int stencil_count(int cut) const;
@@ -120,7 +139,7 @@ class Engine {
     template <class T>
     void run();
     template <class T>
-    void enqueue(const Params<T> &P);
+    void enqueue(const Params<T> &P, bool reuse);
     void build_geometry();
     void upload_geometry();
     void alloc_atoms();
@@ -129,6 +148,23 @@ class Engine {
     double default_bin_width() const;
     void ensure_surrogate();
     int type_of(uint16_t Z) const;
+
+    // neighbour state (set_skin)
+    double skin_ = 0.0;
+    int fl_mode_ = -1;
+    bool fl_use_ = false;           // forward lists in use for the next evaluation
+    bool nbr_valid_ = false;        // bin order and lists of the last rebuild are usable
+    bool nbr_moved_far_ = true;     // an atom moved more than skin / 2 since the last rebuild
+    std::vector<double> h_pos_;     // bohr, caller order, of the last set_atoms
+    std::vector<uint16_t> h_typ_;   // types of the last set_atoms
+    std::vector<double> nbr_pos_;   // positions at the last rebuild
+    std::vector<uint16_t> nbr_typ_;
+    int nbr_rebuilds_ = 0, nbr_reuses_ = 0;
+    bool nbr_fl_ = false;  // forward lists were built at the last rebuild
+    int nbr_cap_ = 0;      // with this capacity
+    void *d_fl = nullptr, *d_fl_n = nullptr, *d_shift = nullptr;
+    size_t fl_alloc_ = 0;
+    bool reuse_now() const { return skin_ > 0 && nbr_valid_ && !nbr_moved_far_ && sys_.size() == 1; }
 
     // three-body stand-in state
     int atm_mode_ = 0, judge_every_ = 0, fallbacks_ = 0;

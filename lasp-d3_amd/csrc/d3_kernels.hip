@@ -32,6 +32,20 @@ __global__ void k_fill_i32(int *p, int n, int v) {
     if (t < n) p[t] = v;
 }
 
+// Wrapped coordinate c of position r with lattice shifts fl, and the position relative to the
+// centre of bin (b0, b1, b2): the expressions of the bin sort, shared with the regather of moved
+// positions so that an unmoved atom lands on the same bits.
+__device__ __forceinline__ double wrapped_coord(const double r[3], const double fl[3], const SysDev &S, int c) {
+    return r[c] - fl[0] * S.H[0 * 3 + c] - fl[1] * S.H[1 * 3 + c] - fl[2] * S.H[2 * 3 + c];
+}
+__device__ __forceinline__ void local_coords(const double w[3], int key, const SysDev &S, double l[3]) {
+    const int b = key - S.bin_base;
+    const int b0 = b % S.nb[0], b1 = (b / S.nb[0]) % S.nb[1], b2 = b / (S.nb[0] * S.nb[1]);
+    const double u0 = (b0 + 0.5) / S.nb[0], u1 = (b1 + 0.5) / S.nb[1], u2 = (b2 + 0.5) / S.nb[2];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) l[c] = w[c] - (u0 * S.H[0 * 3 + c] + u1 * S.H[1 * 3 + c] + u2 * S.H[2 * 3 + c]);
+}
+
 template <class T>
 __global__ void k_bin(Params<T> P) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -54,8 +68,9 @@ __global__ void k_bin(Params<T> P) {
         b[k] = bk;
     }
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
-        P.wrapped[3 * i + c] = r[c] - fl[0] * S.H[0 * 3 + c] - fl[1] * S.H[1 * 3 + c] - fl[2] * S.H[2 * 3 + c];
+    for (int c = 0; c < 3; ++c) P.wrapped[3 * i + c] = wrapped_coord(r, fl, S, c);
+    if (P.shift)
+        for (int k = 0; k < 3; ++k) P.shift[3 * i + k] = fl[k];
     const int key = S.bin_base + b[0] + S.nb[0] * (b[1] + S.nb[1] * b[2]);
     P.key_in[i] = key;
     atomicAdd(&P.bin_count[key], 1);
@@ -118,13 +133,9 @@ __global__ void k_gather(Params<T> P) {
     const int key = P.key_in[i];
     const int sid = P.sys_in[i];
     const SysDev &S = P.sys[sid];
-    const int b = key - S.bin_base;
-    const int b0 = b % S.nb[0], b1 = (b / S.nb[0]) % S.nb[1], b2 = b / (S.nb[0] * S.nb[1]);
-    const double u0 = (b0 + 0.5) / S.nb[0], u1 = (b1 + 0.5) / S.nb[1], u2 = (b2 + 0.5) / S.nb[2];
+    const double w[3] = {P.wrapped[3 * i], P.wrapped[3 * i + 1], P.wrapped[3 * i + 2]};
     double l[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-        l[c] = P.wrapped[3 * i + c] - (u0 * S.H[0 * 3 + c] + u1 * S.H[1 * 3 + c] + u2 * S.H[2 * 3 + c]);
+    local_coords(w, key, S, l);
     P.lx[p] = (T)l[0];
     P.ly[p] = (T)l[1];
     P.lz[p] = (T)l[2];
@@ -132,6 +143,25 @@ __global__ void k_gather(Params<T> P) {
     P.orig_s[p] = i;
     P.bin_s[p] = key;
     P.sys_s[p] = sid;
+}
+
+// Moved positions of the same atoms into the kept order: the wrap shifts and bins of the last
+// sort, the new caller positions (atoms within half the skin of where they were binned).
+template <class T>
+__global__ void k_regather(Params<T> P) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P.n) return;
+    const int i = P.orig_s[p];
+    const SysDev &S = P.sys[P.sys_s[p]];
+    const double r[3] = {P.pos_in[3 * i], P.pos_in[3 * i + 1], P.pos_in[3 * i + 2]};
+    const double fl[3] = {P.shift[3 * i], P.shift[3 * i + 1], P.shift[3 * i + 2]};
+    double w[3], l[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) w[c] = wrapped_coord(r, fl, S, c);
+    local_coords(w, P.bin_s[p], S, l);
+    P.lx[p] = (T)l[0];
+    P.ly[p] = (T)l[1];
+    P.lz[p] = (T)l[2];
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -567,6 +597,11 @@ void Launch<T>::bin_sort(const Params<T> &P, int nbins, int *bin_start_mut, hipS
         hipLaunchKernelGGL(k_canon, dim3(nbins), dim3(tb), 0, s, P.bin_start, P.perm, P.perm2);
         hipLaunchKernelGGL(k_gather<T>, dim3((P.n + tb - 1) / tb), dim3(tb), 0, s, P);
     }
+}
+
+template <class T>
+void Launch<T>::regather(const Params<T> &P, hipStream_t s) {
+    if (P.n > 0) hipLaunchKernelGGL(k_regather<T>, dim3((P.n + 255) / 256), dim3(256), 0, s, P);
 }
 
 template <class T>

@@ -68,6 +68,9 @@ SIGNATURES = {
     "d3x_get_tuning": (C.c_int, [_vp, _dp, _ip]),
     "d3x_get_atm_variant": (C.c_int, [_vp]),
     "d3x_get_atm_capacity": (C.c_int, [_vp]),
+    "d3x_set_skin": (C.c_int, [_vp, C.c_double]),
+    "d3x_set_neighbour_lists": (C.c_int, [_vp, C.c_int]),
+    "d3x_neighbour_stats": (C.c_int, [_vp, _ip, _ip]),
     "d3x_autotune": (C.c_double, [_vp, C.c_char_p]),
     "d3x_get_stream": (_vp, [_vp]),
     "d3x_set_stream": (C.c_int, [_vp, _vp]),
@@ -257,6 +260,20 @@ class Handle:
     def atm_capacity(self) -> int:
         """Forward-neighbour capacity per centre of the three-body scratch."""
         return int(self.lib.d3x_get_atm_capacity(self.h))
+
+    def set_skin(self, skin_bohr: float):
+        """Keep the neighbour state across evaluations while atoms move less than skin / 2."""
+        self._check(self.lib.d3x_set_skin(self.h, float(skin_bohr)), "d3x_set_skin")
+
+    def set_neighbour_lists(self, mode: int):
+        """Three-body forward lists: -1 automatic, 0 never, 1 always."""
+        self._check(self.lib.d3x_set_neighbour_lists(self.h, int(mode)), "d3x_set_neighbour_lists")
+
+    def neighbour_stats(self):
+        """(evaluations that rebuilt the neighbour state, evaluations that reused it)"""
+        a, b = C.c_int(), C.c_int()
+        self._check(self.lib.d3x_neighbour_stats(self.h, C.byref(a), C.byref(b)), "d3x_neighbour_stats")
+        return a.value, b.value
 
     def autotune(self, cache_path: str | None = None) -> float:
         t = self.lib.d3x_autotune(self.h, cache_path.encode() if cache_path else None)
