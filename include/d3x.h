@@ -92,6 +92,12 @@ int d3x_set_tuning(d3_handle_t *handle, double bin_width_bohr, int atm_blocks);
 int d3x_get_tuning(d3_handle_t *handle, double *bin_width_bohr, int *atm_blocks);
 int d3x_get_atm_variant(d3_handle_t *handle);
 
+/* Smooth CN cutoff (the reference's SMOOTH_CUTOFF_CN build option, src/d3_kernel.cu:550-561,
+ * off in its default build): on != 0 multiplies every CN term by tanh^3(rc_CN - r) (bohr), with
+ * its derivative in the chain-rule forces and stress.  D3_SMOOTH_CN=1 in the environment sets it
+ * for handles made by the reference-compatible init functions. */
+int d3x_set_cn_smooth(d3_handle_t *handle, int on);
+
 /* Neighbour state across evaluations, for MD loops that call set_atoms with slightly moved atoms
  * (no counterpart in the reference, which sorts its cell list again at every compute,
  * src/d3.cu:141-143).  skin_bohr > 0: every stencil covers its cutoff plus the skin, and an

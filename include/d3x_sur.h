@@ -4,9 +4,11 @@
  * (src/d3_kernel.cu:1199-1509, reached through compute_dispersion_energy_from_handle in
  * src/d3.h:114-128).  The stand-in is a per-atom network: features x_i[64] = {CN_i, 63 Gaussian
  * radial sums over neighbours grouped in 3 element classes (21 centres 1..8 A, sigma 0.5 A,
- * smooth cutoff 9.5 A)}; per class an MLP 64 -> 128 -> 128 -> 1 (SiLU) whose two hidden layers
- * and input-gradient backward run on bf16 MFMA with fp32 accumulation; E = sum_i E_i, forces and
- * stress by the hand-written backward through the features and CN.
+ * smooth cutoff 9.5 A)}, each neighbour weighted by sqrt(C6_ij) C6_jj^(1/4) (D3 C6 at the atoms'
+ * CNs) times (1 + 6 ((4/3) R0_ij / r_ij)^16)^(-1/2) (R0 from r0ab); per class an MLP
+ * 64 -> 128 -> 128 -> 1 (SiLU) whose two hidden layers and input-gradient backward run on bf16
+ * MFMA with fp32 accumulation; E = sum_i E_i, forces and stress by the hand-written backward
+ * through the features, their weights and CN.
  *
  * Parameter vector (D3X_SUR_NPARAMS floats): mean[64] | istd[64] | per class c = 0, 1, 2:
  * W1[64][128] | b1[128] | W2[128][128] | b2[128] | w3[128] | b3, scale, shift, pad;
@@ -95,6 +97,12 @@ int d3x_sur_train_grad_noalloc(const float *params, const void *packed, int cls,
  * lambda L.  D summed over a cell's rows is -sum F_net . (F_net - F_ref). */
 int d3x_sur_train_force_grad(const float *params, const void *packed, int cls, uint64_t n, const float *X,
                              const float *U, const float *w, float *D, void *work, float *grad, void *stream);
+/* Tied training (one network for every element class): each trainable entry's gradient becomes
+ * the sum over the three class blocks; blocks != 0 also sums the radial rows of W1 over the three
+ * neighbour-class blocks (rows 1 + 21 c + m for c = 0, 1, 2), so that the network sees the
+ * neighbour sums of all classes through one set of weights.  Identical copies stay identical
+ * under d3x_sur_adam. */
+int d3x_sur_tie_grad(float *grad, int blocks, void *stream);
 /* Adam step (bias-corrected, step >= 1) on the trainable entries; clears grad */
 int d3x_sur_adam(float *params, float *grad, float *m, float *v, float lr, float beta1, float beta2, float eps,
                  int step, void *stream);

@@ -129,11 +129,14 @@ d3_handle_t *make_handle(const uint16_t *elements, uint64_t ne, uint64_t max_len
 }
 
 // Options of reference-compatible handles from the environment: D3_GRAPH=1 evaluates through a
-// captured hipGraph, D3_SKIN=<bohr> keeps the neighbour state across MD steps.
+// captured hipGraph, D3_SMOOTH_CN=1 the smooth CN cutoff, D3_SKIN=<bohr> keeps the neighbour
+// state across MD steps.
 d3_handle_t *with_env_graph(d3_handle_t *h) {
     if (!h) return h;
     const char *g = std::getenv("D3_GRAPH");
     if (g && std::strcmp(g, "1") == 0) H(h)->eng->set_graph(true);
+    const char *sm = std::getenv("D3_SMOOTH_CN");
+    if (sm && std::strcmp(sm, "1") == 0) H(h)->eng->set_cn_smooth(true);
     const char *sk = std::getenv("D3_SKIN");
     if (sk && *sk) {
         const double v = std::atof(sk);
@@ -325,6 +328,12 @@ int d3x_get_tuning(d3_handle_t *handle, double *bin_width_bohr, int *atm_blocks)
     if (!handle) return (int)(D3X_ERR_GENERIC | D3X_ERR_INPUT);
     if (bin_width_bohr) *bin_width_bohr = H(handle)->eng->bin_width();
     if (atm_blocks) *atm_blocks = H(handle)->eng->atm_blocks_effective();
+    return 0;
+}
+
+int d3x_set_cn_smooth(d3_handle_t *handle, int on) {
+    if (!handle) return (int)(D3X_ERR_GENERIC | D3X_ERR_INPUT);
+    H(handle)->eng->set_cn_smooth(on != 0);
     return 0;
 }
 

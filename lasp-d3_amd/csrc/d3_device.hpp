@@ -50,6 +50,17 @@ __device__ __forceinline__ double xcbrt(double x) { return cbrt(x); }
 __device__ __forceinline__ float xcbrt(float x) { return cbrtf(x); }
 __device__ __forceinline__ double xrsqrt(double x) { return rsqrt(x); }
 __device__ __forceinline__ float xrsqrt(float x) { return rsqrtf(x); }
+__device__ __forceinline__ double xtanh(double x) { return tanh(x); }
+__device__ __forceinline__ float xtanh(float x) { return tanhf(x); }
+
+// Smooth CN cutoff, src/d3_kernel.cu:550-561 (SMOOTH_CUTOFF_CN): the CN term c times
+// s = tanh^3(rc - r); c'/r (dcn) becomes (c s)'/r = (c'/r) s - 3 tanh^2 (1 - tanh^2) c / r.
+template <class T>
+__device__ __forceinline__ void smooth_cn_term(T rc, T r, T ir, T &c, T &dcn) {
+    const T th = xtanh(rc - r), th2 = th * th;
+    dcn = dcn * th2 * th - T(3) * th2 * (T(1) - th2) * c * ir;
+    c *= th2 * th;
+}
 template <class T>
 __device__ __forceinline__ T finite_or_zero(T x) {
     return isfinite(x) ? x : T(0);

@@ -59,6 +59,8 @@ struct Params {
     int nsys;
     int damping;
     T rc_cn2, rc_22, rc_32;  // squared cutoffs (bohr^2)
+    T rc_cn;                 // CN cutoff (bohr), for the smooth cutoff
+    int cn_smooth;           // CN terms times tanh^3(rc_cn - r) (the reference's SMOOTH_CUTOFF_CN)
     T s6, s8, p1, p2;        // s6, s8 and (sr6, sr8) or (a1, a2)
     // structure
     const SysDev *sys;
@@ -133,6 +135,9 @@ struct Params {
     const int *type_cls;
     float *feat, *sur_G, *sur_E;
     int *sur_cls;
+    // per atom (sorted order): q = C6_ii^(1/4) at its CN and dq/dCN (sur_q[i], sur_q[n + i]),
+    // the neighbour-weight factors of the features; the tangent pass's CN push-forward sur_dcn[i]
+    float *sur_q, *sur_dcn;
     const float *sur_P;
     const void *sur_PK;
     unsigned *status;

@@ -240,7 +240,7 @@ Engine::~Engine() {
                     &d_sys, &d_st_d, &d_st_D, &d_chunk_b, &d_chunk_e, &d_chunk_s, &d_sys_chunk, &d_partial,
                     &d_out_sys, &d_tab, &d_status, &d_atm_i32, &d_atm_T, &d_atm_acc, &d_atm_pa, &d_type_cls,
                     &d_feat, &d_sur_G, &d_sur_E, &d_sur_cls, &d_sur_P, &d_sur_PK, &d_tan_dir, &d_tan_U, &d_atm_roff, &d_atm_eatc, &d_tb_part,
-                    &d_fl, &d_fl_n, &d_shift};
+                    &d_fl, &d_fl_n, &d_shift, &d_sur_q, &d_sur_dcn};
     for (void **p : all) dfree(*p);
     for (auto &e : ev_)
         if (e) (void)hipEventDestroy(e);
@@ -630,6 +630,8 @@ Params<T> Engine::params() {
     P.rc_cn2 = (T)(rccn_ * rccn_);
     P.rc_22 = (T)(rc2_ * rc2_);
     P.rc_32 = (T)(rc3_ * rc3_);
+    P.rc_cn = (T)rccn_;
+    P.cn_smooth = cn_smooth_ ? 1 : 0;
     P.s6 = (T)fp_[0];
     if (damping_ == 0) {
         P.s8 = (T)fp_[1];
@@ -703,6 +705,8 @@ Params<T> Engine::params() {
     P.sur_G = (float *)d_sur_G;
     P.sur_E = (float *)d_sur_E;
     P.sur_cls = (int *)d_sur_cls;
+    P.sur_q = (float *)d_sur_q;
+    P.sur_dcn = (float *)d_sur_dcn;
     P.sur_P = (const float *)d_sur_P;
     P.sur_PK = d_sur_PK;
     return P;
@@ -1057,6 +1061,7 @@ void Engine::ensure_surrogate() {
         d_sur_G = dalloc(max_len_ * sur::NFEAT * sizeof(float));
         d_sur_E = dalloc(max_len_ * sizeof(float));
         d_sur_cls = dalloc(max_len_ * sizeof(int));
+        d_sur_q = dalloc(2 * max_len_ * sizeof(float));
     }
     if (sur_dirty_ && !h_sur_P_.empty()) {
         if (!d_sur_P) {
@@ -1127,6 +1132,7 @@ void Engine::sur_tangent(const double *dir, float *U) {
     if (!d_tan_dir) {
         d_tan_dir = dalloc(max_len_ * 3 * sizeof(float));
         d_tan_U = dalloc(max_len_ * sur::NFEAT * sizeof(float));
+        d_sur_dcn = dalloc(max_len_ * sizeof(float));
     }
     HIPCHK(hipMemcpyAsync(d_tan_dir, ds.data(), ds.size() * sizeof(float), hipMemcpyHostToDevice, stream_));
     if (precision_ == D3X_FP64)

@@ -66,6 +66,10 @@ class Engine {
     // skin / 2 from where it was at the last rebuild.  lists: -1 = forward lists when the skin is
     // set or centres are split into more work items than four per resident block, 0 = never,
     // 1 = always (within a 4 GiB budget).  Counters: evaluations that rebuilt / reused the neighbour state.
+    // Smooth CN cutoff (the reference's SMOOTH_CUTOFF_CN build option): CN terms times
+    // tanh^3(rc_CN - r).
+    void set_cn_smooth(bool on) { cn_smooth_ = on; }
+    bool cn_smooth() const { return cn_smooth_; }
     void set_skin(double skin);
     double skin() const { return skin_; }
     void set_neighbour_lists(int mode);
@@ -149,6 +153,7 @@ class Engine {
     void ensure_surrogate();
     int type_of(uint16_t Z) const;
 
+    bool cn_smooth_ = false;
     // neighbour state (set_skin)
     double skin_ = 0.0;
     int fl_mode_ = -1;
@@ -174,7 +179,7 @@ class Engine {
     std::vector<float> h_sur_P_;
     void *d_type_cls = nullptr, *d_feat = nullptr, *d_sur_G = nullptr, *d_sur_E = nullptr, *d_sur_cls = nullptr,
          *d_sur_P = nullptr, *d_sur_PK = nullptr, *d_tan_dir = nullptr, *d_tan_U = nullptr, *d_atm_roff = nullptr,
-         *d_atm_eatc = nullptr, *d_tb_part = nullptr;
+         *d_atm_eatc = nullptr, *d_tb_part = nullptr, *d_sur_q = nullptr, *d_sur_dcn = nullptr;
 
     // configuration
     std::vector<uint16_t> elems_;  // sorted unique atomic numbers of the handle

@@ -199,7 +199,12 @@ __global__ __launch_bounds__(BLOCK) void k_cn(Params<T> P) {
                 if (r2 <= P.rc_cn2 && r2 > T(0)) {
                     const T r = xsqrt(r2);
                     const T ex = xexp(-T(K1) * ((rci + P.rcov[P.type_s[j]]) / r - T(1)));
-                    acc.add(T(1) / (T(1) + ex));
+                    T c = T(1) / (T(1) + ex);
+                    if (P.cn_smooth) {
+                        T unused = T(0);
+                        smooth_cn_term(P.rc_cn, r, T(1) / r, c, unused);
+                    }
+                    acc.add(c);
                     cnt += 1.0f;
                 }
             }
@@ -558,7 +563,11 @@ __global__ __launch_bounds__(BLOCK) void k_chain(Params<T> P) {
                     const T rr = rci + P.rcov[P.type_s[j]];
                     const T ex = xexp(-T(K1) * (rr * ir - T(1)));
                     const T ope = T(1) + ex;
-                    const T dcn = -T(K1) * ex * rr * ir * ir * ir / (ope * ope);
+                    T dcn = -T(K1) * ex * rr * ir * ir * ir / (ope * ope);
+                    if (P.cn_smooth) {
+                        T c = T(1) / ope;
+                        smooth_cn_term(P.rc_cn, r2 * ir, ir, c, dcn);
+                    }
                     const T sc = finite_or_zero((gi + (T)P.g[j]) * dcn);
                     F[0].add(sc * dx);
                     F[1].add(sc * dy);

@@ -609,6 +609,30 @@ __global__ __launch_bounds__(64) void k_wgrad_tan(const float *__restrict__ P, i
     }
 }
 
+// Tied training: parameter k of class 0's block stands for its copies in the other classes and,
+// for a radial row of W1 (row 1 + 21 c + m), for the rows of the same radial centre m in the
+// other neighbour-class blocks; the gradient of every copy becomes the sum over the copies.
+__global__ void k_tie(float *__restrict__ grad, int blocks) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    // owners: the W1 rows (with blocks only CN's and the first block's), then every entry from b1 on
+    const int nw1 = (blocks ? 1 + NRBF : NFEAT) * NHID, nrest = CLASS_SIZE - O_B1;
+    if (t >= nw1 + nrest) return;
+    const int k = t < nw1 ? t : O_B1 + (t - nw1);
+    const int row = t < nw1 ? t / NHID : -1, col = t < nw1 ? t % NHID : 0;
+    const int nb = (blocks && row >= 1) ? 3 : 1;
+    float s = 0.f;
+    for (int c = 0; c < NCLASS; ++c)
+        for (int b = 0; b < nb; ++b) {
+            const int kk = b == 0 ? k : O_W1 + (row + NRBF * b) * NHID + col;
+            s += grad[HEAD + c * CLASS_SIZE + kk];
+        }
+    for (int c = 0; c < NCLASS; ++c)
+        for (int b = 0; b < nb; ++b) {
+            const int kk = b == 0 ? k : O_W1 + (row + NRBF * b) * NHID + col;
+            grad[HEAD + c * CLASS_SIZE + kk] = s;
+        }
+}
+
 // Adam on the trainable entries (W1, b1, W2, b2, w3, b3 of every class); clears the gradient.
 __global__ void k_adam(float *__restrict__ P, float *__restrict__ grad, float *__restrict__ m, float *__restrict__ v,
                        float lr, float b1, float b2, float eps, float c1, float c2) {
@@ -677,6 +701,11 @@ void launch_train_force(const float *P, const void *PK, int c, int n, const floa
     const int tpc = 64;
     hipLaunchKernelGGL(k_wgrad_tan, dim3(96, (ntile + tpc - 1) / tpc), dim3(64), 0, s, P, c, n, X, U,
                        (const v4s *)work, tpc, grad);
+}
+
+void launch_tie(float *grad, int blocks, hipStream_t s) {
+    const int n = (blocks ? 1 + NRBF : NFEAT) * NHID + (CLASS_SIZE - O_B1);
+    hipLaunchKernelGGL(k_tie, dim3((n + 255) / 256), dim3(256), 0, s, grad, blocks);
 }
 
 void launch_adam(float *P, float *grad, float *m, float *v, float lr, float b1, float b2, float eps, int step,

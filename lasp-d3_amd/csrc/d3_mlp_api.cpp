@@ -95,6 +95,12 @@ int d3x_sur_train_force_grad(const float *params, const void *packed, int cls, u
     return launched("d3x_sur_train_force_grad");
 }
 
+int d3x_sur_tie_grad(float *grad, int blocks, void *stream) {
+    if (!grad) return bad("d3x_sur_tie_grad", "NULL buffer");
+    launch_tie(grad, blocks != 0 ? 1 : 0, (hipStream_t)stream);
+    return launched("d3x_sur_tie_grad");
+}
+
 int d3x_sur_adam(float *params, float *grad, float *m, float *v, float lr, float beta1, float beta2, float eps,
                  int step, void *stream) {
     if (!params || !grad || !m || !v) return bad("d3x_sur_adam", "NULL buffer");

@@ -3,7 +3,8 @@
 // has only the exact ATM kernel (src/d3_kernel.cu:1199-1509).
 //
 // Per atom i: features x_i[64] = {CN_i, 63 Gaussian radial sums} (21 centres 1..8 A, sigma 0.5 A,
-// over neighbours of each of 3 element classes, smooth cosine cutoff at 9.5 A), then the network
+// over neighbours of each of 3 element classes, smooth cosine cutoff at 9.5 A, each neighbour
+// weighted by sqrt(C6_ij) C6_jj^(1/4) times the edge damping h_ij), then the network
 // of the atom's own class: x' = (x - mean) * istd, h1 = silu(W1^T x' + b1), h2 = silu(W2^T h1 + b2),
 // E_i = scale (w3 . h2 + b3) + shift (eV).
 //
@@ -63,6 +64,9 @@ void launch_train_force(const float *P, const void *PK, int c, int n, const floa
                         float *D, void *work, float *grad, hipStream_t s);
 void launch_adam(float *P, float *grad, float *m, float *v, float lr, float b1, float b2, float eps, int step,
                  hipStream_t s);
+// Tied training: every class's gradient entry := the sum over the classes; with blocks, the
+// radial rows of W1 summed over the three neighbour-class blocks as well.
+void launch_tie(float *grad, int blocks, hipStream_t s);
 
 }  // namespace sur
 }  // namespace d3

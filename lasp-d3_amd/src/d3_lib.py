@@ -68,6 +68,7 @@ SIGNATURES = {
     "d3x_get_tuning": (C.c_int, [_vp, _dp, _ip]),
     "d3x_get_atm_variant": (C.c_int, [_vp]),
     "d3x_get_atm_capacity": (C.c_int, [_vp]),
+    "d3x_set_cn_smooth": (C.c_int, [_vp, C.c_int]),
     "d3x_set_skin": (C.c_int, [_vp, C.c_double]),
     "d3x_set_neighbour_lists": (C.c_int, [_vp, C.c_int]),
     "d3x_neighbour_stats": (C.c_int, [_vp, _ip, _ip]),
@@ -97,6 +98,7 @@ SIGNATURES = {
     "d3x_sur_train_grad": (C.c_int, [_vp, _vp, C.c_int, _u64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "d3x_sur_train_grad_noalloc": (C.c_int, [_vp, _vp, C.c_int, _u64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "d3x_sur_train_force_grad": (C.c_int, [_vp, _vp, C.c_int, _u64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "d3x_sur_tie_grad": (C.c_int, [_vp, C.c_int, _vp]),
     "d3x_sur_adam": (C.c_int, [_vp, _vp, _vp, _vp, _f, _f, _f, _f, C.c_int, _vp]),
 }
 
@@ -260,6 +262,10 @@ class Handle:
     def atm_capacity(self) -> int:
         """Forward-neighbour capacity per centre of the three-body scratch."""
         return int(self.lib.d3x_get_atm_capacity(self.h))
+
+    def set_cn_smooth(self, on: bool = True):
+        """The smooth CN cutoff (tanh^3(rc_CN - r) on every CN term)."""
+        self._check(self.lib.d3x_set_cn_smooth(self.h, int(bool(on))), "d3x_set_cn_smooth")
 
     def set_skin(self, skin_bohr: float):
         """Keep the neighbour state across evaluations while atoms move less than skin / 2."""

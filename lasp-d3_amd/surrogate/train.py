@@ -7,13 +7,20 @@ library's own kernels (include/d3x_sur.h: d3x_sur_pack / d3x_sur_train_grad / d3
    (default within +-3 %); the handle runs PBE-D3(BJ) with s6 = s8 = 0,
    so energies, forces and per-atom energies are those of the three-body term alone, in the mode
    that also returns the network's features.
-2. One network per element class (H; Li..Ne; heavier), fitted jointly to per-atom three-body
-   energies and to the frames' three-body forces with Adam: each step takes random batches of
-   atoms per class for the energy loss, and the force loss of one random training frame -- the
-   network's forces on it from the engine, dF = F_net - F_exact, u = J dF through the features
-   (d3x_sur_tangent) and the tangent-pass gradient of the network (d3x_sur_train_force_grad).
-   Inputs normalised by the training set's feature statistics, outputs by each class's label
-   mean and spread, forces by each kind's RMS exact force.
+2. The networks are fitted jointly to per-atom three-body energies and to the frames' three-body
+   forces with Adam: each step takes random batches of atoms per class for the energy loss, and
+   the force loss of one random training frame -- the network's forces on it from the engine,
+   dF = F_net - F_exact, u = J dF through the features (d3x_sur_tangent) and the tangent-pass
+   gradient of the network (d3x_sur_train_force_grad).  Forces normalised by each kind's RMS exact
+   force.  --tie all (default): one network for every element class, whose first layer sees the
+   three neighbour-class blocks of each radial centre through the same weights (gradients summed
+   over the copies by d3x_sur_tie_grad), i.e. a function of CN and of the C6-weighted radial
+   densities of all neighbours -- the training set holds H, O and Si centres only, and a block or
+   a class network that no training atom exercises would answer arbitrarily for C, N or Cu;
+   inputs normalised per radial centre by the statistics of the summed density, outputs by the
+   label mean and spread of all atoms.  --tie classes: one network, neighbour blocks apart;
+   --tie none: one network per element class (H; Li..Ne; heavier), each normalised by its own
+   feature and label statistics.
 3. Validation on held-out frames (every 20th): per-atom and per-frame energy errors, and through
    the engine: the judge's |dE| / atoms, force errors against the exact kernel, and the time of
    the three-body phase (exact kernel vs network).
@@ -188,8 +195,16 @@ def train(args, X, y, Zall, fid, frames):
     vp = lambda t: C.c_void_p(t.data_ptr())  # noqa: E731
     cls = d3_sur.element_class(Zall)
     train_rows = (fid % HOLDOUT) != HOLDOUT - 1
-    mean = X[train_rows].mean(0).astype(np.float64)
-    std = X[train_rows].std(0).astype(np.float64)
+    nr = d3_sur.NRBF
+    Xt = X[train_rows].astype(np.float64)
+    mean = Xt.mean(0)
+    std = Xt.std(0)
+    if args.tie == "all":
+        # one weight per radial centre for the three neighbour blocks: normalise the summed density
+        dens = Xt[:, 1:1 + nr] + Xt[:, 1 + nr:1 + 2 * nr] + Xt[:, 1 + 2 * nr:1 + 3 * nr]
+        for b in range(3):
+            mean[1 + b * nr:1 + (b + 1) * nr] = dens.mean(0) / 3.0
+            std[1 + b * nr:1 + (b + 1) * nr] = dens.std(0)
     istd = 1.0 / np.maximum(std, 1e-3 * std.max())
     scale, shift = np.ones(3), np.zeros(3)
     present = []
@@ -199,7 +214,19 @@ def train(args, X, y, Zall, fid, frames):
             present.append(c)
             shift[c] = y[m].mean()
             scale[c] = max(y[m].std(), 1e-7)
+    if args.tie != "none":  # one network: one output scale
+        shift[:] = y[train_rows].mean()
+        scale[:] = max(y[train_rows].std(), 1e-7)
     p = d3_sur.init_params(args.seed, mean, istd, scale, shift)
+    if args.tie != "none":  # identical copies (they stay identical: tied gradients, same Adam state)
+        cs = d3_sur.CLASS_SIZE
+        blk = p[d3_sur.HEAD:d3_sur.HEAD + cs].copy()
+        if args.tie == "all":
+            W1 = blk[:d3_sur.NFEAT * d3_sur.NHID].reshape(d3_sur.NFEAT, d3_sur.NHID)
+            W1[1 + nr:1 + 2 * nr] = W1[1:1 + nr]
+            W1[1 + 2 * nr:1 + 3 * nr] = W1[1:1 + nr]
+        for c in range(3):
+            p[d3_sur.HEAD + c * cs:d3_sur.HEAD + (c + 1) * cs] = blk
     dev = torch.device("cuda")
     P = torch.from_numpy(p).to(dev)
     Xd = torch.from_numpy(X).to(dev)
@@ -232,6 +259,8 @@ def train(args, X, y, Zall, fid, frames):
                                           vp(loss), st)
         if fl is not None and step % args.force_every == 0:
             chk |= fl.step(P, pk)
+        if args.tie != "none":
+            chk |= lib.d3x_sur_tie_grad(vp(grad), 1 if args.tie == "all" else 0, st)
         frac = step / args.steps
         lr = args.lr * (0.5 * (1 + np.cos(np.pi * frac))) * min(1.0, step / 200) + 1e-6
         chk |= lib.d3x_sur_adam(vp(P), vp(grad), vp(m1), vp(m2), C.c_float(lr), C.c_float(0.9), C.c_float(0.999),
@@ -274,22 +303,35 @@ def engine_checks(p, cases):
         h.set_timing(True)
         e_x, f_x, s_x = h.compute()
         t_x = h.phase_times()[0]["atm"]
+        ea_x = h.atom_data()[2]
         h.sur_load(p)
         h.set_atm_mode(d3_lib.D3X_ATM_SURROGATE, 0, 1e-3)
         h.set_timing(True)
         e_n, f_n, s_n = h.compute()  # judged: network and exact
         mode, fb, dev_ = h.sur_state()
         h.set_timing(True)
+        h.set_atm_mode(d3_lib.D3X_ATM_SURROGATE, 0, 1e9)
+        h.set_judge_force_tol(0.0)
+        h.compute()  # judged with open tolerances: accepted
+        h.set_timing(True)
         e_n2, f_n2, s_n2 = h.compute()  # not judged: the network alone
         t_n = h.phase_times()[0]["atm"]
+        ea_n = h.atom_data()[2]
         h.close()
         n = len(Z)
+        per_element = {}
+        for z in np.unique(Z):
+            m = Z == z
+            d = 1e3 * (ea_n[m] - ea_x[m])
+            per_element[int(z)] = dict(atoms=int(m.sum()), exact_mev_per_atom=float(1e3 * ea_x[m].mean()),
+                                       mean_err_mev=float(d.mean()), rms_err_mev=float(np.sqrt(np.mean(d * d))),
+                                       share_of_cell_err_mev_per_atom=float(d.sum() / n))
         out.append(dict(kind=kind, atoms=n, e_exact_mev_per_atom=1e3 * e_x / n,
                         e_net_mev_per_atom=1e3 * e_n2 / n, judge_dev_mev_per_atom=1e3 * dev_, fallback=fb,
                         mode_after=mode, force_rms_exact=float(np.sqrt(np.mean(f_x ** 2))),
                         force_rms_err=float(np.sqrt(np.mean((f_n2 - f_x) ** 2))),
                         stress_max_err=float(np.max(np.abs(s_n2 - s_x))), stress_max_exact=float(np.max(np.abs(s_x))),
-                        atm_ms_exact=t_x, atm_ms_net=t_n))
+                        atm_ms_exact=t_x, atm_ms_net=t_n, per_element=per_element))
         log("engine check", json.dumps(out[-1]))
     return out
 
@@ -307,6 +349,9 @@ def main():
     ap.add_argument("--force-weight", type=float, default=1.0,
                     help="weight of the force loss (per kind normalised by the RMS exact force); 0: energies only")
     ap.add_argument("--force-every", type=int, default=1, help="force-loss frame every this many steps")
+    ap.add_argument("--tie", choices=["all", "classes", "none"], default="all",
+                    help="all: one network, neighbour-class blocks through shared weights; classes: one network; "
+                         "none: one network per element class")
     ap.add_argument("--eval-random", type=int, default=0,
                     help="also check a random H/C/N/O/Si/Cu cell of this many atoms (0.08 atoms/A^3)")
     ap.add_argument("--out", default=os.path.join(HERE, "atm_surrogate.d3sur"))
@@ -347,6 +392,7 @@ def main():
     checks = engine_checks(p, cases)
     report = dict(frames=len(frames), atoms=int(len(y)), label_seconds=t_lab, train_seconds=t_train,
                   steps=args.steps, batch=args.batch, force_weight=args.force_weight, force_every=args.force_every,
+                  tie=args.tie,
                   curve=curve, heldout_per_class=per_class,
                   heldout_max_frame_error_mev_per_atom=frame_err, engine=checks, stats=stats)
     if args.report:

@@ -33,6 +33,10 @@ constexpr double BOHR = 0.52917726;              // src/d3.cu:50
 constexpr double HARTREE_EV = 27.211396641308;   // src/d3.cu:204
 constexpr double K1 = 16.0, K3 = 4.0;            // src/d3_kernel.cuh:17-19
 
+// Smooth CN cutoff (the reference's optional SMOOTH_CUTOFF_CN, src/d3_kernel.cu:550-561 and
+// 1525-1533): each CN term times s(r) = tanh^3(rc_CN - r), r and rc_CN in bohr.
+int g_smooth_cn = 0;
+
 struct Tab {
     const float *c6ab, *r0ab, *rcov, *r2r4;
     int nel, nref, zoff;
@@ -167,7 +171,12 @@ void pass_cn(const Sys &S, const Tab &t, Out &o, const int64_t *centers, int64_t
             if (r2 <= 0.0) return;
             double r = std::sqrt(r2);
             double e = std::exp(-K1 * ((ri + t.rcov[S.s[j]]) / r - 1.0));
-            acc += 1.0 / (1.0 + e);
+            double sm = 1.0;
+            if (g_smooth_cn) {
+                const double th = std::tanh(S.rccn - r);
+                sm = th * th * th;
+            }
+            acc += sm / (1.0 + e);
             ++cnt;
         });
         o.cn[i] = acc;
@@ -316,6 +325,10 @@ void pass_chain(const Sys &S, const Tab &t, Out &o, const int64_t *centers, int6
             double r = std::sqrt(r2), rr = ri + t.rcov[S.s[j]];
             double e = std::exp(-K1 * (rr / r - 1.0));
             double dcn = -K1 * e * rr / ((1.0 + e) * (1.0 + e) * r2 * r);  // c'(r)/r
+            if (g_smooth_cn) {  // d(c s)/dr / r = (c'/r) s + c s'/r,  s' = -3 tanh^2 (1 - tanh^2)
+                const double th = std::tanh(S.rccn - r);
+                dcn = dcn * th * th * th - 3.0 * th * th * (1.0 - th * th) / ((1.0 + e) * r);
+            }
             double s = (o.g[i] + o.g[j]) * dcn;
             if (!std::isfinite(s)) s = 0.0;
             for (int a = 0; a < 3; ++a) {
@@ -500,5 +513,8 @@ int d3o_centers(const double *pos_A, const int32_t *Z, int64_t n, const double *
 }
 
 int d3o_max_threads(void) { return omp_get_max_threads(); }
+
+// Smooth CN cutoff on (1) or off (0, the reference's default build) for the following calls.
+void d3o_set_cn_smooth(int on) { g_smooth_cn = on ? 1 : 0; }
 
 }  // extern "C"

@@ -54,6 +54,8 @@ def load():
                                     C.c_double, C.c_double, C.c_double, C.c_int32, dp, i64p, C.c_int64, dp, dp,
                                     dp, dp, dp, dp]
         lib.d3o_max_threads.restype = C.c_int
+        lib.d3o_set_cn_smooth.restype = None
+        lib.d3o_set_cn_smooth.argtypes = [C.c_int32]
         _lib = lib
     return _lib
 
@@ -63,9 +65,11 @@ def _p(a, t):
 
 
 def compute(coords, Z, cell, table: dict, cutoff=94.8683298, cn_cutoff=40.0, atm_cutoff=0.0, damping=1,
-            functional=0, params=None) -> dict:
-    """Full FP64 evaluation.  table: dict with c6ab, r0ab, rcov, r2r4 (float32), nel, nref, zoff."""
+            functional=0, params=None, smooth_cn=False) -> dict:
+    """Full FP64 evaluation.  table: dict with c6ab, r0ab, rcov, r2r4 (float32), nel, nref, zoff.
+    smooth_cn: the tanh^3 smooth CN cutoff (the reference's SMOOTH_CUTOFF_CN build option)."""
     lib = load()
+    lib.d3o_set_cn_smooth(1 if smooth_cn else 0)
     x = np.ascontiguousarray(coords, dtype=np.float64).reshape(-1, 3)
     z = np.ascontiguousarray(Z, dtype=np.int32)
     cl = np.ascontiguousarray(cell, dtype=np.float64).reshape(9)
@@ -83,6 +87,7 @@ def compute(coords, Z, cell, table: dict, cutoff=94.8683298, cn_cutoff=40.0, atm
                          float(atm_cutoff), int(damping), _p(prm, C.c_double), _p(e, C.c_double),
                          _p(f, C.c_double), _p(s, C.c_double), _p(cn, C.c_double), _p(g, C.c_double),
                          _p(ea, C.c_double), _p(counts, C.c_uint64))
+    lib.d3o_set_cn_smooth(0)
     if rc:
         raise ValueError(f"oracle rejected the input (code {rc})")
     return dict(energy=float(e[0]), forces=f, stress=s.reshape(3, 3), cn=cn, dedcn=g, atom_energy=ea,

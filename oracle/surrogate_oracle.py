@@ -8,14 +8,19 @@ Model (no counterpart in the reference, which has only the exact ATM kernel,
 src/d3_kernel.cu:1199-1509):
   features x_i (64): x_i[0] = CN_i (the D3 coordination number, src/d3_kernel.cu CN kernel with
     the handle's CN cutoff), x_i[1 + 21 c + m] = sum over neighbours j of element class c within
-    RC_F of  exp(-(r_ij - mu_m)^2 / (2 sigma^2)) fc(r_ij),  sigma = 0.5 A, mu_m = 1 + 7 m / 20 A
+    RC_F of  u_ij exp(-(r_ij - mu_m)^2 / (2 sigma^2)) fc(r_ij),  sigma = 0.5 A, mu_m = 1 + 7 m / 20 A
     (m = 0..20), fc(r) = (1 + cos(pi r / RC_F)) / 2, RC_F = 9.5 A; classes: 0 = H, He;
-    1 = Li..Ne; 2 = the rest.
+    1 = Li..Ne; 2 = the rest.  Neighbour weight u_ij h_ij: u_ij = sqrt(C6_ij) C6_jj^(1/4) with the
+    D3 C6 at the atoms' CNs (src/d3_kernel.cu:709-800 interpolation, in the engine's factorised
+    form C6_ij = w_i^T C6ref w_j) -- for C6 close to the combination rule u_ij u_ik ~ sqrt(C6_ij
+    C6_ik C6_jk), the three-body prefactor -- and h_ij = (1 + 6 ((4/3) R0_ij / r_ij)^16)^(-1/2) with
+    R0 from r0ab, so that h_ij h_ik is the three-body damping (src/d3_kernel.cu:1386-1389) of an
+    equilateral triangle of equal radii: an element enters through its C6 and its damping radii.
   per-atom network of the atom's own class c:  x' = (x - mean) * istd,
     h1 = silu(W1^T x' + b1) (64 -> 128),  h2 = silu(W2^T h1 + b2) (128 -> 128),
     E_i = scale_c (w3 . h2 + b3) + shift_c   [eV]
   energy E = sum_i E_i; forces and stress by the chain rule through the features
-  (and through CN_i for x_i[0]).
+  (and through the CNs, on which x_i[0] and the weights u depend).
 Force matching: with F = -J^T G (J: Jacobian of the features of every atom with respect to every
   position, G = dE_i/dx_i), the force loss sum |F - F_ref|^2 has the parameter gradient
   -2 sum_i d(G_i . u_i)/dparams at fixed u = J (F - F_ref) (feature_tangent); G_i . u_i is the
@@ -123,6 +128,56 @@ def coordination(pos_b, Z, cell_b, rcov_of_Z, rc_cn):
     return cn, (i, j, d, r, dc)
 
 
+def c6_weights(Z, cn, table):
+    """Normalised Gaussian weights w[n, nref] of each atom's C6 references at its CN and dw/dCN
+    (the engine's per-atom weights): w_a = exp(-4 (CN - CNref_a)^2) / sum_b exp(-4 (CN - CNref_b)^2)
+    over the element's valid references (CNref from the homonuclear entry c6ab[Z][Z][a][a][1])."""
+    c6ab = np.asarray(table["c6ab"], np.float64)
+    nr = c6ab.shape[2]
+    s = np.asarray(Z, np.int64) - table["zoff"]
+    ref = c6ab[s, s][:, np.arange(nr), np.arange(nr), 1]
+    valid = ref > -1
+    dcn = np.asarray(cn, np.float64)[:, None] - ref
+    ex = np.where(valid, -4.0 * dcn * dcn, -np.inf)
+    e = np.where(valid, np.exp(ex - ex.max(1, keepdims=True)), 0.0)
+    w = e / e.sum(1, keepdims=True)
+    la = np.where(valid, -8.0 * dcn, 0.0)
+    dw = w * (la - np.sum(w * la, 1, keepdims=True))
+    return w, dw
+
+
+def neighbour_weights(i, j, Z, cn, table):
+    """u_ij = sqrt(C6_ij) q_j with q_j = C6_jj^(1/4), per ordered pair (i, j), and its derivatives
+    with respect to CN_i and CN_j."""
+    c6ab = np.asarray(table["c6ab"], np.float64)
+    s = np.asarray(Z, np.int64) - table["zoff"]
+    w, dw = c6_weights(Z, cn, table)
+    ok = (c6ab[..., 1] > -1) & (c6ab[..., 2] > -1)
+    ref = np.where(ok, c6ab[..., 0], 0.0)
+    Cs = ref[s, s]  # (n, nr, nr)
+    c6s = np.einsum("na,nab,nb->n", w, Cs, w)
+    dc6s = 2.0 * np.einsum("na,nab,nb->n", dw, Cs, w)
+    q = c6s ** 0.25
+    dq = 0.25 * dc6s / c6s ** 0.75
+    Cp = ref[s[i], s[j]]  # (npairs, nr, nr)
+    c6 = np.einsum("pa,pab,pb->p", w[i], Cp, w[j])
+    d_i = np.einsum("pa,pab,pb->p", dw[i], Cp, w[j])
+    d_j = np.einsum("pa,pab,pb->p", w[i], Cp, dw[j])
+    sc = np.sqrt(c6)
+    u = sc * q[j]
+    du_i = q[j] * d_i / (2.0 * sc)
+    du_j = q[j] * d_j / (2.0 * sc) + sc * dq[j]
+    return u, du_i, du_j
+
+
+def edge_damping(r, R0):
+    """h = (1 + 6 ((4/3) R0 / r)^16)^(-1/2) and dh/dr (0 for (4/3) R0 / r >= 100)."""
+    t = np.minimum((4.0 / 3.0) * R0 / r, 100.0)
+    s = t ** 16
+    h = np.where(t < 100.0, (1.0 + 6.0 * s) ** -0.5, 0.0)
+    return h, 48.0 * s * h ** 3 / r
+
+
 def rbf(r):
     """phi_m(r) and dphi_m/dr, shape (len(r), 21)."""
     fc = 0.5 * (1.0 + np.cos(np.pi * r / RC_F))
@@ -132,16 +187,21 @@ def rbf(r):
     return g * fc[:, None], g * (dfc[:, None] - fc[:, None] * u / (SIGMA * SIGMA))
 
 
-def features(pos_b, Z, cell_b, cn):
+def features(pos_b, Z, cell_b, cn, table):
+    """Features of every atom and, per ordered neighbour pair, what the derivatives need."""
     n = len(Z)
     i, j, d, r = pair_images(pos_b, cell_b, RC_F)
     phi, dphi = rbf(r)
+    s = np.asarray(Z, np.int64) - table["zoff"]
+    h, dh = edge_damping(r, np.asarray(table["r0ab"], np.float64)[s[i], s[j]])
+    phi, dphi = phi * h[:, None], dphi * h[:, None] + phi * dh[:, None]  # the damped radial basis
+    u, du_i, du_j = neighbour_weights(i, j, Z, cn, table)
     cls = element_class(Z)
     X = np.zeros((n, N_FEAT))
     X[:, 0] = cn
     col = 1 + N_RBF * cls[j][:, None] + np.arange(N_RBF)[None, :]
-    np.add.at(X, (np.repeat(i, N_RBF), col.ravel()), phi.ravel())
-    return X, (i, j, d, r, dphi, col)
+    np.add.at(X, (np.repeat(i, N_RBF), col.ravel()), (u[:, None] * phi).ravel())
+    return X, dict(i=i, j=j, d=d, r=r, phi=phi, dphi=dphi, col=col, u=u, du_i=du_i, du_j=du_j)
 
 
 # ---------------------------------------------------------------------------- network
@@ -330,7 +390,13 @@ def mlp_tangent_bf16(p, X, U, cls, w):
 
 # ---------------------------------------------------------------------------- energy and forces
 
-def evaluate(p, pos_A, Z, cell_A, rcov_of_Z, rc_cn, bf16_net=False):
+def rcov_of(table):
+    """rcov indexed by atomic number."""
+    rc = np.asarray(table["rcov"], np.float64)
+    return np.concatenate([np.zeros(table["zoff"]), rc]) if table["zoff"] else rc
+
+
+def evaluate(p, pos_A, Z, cell_A, table, rc_cn, bf16_net=False):
     """Stand-in three-body energy (eV), per-atom energies, forces (eV/A), stress (eV/A^3, the
     strain derivative over the signed volume), features and dE/dCN (eV).  bf16_net: the network
     with the MFMA kernels' roundings (mlp_bf16) instead of float64."""
@@ -339,16 +405,22 @@ def evaluate(p, pos_A, Z, cell_A, rcov_of_Z, rc_cn, bf16_net=False):
     cell_b = np.asarray(cell_A, float) / BOHR
     n = len(Z)
     cls = element_class(Z)
-    cn, (ci, cj, cd, cr, dc) = coordination(pos_b, Z, cell_b, rcov_of_Z, rc_cn)
-    X, (i, j, d, r, dphi, col) = features(pos_b, Z, cell_b, cn)
+    cn, (ci, cj, cd, cr, dc) = coordination(pos_b, Z, cell_b, rcov_of(table), rc_cn)
+    X, a = features(pos_b, Z, cell_b, cn, table)
+    i, j, d, r, col = a["i"], a["j"], a["d"], a["r"], a["col"]
     if bf16_net:
         E, G, _ = mlp_bf16(p, X, cls)
     else:
         E, cache = mlp_forward(p, X, cls)
         G = mlp_input_grad(p, cache, n)
-    # dE/dr for each ordered pair (i, j, image): the features of i depend on r_ij
-    s = np.einsum("pm,pm->p", dphi, G[np.repeat(i, N_RBF), col.ravel()].reshape(-1, N_RBF))
-    s_cn = G[ci, 0] * dc
+    # per ordered pair (i, j, image) the features of i depend on r_ij and, through u_ij, on CN_i
+    # and CN_j
+    Gp = G[np.repeat(i, N_RBF), col.ravel()].reshape(-1, N_RBF)
+    s = a["u"] * np.einsum("pm,pm->p", a["dphi"], Gp)
+    A = np.einsum("pm,pm->p", a["phi"], Gp)
+    dedcn = G[:, 0] + np.bincount(i, weights=A * a["du_i"], minlength=n) + \
+        np.bincount(j, weights=A * a["du_j"], minlength=n)
+    s_cn = dedcn[ci] * dc
     grad = np.zeros((n, 3))  # dE/dr (bohr)
     vir = np.zeros((3, 3))
     for (a, b, dd, rr, ss) in ((i, j, d, r, s), (ci, cj, cd, cr, s_cn)):
@@ -358,23 +430,29 @@ def evaluate(p, pos_A, Z, cell_A, rcov_of_Z, rc_cn, bf16_net=False):
         vir += dd.T @ v
     V = np.linalg.det(cell_b)
     return dict(energy=E.sum(), atom_energy=E, forces=-grad / BOHR, stress=vir / V / BOHR ** 3, features=X,
-                dedcn=G[:, 0], cn=cn)
+                dedcn=dedcn, cn=cn)
 
 
-def feature_tangent(pos_A, Z, cell_A, rcov_of_Z, rc_cn, dirv):
+def feature_tangent(pos_A, Z, cell_A, table, rc_cn, dirv):
     """u_i = sum_j (dx_i / dr_j) . dir_j: the change of every atom's features per unit step of
-    the positions along dir (dir per A; u in feature units per A), through the radial sums and
-    through CN_i (x_i[0])."""
+    the positions along dir (dir per A; u in feature units per A), through the radial sums, the
+    neighbour weights (through CN_i and CN_j) and CN_i (x_i[0])."""
     Z = np.asarray(Z)
     pos_b = np.asarray(pos_A, float) / BOHR
     cell_b = np.asarray(cell_A, float) / BOHR
     dirv = np.asarray(dirv, float)
-    cn, (ci, cj, cd, cr, dc) = coordination(pos_b, Z, cell_b, rcov_of_Z, rc_cn)
-    _, (i, j, d, r, dphi, col) = features(pos_b, Z, cell_b, cn)
-    U = np.zeros((len(Z), N_FEAT))
+    n = len(Z)
+    cn, (ci, cj, cd, cr, dc) = coordination(pos_b, Z, cell_b, rcov_of(table), rc_cn)
+    _, a = features(pos_b, Z, cell_b, cn, table)
+    i, j, d, r, col = a["i"], a["j"], a["d"], a["r"], a["col"]
+    U = np.zeros((n, N_FEAT))
+    # CN along dir (per A of step)
+    pc = np.einsum("pk,pk->p", cd, dirv[ci] - dirv[cj]) / cr / BOHR
+    dcn = np.bincount(ci, weights=dc * pc, minlength=n)
+    U[:, 0] = dcn
     # dr_ij along dir (bohr per A of step): d_ij / r_ij . (dir_i - dir_j) / BOHR
     pr = np.einsum("pk,pk->p", d, dirv[i] - dirv[j]) / r / BOHR
-    np.add.at(U, (np.repeat(i, N_RBF), col.ravel()), (dphi * pr[:, None]).ravel())
-    pc = np.einsum("pk,pk->p", cd, dirv[ci] - dirv[cj]) / cr / BOHR
-    U[:, 0] = np.bincount(ci, weights=dc * pc, minlength=len(Z))
+    du = a["du_i"] * dcn[i] + a["du_j"] * dcn[j]
+    v = a["u"][:, None] * a["dphi"] * pr[:, None] + a["phi"] * du[:, None]
+    np.add.at(U, (np.repeat(i, N_RBF), col.ravel()), v.ravel())
     return U

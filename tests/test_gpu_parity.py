@@ -159,6 +159,33 @@ def test_drop_in_md_loop(table, precision, monkeypatch):
     del calc
 
 
+@pytest.mark.parametrize("precision", [d3_lib.D3X_FP64, d3_lib.D3X_FP32])
+def test_smooth_cn_cutoff(table, precision):
+    """The smooth CN cutoff option against the oracle with the same option, three-body term on, on a
+    CN cutoff short enough (7 bohr) that the smooth factor changes CN and forces."""
+    s = systems.triclinic_mixed(n=40, seed=71)
+    kw = dict(cutoff=40.0, cn_cutoff=7.0, atm_cutoff=25.0, damping=1, functional=0)
+    print(f"case smooth CN precision={precision}", flush=True)
+    h = d3_lib.Handle(np.unique(s["Z"]), len(s["Z"]), precision=precision, **kw)
+    h.set_cn_smooth(True)
+    h.set_atoms(s["pos"], s["Z"])
+    h.set_cell(s["cell"])
+    e, f, st = h.compute()
+    cn, g, _ = h.atom_data()
+    h.close()
+    o = d3_oracle.compute(s["pos"], s["Z"], s["cell"], table, smooth_cn=True, **kw)
+    sharp = d3_oracle.compute(s["pos"], s["Z"], s["cell"], table, **kw)
+    assert np.max(np.abs(sharp["cn"] - o["cn"])) > 1e-3  # the option matters in this cell
+    if precision == d3_lib.D3X_FP64:
+        assert abs(e - o["energy"]) <= 1e-12 * abs(o["energy"])
+        assert np.max(np.abs(f - o["forces"])) <= 1e-9
+        assert np.max(np.abs(st - o["stress"])) <= 1e-12 * np.max(np.abs(o["stress"])) + 1e-12
+        np.testing.assert_allclose(cn, o["cn"], rtol=1e-12, atol=1e-14)
+    else:
+        assert abs(e - o["energy"]) <= 2e-6 * abs(o["energy"])
+        assert np.max(np.abs(f - o["forces"])) <= 2e-5 * np.max(np.abs(o["forces"])) + 1e-7
+
+
 @pytest.mark.parametrize("atm", [0.0, 40.0])
 def test_repeat_is_stable(table, atm):
     """ResultStable (test/d3_test.cpp:89-129, 1e-7 eV / 1e-5 there): without the three-body term

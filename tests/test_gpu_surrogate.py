@@ -134,7 +134,7 @@ def test_features_match_oracle(table, precision):
     h.close()
     rcov = np.asarray(table["rcov"], np.float64)
     cn, _ = SO.coordination(pos / SO.BOHR, Z, cell / SO.BOHR, rcov, RC["cn_cutoff"])
-    Xo, _ = SO.features(pos / SO.BOHR, Z, cell / SO.BOHR, cn)
+    Xo, _ = SO.features(pos / SO.BOHR, Z, cell / SO.BOHR, cn, table)
     np.testing.assert_allclose(X[:, 0], Xo[:, 0], rtol=2e-6, atol=1e-6)
     np.testing.assert_allclose(X[:, 1:], Xo[:, 1:], rtol=1e-5, atol=1e-5 * np.max(Xo[:, 1:]))
 
@@ -158,8 +158,7 @@ def test_surrogate_energy_forces_stress_match_oracle(table):
     assert mode == d3_lib.D3X_ATM_SURROGATE and fb == 0 and dev_ >= 0
     o2 = d3_oracle.compute(pos, Z, cell, table, cutoff=RC["cutoff"], cn_cutoff=RC["cn_cutoff"], atm_cutoff=0.0,
                            damping=1, functional=0)
-    so = SO.evaluate(p.astype(np.float64), pos, Z, cell, np.asarray(table["rcov"], np.float64), RC["cn_cutoff"],
-                     bf16_net=True)
+    so = SO.evaluate(p.astype(np.float64), pos, Z, cell, table, RC["cn_cutoff"], bf16_net=True)
     assert abs((e - o2["energy"]) - so["energy"]) <= 1e-4 * np.sum(np.abs(so["atom_energy"])) + 1e-8
     df = f - o2["forces"]
     assert np.max(np.abs(df - so["forces"])) <= 2e-3 * np.max(np.abs(so["forces"])) + 1e-8
@@ -236,6 +235,30 @@ def test_judge_force_criterion():
     assert h.sur_state()[0] == d3_lib.D3X_ATM_SURROGATE
     assert abs(e3 - e1) <= 1e-9 * abs(e1) and np.max(np.abs(f3 - f1)) <= 1e-9
     h.close()
+
+
+def test_tie_grad_sums_the_copies():
+    """d3x_sur_tie_grad against numpy: every class block's entries summed over the classes, and
+    with blocks the radial rows of W1 summed over the three neighbour-class blocks too."""
+    lib = d3_lib.load()
+    rng = np.random.default_rng(31)
+    g0 = rng.normal(0, 1, d3_lib.SUR_NPARAMS).astype(np.float32)
+    for blocks in (0, 1):
+        print(f"case sur-tie blocks={blocks}", flush=True)
+        g = dev(g0)
+        assert lib.d3x_sur_tie_grad(vp(g), blocks, stream()) == 0
+        torch.cuda.synchronize()
+        got = g.cpu().numpy().astype(np.float64)
+        ref = g0.astype(np.float64).copy()
+        cb = [ref[SO.HEAD + c * SO.CLASS_SIZE:SO.HEAD + (c + 1) * SO.CLASS_SIZE] for c in range(3)]
+        tot = cb[0] + cb[1] + cb[2]
+        W = tot[SO.O_W1:SO.O_B1].reshape(SO.N_FEAT, SO.N_HID)
+        if blocks:
+            s = W[1:22] + W[22:43] + W[43:64]
+            W[1:22], W[22:43], W[43:64] = s, s, s
+        for c in range(3):
+            ref[SO.HEAD + c * SO.CLASS_SIZE:SO.HEAD + (c + 1) * SO.CLASS_SIZE] = tot
+        np.testing.assert_allclose(got, ref, rtol=1e-6, atol=1e-6)
 
 
 def test_adam_step_and_training_reduces_loss():
@@ -344,7 +367,7 @@ def test_feature_tangent_matches_oracle(table, precision):
     with pytest.raises(d3_lib.D3Error):
         h.sur_tangent(dirv)
     h.close()
-    Uo = SO.feature_tangent(pos, Z, cell, np.asarray(table["rcov"], np.float64), RC["cn_cutoff"], dirv)
+    Uo = SO.feature_tangent(pos, Z, cell, table, RC["cn_cutoff"], dirv)
     assert np.max(np.abs(U[:, 0] - Uo[:, 0])) <= 1e-5 * np.max(np.abs(Uo[:, 0]))
     assert np.max(np.abs(U[:, 1:] - Uo[:, 1:])) <= 1e-5 * np.max(np.abs(Uo[:, 1:]))
 

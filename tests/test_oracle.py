@@ -68,6 +68,49 @@ def test_pair_cn_closed_form(table):
     np.testing.assert_allclose(o["cn"], [cn, cn], rtol=1e-14)
 
 
+def test_smooth_cn_cutoff_closed_form(table):
+    """The reference's optional smooth CN cutoff (SMOOTH_CUTOFF_CN, src/d3_kernel.cu:550-561):
+    an isolated pair's CN is tanh^3(rc_CN - r) / (1 + exp(-k1 (R / r - 1))) with r, rc_CN in bohr,
+    and a pair just inside the cutoff contributes next to nothing."""
+    r = 1.9
+    pos, cell = iso([[0, 0, 0], [0, r, 0]])
+    rc = 6.0  # bohr
+    o = run(pos, [6, 8], cell, table, cutoff=40.0, cn_cutoff=rc, damping=1, smooth_cn=True)
+    R = float(table["rcov"][6]) + float(table["rcov"][8])
+    rb = r / BOHR
+    cn = np.tanh(rc - rb) ** 3 / (1.0 + np.exp(-16.0 * (R / rb - 1.0)))
+    np.testing.assert_allclose(o["cn"], [cn, cn], rtol=1e-13)
+    sharp = run(pos, [6, 8], cell, table, cutoff=40.0, cn_cutoff=rc, damping=1)
+    assert abs(o["cn"][0] - sharp["cn"][0]) > 1e-3  # the option changes the result here
+    edge = run(iso([[0, 0, 0], [0, (rc - 1e-4) * BOHR, 0]])[0], [6, 8], iso([[0, 0, 0]])[1], table, cutoff=40.0,
+               cn_cutoff=rc, damping=1, smooth_cn=True)
+    assert 0 < edge["cn"][0] < 1e-11
+
+
+@pytest.mark.parametrize("atm", [False, True])
+def test_smooth_cn_forces_and_stress_match_fd(table, atm):
+    pos, Z, cell, kw = small_triclinic(atm)
+    # a CN cutoff (7 bohr) where the terms near it are large enough for the smooth factor to count
+    kw = dict(kw, cn_cutoff=7.0, smooth_cn=True)
+    o = run(pos, Z, cell, table, **kw)
+    h = 1e-4
+    for i in (0, 3, 5):
+        for c in range(3):
+            p1, p2 = pos.copy(), pos.copy()
+            p1[i, c] += h
+            p2[i, c] -= h
+            fd = -(run(p1, Z, cell, table, **kw)["energy"] - run(p2, Z, cell, table, **kw)["energy"]) / (2 * h)
+            assert abs(fd - o["forces"][i, c]) < 1e-6, (i, c)
+    V = abs(np.linalg.det(cell))
+    h = 1e-5
+    for a, b in ((0, 0), (1, 2)):
+        eps = np.zeros((3, 3))
+        eps[a, b] = h
+        ep = run(pos @ (np.eye(3) + eps), Z, cell @ (np.eye(3) + eps), table, **kw)["energy"]
+        em = run(pos @ (np.eye(3) - eps), Z, cell @ (np.eye(3) - eps), table, **kw)["energy"]
+        assert abs((ep - em) / (2 * h) / V - o["stress"][a, b]) < 1e-8 * max(1.0, np.max(np.abs(o["stress"]))) + 1e-9
+
+
 def test_cn_cutoff_is_inclusive_and_twobody_cutoff_strict(table):
     """CN counts r^2 <= rc^2, the two-body sum r^2 < rc^2 (src/d3_kernel.cu:626, :1073)."""
     rb = 12.0

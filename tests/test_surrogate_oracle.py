@@ -41,19 +41,63 @@ def test_cn_matches_d3_oracle(table, rcov):
     np.testing.assert_allclose(cn, o["cn"], rtol=1e-12, atol=1e-12)
 
 
-def test_isolated_pair_features():
-    """Two atoms (H and O) far from every image: each sees one neighbour at r."""
+def _c6_pair(table, Zi, Zj, cni, cnj):
+    """C6(CN_i, CN_j) as the 25-term sum over the elements' valid references (CNref from the
+    homonuclear entries), written out independently of the restatement."""
+    c = np.asarray(table["c6ab"], np.float64)
+    refs = lambda Zx: [(a, c[Zx, Zx, a, a, 1]) for a in range(table["nref"]) if c[Zx, Zx, a, a, 1] > -1]
+    num = den = 0.0
+    for a, ra in refs(Zi):
+        for b, rb in refs(Zj):
+            L = np.exp(-4.0 * (cni - ra) ** 2 - 4.0 * (cnj - rb) ** 2)
+            num += c[Zi, Zj, a, b, 0] * L
+            den += L
+    return num / den
+
+
+def test_isolated_pair_features(table):
+    """Two atoms (H and O) far from every image: each sees one neighbour at r, weighted by
+    sqrt(C6_HO) C6_jj^(1/4) at the atoms' CNs and the edge damping of r / R0_HO."""
     cell = np.eye(3) * 40.0
     r = 2.3
     pos = np.array([[1.0, 1.0, 1.0], [1.0 + r, 1.0, 1.0]])
     Z = np.array([1, 8])
-    X, _ = SO.features(pos / SO.BOHR, Z, cell / SO.BOHR, np.zeros(2))
+    cn = np.array([0.37, 1.21])
+    X, _ = SO.features(pos / SO.BOHR, Z, cell / SO.BOHR, cn, table)
     rb = r / SO.BOHR
     fc = 0.5 * (1 + np.cos(np.pi * rb / SO.RC_F))
     phi = np.exp(-(rb - SO.CENTRES) ** 2 / (2 * SO.SIGMA ** 2)) * fc
-    np.testing.assert_allclose(X[0, 1 + 21:1 + 42], phi, rtol=1e-13)  # H sees O (class 1)
-    np.testing.assert_allclose(X[1, 1:22], phi, rtol=1e-13)           # O sees H (class 0)
+    c6 = _c6_pair(table, 1, 8, cn[0], cn[1])
+    h = (1 + 6 * (4.0 / 3.0 * float(table["r0ab"][1, 8]) / rb) ** 16) ** -0.5
+    u_HO = np.sqrt(c6) * _c6_pair(table, 8, 8, cn[1], cn[1]) ** 0.25 * h
+    u_OH = np.sqrt(c6) * _c6_pair(table, 1, 1, cn[0], cn[0]) ** 0.25 * h
+    np.testing.assert_allclose(X[0, 1 + 21:1 + 42], u_HO * phi, rtol=1e-12)  # H sees O (class 1)
+    np.testing.assert_allclose(X[1, 1:22], u_OH * phi, rtol=1e-12)           # O sees H (class 0)
     assert np.count_nonzero(X[0, 1:]) == 21 and np.count_nonzero(X[1, 1:]) == 21
+
+
+def test_neighbour_weight_derivatives_match_fd(table):
+    """du_ij/dCN_i and du_ij/dCN_j of the restatement against central differences of u in CN."""
+    Z = np.array([1, 8, 7, 29, 14])
+    cn = np.array([0.6, 1.9, 2.4, 7.5, 3.3])
+    i = np.array([0, 1, 2, 3, 4, 0, 3])
+    j = np.array([1, 0, 3, 2, 1, 4, 3])
+    u, du_i, du_j = SO.neighbour_weights(i, j, Z, cn, table)
+    h = 1e-6
+    for k in range(len(i)):
+        for which, ref in ((i[k], du_i[k]), (j[k], du_j[k])):
+            if which == i[k] and which == j[k]:
+                continue  # a self pair moves both ends at once
+            cp, cm = cn.copy(), cn.copy()
+            cp[which] += h
+            cm[which] -= h
+            fd = (SO.neighbour_weights(i, j, Z, cp, table)[0][k] - SO.neighbour_weights(i, j, Z, cm, table)[0][k]) / (2 * h)
+            assert abs(fd - ref) <= 1e-6 * max(1.0, abs(u[k])), (k, which)
+    # and u against the written-out C6
+    for k in range(len(i)):
+        ref = np.sqrt(_c6_pair(table, Z[i[k]], Z[j[k]], cn[i[k]], cn[j[k]])) * \
+            _c6_pair(table, Z[j[k]], Z[j[k]], cn[j[k]], cn[j[k]]) ** 0.25
+        assert u[k] == pytest.approx(ref, rel=1e-12)
 
 
 def test_input_gradient_matches_fd():
@@ -104,17 +148,18 @@ def test_param_gradient_matches_fd():
         assert g[o + 1] == 0 and g[o + 2] == 0 and g[o + 3] == 0
 
 
-def test_forces_and_stress_match_fd(rcov):
-    pos, Z, cell = small(seed=8, n=10)
+def test_forces_and_stress_match_fd(table):
+    # CNs inside the references' range (0.9-8.6), where the neighbour weights depend on them
+    pos, Z, cell = small(seed=9, n=10)
     p = random_params(seed=2)
-    o = SO.evaluate(p, pos, Z, cell, rcov, 25.0)
+    o = SO.evaluate(p, pos, Z, cell, table, 25.0)
     h = 1e-5
     for a in (0, 3, 7):
         for k in range(3):
             pp, pm = pos.copy(), pos.copy()
             pp[a, k] += h
             pm[a, k] -= h
-            fd = -(SO.evaluate(p, pp, Z, cell, rcov, 25.0)["energy"] - SO.evaluate(p, pm, Z, cell, rcov, 25.0)["energy"]) / (2 * h)
+            fd = -(SO.evaluate(p, pp, Z, cell, table, 25.0)["energy"] - SO.evaluate(p, pm, Z, cell, table, 25.0)["energy"]) / (2 * h)
             assert abs(fd - o["forces"][a, k]) <= 1e-6 * max(1.0, np.max(np.abs(o["forces"])))
     V = abs(np.linalg.det(cell))
     # the D3 CN has a hard cutoff (a pair crossing it moves CN by ~1e-6): a strain step small
@@ -123,8 +168,8 @@ def test_forces_and_stress_match_fd(rcov):
     for a, b in ((0, 0), (1, 2), (2, 1)):
         eps = np.zeros((3, 3))
         eps[a, b] = h
-        ep = SO.evaluate(p, pos @ (np.eye(3) + eps), Z, cell @ (np.eye(3) + eps), rcov, 25.0)["energy"]
-        em = SO.evaluate(p, pos @ (np.eye(3) - eps), Z, cell @ (np.eye(3) - eps), rcov, 25.0)["energy"]
+        ep = SO.evaluate(p, pos @ (np.eye(3) + eps), Z, cell @ (np.eye(3) + eps), table, 25.0)["energy"]
+        em = SO.evaluate(p, pos @ (np.eye(3) - eps), Z, cell @ (np.eye(3) - eps), table, 25.0)["energy"]
         assert abs((ep - em) / (2 * h) / V - o["stress"][a, b]) <= 1e-6 * max(1e-3, np.max(np.abs(o["stress"])))
     # translation invariance: forces sum to zero
     assert np.max(np.abs(o["forces"].sum(0))) <= 1e-10 * max(1.0, np.max(np.abs(o["forces"])))
@@ -207,42 +252,42 @@ def test_tangent_param_gradient_matches_fd():
         assert g[o] == 0 and g[o + 1] == 0 and g[o + 2] == 0 and g[o + 3] == 0
 
 
-def test_feature_tangent_matches_fd(rcov):
+def test_feature_tangent_matches_fd(table, rcov):
     """u = J dir equals the finite difference of the features (radial sums and CN) along dir."""
     pos, Z, cell = small(seed=9, n=10)
     rng = np.random.default_rng(25)
     dirv = rng.normal(0, 1, pos.shape)
-    U = SO.feature_tangent(pos, Z, cell, rcov, 25.0, dirv)
+    U = SO.feature_tangent(pos, Z, cell, table, 25.0, dirv)
 
     def feats(q):
         qb = q / SO.BOHR
         cn, _ = SO.coordination(qb, Z, cell / SO.BOHR, rcov, 25.0)
-        return SO.features(qb, Z, cell / SO.BOHR, cn)[0]
+        return SO.features(qb, Z, cell / SO.BOHR, cn, table)[0]
 
     h = 1e-6
     fd = (feats(pos + h * dirv) - feats(pos - h * dirv)) / (2 * h)
     assert np.max(np.abs(U[:, 0] - fd[:, 0])) <= 1e-6 * max(1.0, np.max(np.abs(fd[:, 0])))
     assert np.max(np.abs(U[:, 1:] - fd[:, 1:])) <= 1e-6 * np.max(np.abs(fd[:, 1:]))
     # a rigid translation changes no feature
-    Ut = SO.feature_tangent(pos, Z, cell, rcov, 25.0, np.tile([0.3, -1.0, 2.0], (len(Z), 1)))
+    Ut = SO.feature_tangent(pos, Z, cell, table, 25.0, np.tile([0.3, -1.0, 2.0], (len(Z), 1)))
     assert np.max(np.abs(Ut)) <= 1e-12 * np.max(np.abs(U))
 
 
-def test_force_matching_gradient_matches_fd(rcov):
+def test_force_matching_gradient_matches_fd(table):
     """End to end: for L = sum |F(params) - F_ref|^2 of the stand-in on a small cell,
     dL/dparams = -2 sum_i d(G_i . u_i)/dparams with u = J (F - F_ref), which is mlp_tangent's
     gradient with w = -2 on the features' tangent along F - F_ref."""
     pos, Z, cell = small(seed=10, n=8)
     p = random_params(seed=26)
     rng = np.random.default_rng(27)
-    o = SO.evaluate(p, pos, Z, cell, rcov, 25.0)
+    o = SO.evaluate(p, pos, Z, cell, table, 25.0)
     F_ref = o["forces"] + rng.normal(0, np.std(o["forces"]), o["forces"].shape)
 
     def L(q):
-        return np.sum((SO.evaluate(q, pos, Z, cell, rcov, 25.0)["forces"] - F_ref) ** 2)
+        return np.sum((SO.evaluate(q, pos, Z, cell, table, 25.0)["forces"] - F_ref) ** 2)
 
     dF = o["forces"] - F_ref
-    U = SO.feature_tangent(pos, Z, cell, rcov, 25.0, dF)
+    U = SO.feature_tangent(pos, Z, cell, table, 25.0, dF)
     cls = SO.element_class(Z)
     _, g = SO.mlp_tangent(p, o["features"], U, cls, np.full(len(Z), -2.0))
     # the identity behind it: sum_i G_i . u_i = -sum_i F_i . dF_i
