@@ -92,6 +92,15 @@ int d3x_set_tuning(d3_handle_t *handle, double bin_width_bohr, int atm_blocks);
 int d3x_get_tuning(d3_handle_t *handle, double *bin_width_bohr, int *atm_blocks);
 int d3x_get_atm_variant(d3_handle_t *handle);
 
+/* C6 interpolation of the handle: 0 = through per-atom Gaussian weights of per-element CN
+ * references (w_A^T C6ref_AB w_B, the D3 tables), 1 = per pair over the pair's own entries of
+ * c6ab[A][B][a][b] (src/d3_kernel.cu:709-800), used when the loaded table's CN references do not
+ * factorise per element or when D3_C6_PAIRWISE=1 is in the environment at init.  Both give the
+ * reference's C6; the per-pair form costs nref^2 exponentials per pair and per triangle edge, and
+ * the three-body network (include/d3x_sur.h) needs the factorised form.  -1 on error.  No
+ * counterpart in the reference, which always interpolates per pair. */
+int d3x_get_c6_pairwise(d3_handle_t *handle);
+
 /* Smooth CN cutoff (the reference's SMOOTH_CUTOFF_CN build option, src/d3_kernel.cu:550-561,
  * off in its default build): on != 0 multiplies every CN term by tanh^3(rc_CN - r) (bohr), with
  * its derivative in the chain-rule forces and stress.  D3_SMOOTH_CN=1 in the environment sets it

@@ -368,6 +368,11 @@ int d3x_get_atm_variant(d3_handle_t *handle) {
     return st ? -1 : v;
 }
 
+int d3x_get_c6_pairwise(d3_handle_t *handle) {
+    if (!handle) return -1;
+    return H(handle)->eng->c6_pairwise() ? 1 : 0;
+}
+
 double d3x_autotune(d3_handle_t *handle, const char *cache_path) {
     if (!handle) return -1.0;
     double best = -1.0;

@@ -150,6 +150,7 @@ template <class T, int NR, bool KL, bool DR>
 __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> P) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr bool F32 = sizeof(T) == 4;
+    static_assert(NR > 0 || !DR, "the pairwise-C6 form (NR = 0) stages no C6 or damping rows");
     constexpr int DRO = DR ? 1 : 0;  // C6 rows of a type start after its damping row
     const int lane = threadIdx.x;
     const int ne = P.ne, ynr = ne * NR;
@@ -182,18 +183,20 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
         Centre<T> C;
         C.load(P, i);
         const int ti = C.ti;
+        const T cni = NR == 0 ? P.cn[i] : T(0);  // pairwise C6 only
         __syncthreads();  // previous centre done with LDS and scratch
-        for (int idx = lane; idx < ynr; idx += AW) {
-            const int B = idx / NR, bb = idx % NR;
-            T sy = T(0), sdy = T(0);
-            for (int a = 0; a < NR; ++a) {
-                const T c = P.c6[((ti * ne + B) * NR + a) * NR + bb];
-                sy += c * P.w[a * P.n + i];
-                sdy += c * P.dw[a * P.n + i];
+        if constexpr (NR > 0)
+            for (int idx = lane; idx < ynr; idx += AW) {
+                const int B = idx / NR, bb = idx % NR;
+                T sy = T(0), sdy = T(0);
+                for (int a = 0; a < NR; ++a) {
+                    const T c = P.c6[((ti * ne + B) * NR + a) * NR + bb];
+                    sy += c * P.w[a * P.n + i];
+                    sdy += c * P.dw[a * P.n + i];
+                }
+                S.Cy[idx] = sy;
+                S.Cdy[idx] = sdy;
             }
-            S.Cy[idx] = sy;
-            S.Cdy[idx] = sdy;
-        }
         __syncthreads();
 
         // ---- phase 1: forward neighbours within rc3 (1e-12 <= r^2 <= rc3^2, src/d3_kernel.cu:1268),
@@ -284,11 +287,15 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
             if (p < nf) {
                 const int j = Lj[p], tj = P.type_s[j];
                 T c6 = T(0), dci = T(0), dcj = T(0);
-                for (int b = 0; b < NR; ++b) {
-                    const T wj = P.w[b * P.n + j], dwj = P.dw[b * P.n + j];
-                    c6 += S.Cy[tj * NR + b] * wj;
-                    dci += S.Cdy[tj * NR + b] * wj;
-                    dcj += S.Cy[tj * NR + b] * dwj;
+                if constexpr (NR == 0) {
+                    c6_pairwise(P.c6g, ne, P.nref, ti, tj, cni, P.cn[j], c6, dci, dcj);
+                } else {
+                    for (int b = 0; b < NR; ++b) {
+                        const T wj = P.w[b * P.n + j], dwj = P.dw[b * P.n + j];
+                        c6 += S.Cy[tj * NR + b] * wj;
+                        dci += S.Cdy[tj * NR + b] * wj;
+                        dcj += S.Cy[tj * NR + b] * dwj;
+                    }
                 }
                 const T ic6 = c6 > T(0) ? T(1) / c6 : T(0);
                 Lt[p] = tj;
@@ -326,12 +333,17 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
         // Pair (i, p) terms of a forward neighbour p from its offset, type and weights, the same
         // expressions for the k and the j side: r^2, 1/r^2, 1/r, sqrt C6_ip, damping term
         // (FP32: (8/3) log2(R0^2 / r^2), FP64: R0^2 / r^2).
-        auto pair_terms = [&](T dx, T dy, T dz, int tp, const T *wp, T &a, T &ia, T &ra, T &sc6, T &lam) {
+        auto pair_terms = [&](T dx, T dy, T dz, int tp, const T *wp, T cnp, T &a, T &ia, T &ra, T &sc6, T &lam) {
             a = dx * dx + dy * dy + dz * dz;
             ia = T(1) / a;
             ra = T(1) / xsqrt(a);
             T c6 = T(0);
-            for (int b = 0; b < NR; ++b) c6 += S.Cy[tp * NR + b] * wp[b];
+            if constexpr (NR == 0) {
+                T d1, d2;
+                c6_pairwise(P.c6g, ne, P.nref, ti, tp, cni, cnp, c6, d1, d2);
+            } else {
+                for (int b = 0; b < NR; ++b) c6 += S.Cy[tp * NR + b] * wp[b];
+            }
             sc6 = xsqrt(c6);
             const T r0 = P.r0[ti * ne + tp];
             lam = F32 ? T((8.0f / 3.0f) * log2f((float)(r0 * r0 * ia))) : r0 * r0 * ia;
@@ -341,7 +353,8 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
             const bool kval = k < nf;
             // padding lanes sit far away, so every pair with them fails the cutoff test
             T kx = T(-FAR), ky = T(-FAR), kz = T(-FAR), kb2 = T(1), kib = T(1), krb = T(1), ksc6 = T(0), klam = T(0);
-            T kw[NR], kdw[NR];
+            T kw[NR > 0 ? NR : 1], kdw[NR > 0 ? NR : 1];
+            T kcn = T(0);  // pairwise C6 only
             int tk = 0;
             for (int a = 0; a < NR; ++a) kw[a] = kdw[a] = T(0);
             if (kval) {
@@ -354,8 +367,9 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                     kw[a] = P.w[a * P.n + jk];
                     kdw[a] = P.dw[a * P.n + jk];
                 }
+                if constexpr (NR == 0) kcn = P.cn[jk];
                 T sc6, lam;
-                pair_terms(kx, ky, kz, tk, kw, kb2, kib, krb, sc6, lam);
+                pair_terms(kx, ky, kz, tk, kw, kcn, kb2, kib, krb, sc6, lam);
                 ksc6 = T(-0.5) * sc6;  // -sqrt(C6_ik) / 2: C9 / 2 below
                 klam = F32 ? lam + T(L2K6_83) : T(K6) * lam;
             }
@@ -388,7 +402,8 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                     T jxs = T(FAR), jys = T(FAR), jzs = T(FAR);
                     T a = T(1), ia = T(1), ra = T(1), sc6 = T(0), lam = T(0);
                     int tj = 0;
-                    T wj[NR], dwj[NR];
+                    T wj[NR > 0 ? NR : 1], dwj[NR > 0 ? NR : 1];
+                    T jcn = T(0);  // pairwise C6 only
                     for (int q = 0; q < NR; ++q) wj[q] = dwj[q] = T(0);
                     if (jval) {
                         jxs = LT[F_DX * cap + pj];
@@ -400,13 +415,19 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                             wj[q] = P.w[q * P.n + jj];
                             dwj[q] = P.dw[q * P.n + jj];
                         }
-                        pair_terms(jxs, jys, jzs, tj, wj, a, ia, ra, sc6, lam);
+                        if constexpr (NR == 0) jcn = P.cn[jj];
+                        pair_terms(jxs, jys, jzs, tj, wj, jcn, a, ia, ra, sc6, lam);
                     }
                     S.J0[lane] = T2<T>{jxs, jys};
                     S.J1[lane] = T2<T>{jzs, a};
                     S.J2[lane] = T2<T>{ia, ra};
                     S.J3[lane] = T2<T>{sc6, lam};
                     if constexpr (!DR) S.Jt[lane * S.TI] = tj * ne;  // row of the per-type-pair table
+                    if constexpr (NR == 0) {
+                        // pairwise C6: the j type and CN, one row
+                        S.Jt[lane * S.TI + 1] = tj;
+                        S.Jyd[lane] = T2<T>{jcn, T(0)};
+                    }
                     // Types in ascending order, NR rows each: the rows past a type's own count
                     // (zero: its C6 entries there are 0) land on the next type's rows, which that
                     // type then writes over; the last type's fill the padding.  Every row a pair
@@ -476,12 +497,16 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                     }
                     // C6_jk and its CN derivatives from the j row for type(k)
                     T c6jk = T(0), dk6 = T(0), dj6 = T(0);
+                    if constexpr (NR == 0) {
+                        c6_pairwise(P.c6g, ne, P.nref, S.Jt[jl * S.TI + 1], tk, S.Jyd[jl].x, kcn, c6jk, dj6, dk6);
+                    } else {
 #pragma unroll
-                    for (int bb = 0; bb < NR; ++bb) {
-                        const T2<T> yd = Yk[(DRO + bb) * AW + jl];
-                        c6jk += yd.x * kw[bb];
-                        dk6 += yd.x * kdw[bb];
-                        dj6 += yd.y * kw[bb];
+                        for (int bb = 0; bb < NR; ++bb) {
+                            const T2<T> yd = Yk[(DRO + bb) * AW + jl];
+                            c6jk += yd.x * kw[bb];
+                            dk6 += yd.x * kdw[bb];
+                            dj6 += yd.y * kw[bb];
+                        }
                     }
                     const T rs = prsqrt(c6jk);
                     const T C9h = (j3.x * ksc6) * (c6jk * rs);  // C9 / 2
@@ -724,6 +749,7 @@ __global__ void k_merge(Params<T> P) {
 
 // One instantiation per reference count (1..MAX_NREF): the C6 dot products and the LDS rows
 // are sized by the references the handle's elements actually have.  var: bit 0 = KL, bit 1 = DR.
+// nref 0: the pairwise-C6 form for tables that do not factorise (without DR).
 template <class T, bool KL, bool DR, int... NR>
 const void *atm_pick(int nref, std::integer_sequence<int, NR...>) {
     const void *fn = nullptr;
@@ -734,6 +760,10 @@ const void *atm_pick(int nref, std::integer_sequence<int, NR...>) {
 template <class T>
 const void *atm_kernel(int nref, int var) {
     using S = std::make_integer_sequence<int, MAX_NREF>;
+    if (nref == 0)
+        return (var & ATM_DR) ? nullptr
+                              : (var & ATM_KL) ? reinterpret_cast<const void *>(&k_atm<T, 0, true, false>)
+                                               : reinterpret_cast<const void *>(&k_atm<T, 0, false, false>);
     switch (var & 3) {
         case 0: return atm_pick<T, false, false>(nref, S{});
         case 1: return atm_pick<T, true, false>(nref, S{});
@@ -768,8 +798,9 @@ int atm_choose_variant(int ne, int nref, int yrows, size_t ts) {
     const int order[4] = {ATM_KL | ATM_DR, ATM_DR, ATM_KL, 0};
     int blocks[4], most = 0;
     for (int k = 0; k < 4; ++k) {
-        blocks[k] = atm_lds_bytes(ne, nref, yrows, ts, order[k]) > 65536 ? 0
-                                                                         : blocks_per_cu(ne, nref, yrows, ts, order[k]);
+        blocks[k] = (nref == 0 && (order[k] & ATM_DR)) || atm_lds_bytes(ne, nref, yrows, ts, order[k]) > 65536
+                        ? 0
+                        : blocks_per_cu(ne, nref, yrows, ts, order[k]);
         most = blocks[k] > most ? blocks[k] : most;
     }
     const bool low = most <= 10;
@@ -793,9 +824,10 @@ void launch_atm(const Params<T> &P, int nblocks, hipStream_t s) {
     const size_t nz = (size_t)6 * P.n;
     if (nz) hipLaunchKernelGGL(k_fill_d, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, s, P.fa, nz, P.atm_next);
     if (P.n == 0 || !(P.rc_32 > T(0))) return;
-    const size_t lds = atm_lds_bytes(P.ne, P.nref, P.atm_yrows, sizeof(T), P.atm_var);
-    const void *fn = atm_kernel<T>(P.nref, P.atm_var);
-    if (!fn) return;  // not reached: a handle's reference count is 1..MAX_NREF
+    const int knr = P.c6_pairwise ? 0 : P.nref;
+    const size_t lds = atm_lds_bytes(P.ne, knr, P.atm_yrows, sizeof(T), P.atm_var);
+    const void *fn = atm_kernel<T>(knr, P.atm_var);
+    if (!fn) return;  // not reached: 1..MAX_NREF references, or pairwise C6 without DR
     Params<T> arg = P;
     void *args[] = {&arg};
     (void)hipLaunchKernel(fn, dim3(nblocks), dim3(AW), args, lds, s);

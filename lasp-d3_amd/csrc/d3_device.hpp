@@ -66,6 +66,53 @@ __device__ __forceinline__ T finite_or_zero(T x) {
     return isfinite(x) ? x : T(0);
 }
 
+// C6 of one pair by the Gaussian-weighted interpolation over the pair's own reference entries,
+// src/d3_kernel.cu:709-800 (calculate_c6ab_dual), for parameter tables whose CN references do
+// not factorise per element.  g: the handle's pair table, entry (A, B, a, b) = {C6ref, CN_A ref,
+// CN_B ref} at 3 (((A ne + B) nr + a) nr + b); an entry counts iff both CN refs > -1.  Exponents
+// are shifted by their maximum (C6 = Z / W is unchanged; the shift keeps W from underflowing far
+// from every reference), and the C6 references by that entry's c0: C6 = c0 + Z'/W with Z' =
+// sum (C6ref - c0) L, dC6/dCN = (Z'_1 W - Z' W_1) / W^2 — the same values, but when one entry
+// carries nearly all the weight (a CN far from the others) the derivative is no longer the
+// difference of two nearly equal products, which in FP32 left noise of ~1e-7 C6 per unit CN.
+// Returns C6, dC6/dCN_A, dC6/dCN_B; non-finite results become 0.
+template <class T>
+__device__ void c6_pairwise(const T *g, int ne, int nr, int A, int B, T cna, T cnb, T &c6, T &d1, T &d2) {
+    const T *e = g + (size_t)3 * ((A * ne + B) * nr * nr);
+    const int m = nr * nr;
+    T mx = T(-1e30), c0 = T(0);
+    for (int q = 0; q < m; ++q) {
+        const T r1 = e[3 * q + 1], r2 = e[3 * q + 2];
+        if (r1 > T(-1) && r2 > T(-1)) {
+            const T u = cna - r1, v = cnb - r2;
+            const T x = T(-K3) * (u * u + v * v);
+            if (x > mx) {
+                mx = x;
+                c0 = e[3 * q];
+            }
+        }
+    }
+    T Z = T(0), W = T(0), cz1 = T(0), cz2 = T(0), dw1 = T(0), dw2 = T(0);
+    for (int q = 0; q < m; ++q) {
+        const T cref = e[3 * q] - c0, r1 = e[3 * q + 1], r2 = e[3 * q + 2];
+        if (r1 > T(-1) && r2 > T(-1)) {
+            const T u = cna - r1, v = cnb - r2;
+            const T L = xexp(T(-K3) * (u * u + v * v) - mx);
+            const T l1 = T(-2 * K3) * u * L, l2 = T(-2 * K3) * v * L;
+            Z += cref * L;
+            W += L;
+            cz1 += cref * l1;
+            cz2 += cref * l2;
+            dw1 += l1;
+            dw2 += l2;
+        }
+    }
+    const T W2 = W * W;
+    d1 = finite_or_zero(W2 > T(0) ? (cz1 * W - Z * dw1) / W2 : T(0));
+    d2 = finite_or_zero(W2 > T(0) ? (cz2 * W - Z * dw2) / W2 : T(0));
+    c6 = finite_or_zero(W > T(0) ? c0 + Z / W : T(0));
+}
+
 // Per-lane accumulator: plain sum in FP64; two-sum compensated (hi, lo) in FP32.
 template <class T>
 struct Acc {

@@ -86,6 +86,11 @@ struct Params {
     // q4[ne][ne] = r2r4_A r2r4_B, rdamp[ne][ne] (zero: r0ab; BJ: a1 sqrt(3 q4) + a2),
     // r0[ne][ne] (ATM damping radii)
     const T *c6, *cnref, *rcov, *q4, *rdamp, *r0;
+    // Tables whose CN references do not factorise per element: c6g[ne][ne][nref][nref][3] =
+    // {C6ref, CN_A ref, CN_B ref} of each pair entry, and the pair kernels interpolate per pair
+    // (c6_pairwise) instead of through the per-atom weights w, dw.
+    const T *c6g;
+    int c6_pairwise;
     // ATM: per-block scratch for one centre's forward neighbours (capacity atm_cap):
     //   atm_i32: j, type (2 x cap); atm_T: record fields (ATM_NF x cap: offset, CN log-derivatives)
     //   and per tile of 64 slots its bounding box (8 values: min, max per axis, 2 unused);

@@ -187,6 +187,12 @@ Engine::Engine(const uint16_t *elements, uint64_t num_elements, uint64_t max_len
     std::vector<double> c6((size_t)ne * ne * nr * nr, 0.0), cnref((size_t)ne * nr, -1.0);
     for (int A = 0; A < ne; ++A)
         for (int a = 0; a < nr; ++a) cnref[A * nr + a] = tcnref[A * tnr + a];
+    // Factorisation: every entry (A, B, a, b) valid iff references a of A and b of B are, with CN
+    // references equal to the per-element ones.  Otherwise C6 is interpolated per pair over the
+    // pair's own entries, as src/d3_kernel.cu:709-800 does for every pair.
+    const char *pw_env = std::getenv("D3_C6_PAIRWISE");
+    c6_pairwise_ = pw_env && std::strcmp(pw_env, "1") == 0;
+    int nr_pw = 1;  // references the pairwise table needs: the last valid index of any entry
     for (int A = 0; A < ne; ++A)
         for (int B = 0; B < ne; ++B)
             for (int a = 0; a < tnr; ++a)
@@ -194,12 +200,47 @@ Engine::Engine(const uint16_t *elements, uint64_t num_elements, uint64_t max_len
                     const size_t ix = tab.c6_index(tab.slot(elems_[A]), tab.slot(elems_[B]), a, b);
                     const bool valid = tab.c6ab[ix + 1] > -1.0f && tab.c6ab[ix + 2] > -1.0f;
                     const bool fvalid = tcnref[A * tnr + a] > -1.0 && tcnref[B * tnr + b] > -1.0;
+                    if (valid) nr_pw = std::max(nr_pw, std::max(a, b) + 1);
                     if (valid != fvalid || (valid && ((double)tab.c6ab[ix + 1] != tcnref[A * tnr + a] ||
                                                       (double)tab.c6ab[ix + 2] != tcnref[B * tnr + b])))
-                        fail(D3X_ERR_INPUT, "parameter table CN references do not factorise per element");
-                    if (valid) c6[(((size_t)A * ne + B) * nr + a) * nr + b] = tab.c6ab[ix + 0];
+                        c6_pairwise_ = true;
+                    if (valid && fvalid && a < nr && b < nr) c6[(((size_t)A * ne + B) * nr + a) * nr + b] = tab.c6ab[ix + 0];
                 }
-    if (rc3_ > 0 && atm_lds_bytes(ne_, nref_, yrows_, precision == D3X_FP64 ? 8 : 4, 0) > 65536)
+    h_c6g_.clear();
+    if (c6_pairwise_) {
+        // the per-pair table over nr_pw references; the per-element rows stay for the CN weights
+        // (computed, unused by the pairwise kernels); the three-body tile stages one row (CN_j)
+        nref_ = std::max(nref_, nr_pw);
+        const int np = nref_;
+        h_c6g_.assign((size_t)3 * ne * ne * np * np, 0.0);
+        for (int A = 0; A < ne; ++A)
+            for (int B = 0; B < ne; ++B)
+                for (int a = 0; a < np; ++a)
+                    for (int b = 0; b < np; ++b) {
+                        double *o = &h_c6g_[3 * ((((size_t)A * ne + B) * np + a) * np + b)];
+                        o[1] = o[2] = -1.0;
+                        if (a >= tnr || b >= tnr) continue;
+                        const size_t ix = tab.c6_index(tab.slot(elems_[A]), tab.slot(elems_[B]), a, b);
+                        if (tab.c6ab[ix + 1] > -1.0f && tab.c6ab[ix + 2] > -1.0f)
+                            for (int q = 0; q < 3; ++q) o[q] = tab.c6ab[ix + q];
+                    }
+        if (np != nr) {  // per-element rows at the wider reference count
+            std::vector<double> c6w((size_t)ne * ne * np * np, 0.0), cnw((size_t)ne * np, -1.0);
+            for (int A = 0; A < ne; ++A)
+                for (int a = 0; a < nr; ++a) cnw[A * np + a] = cnref[A * nr + a];
+            for (int A = 0; A < ne; ++A)
+                for (int B = 0; B < ne; ++B)
+                    for (int a = 0; a < nr; ++a)
+                        for (int b = 0; b < nr; ++b)
+                            c6w[(((size_t)A * ne + B) * np + a) * np + b] = c6[(((size_t)A * ne + B) * nr + a) * nr + b];
+            c6.swap(c6w);
+            cnref.swap(cnw);
+        }
+        atm_roff_.assign(ne + 2, 0);
+        atm_roff_[ne + 1] = 1;
+        yrows_ = 1;
+    }
+    if (rc3_ > 0 && atm_lds_bytes(ne_, knr(), yrows_, precision == D3X_FP64 ? 8 : 4, 0) > 65536)
         fail(D3X_ERR_INPUT, "too many distinct elements for the three-body kernel's LDS tile");
     // layout of h_tab_: c6 | cnref | rcov | r2r4 | r0ab  (q4 / rdamp derived at upload)
     h_tab_.clear();
@@ -677,6 +718,8 @@ Params<T> Engine::params() {
     P.q4 = tab + tab_off_[3];
     P.rdamp = tab + tab_off_[4];
     P.r0 = tab + tab_off_[5];
+    P.c6g = c6_pairwise_ ? tab + tab_off_[6] : nullptr;
+    P.c6_pairwise = c6_pairwise_ ? 1 : 0;
     P.atm_cap = atm_cap_;
     P.atm_var = rc3_ > 0 ? atm_var() : 0;
     P.atm_chunks = atm_chunks_;
@@ -739,6 +782,8 @@ void Engine::run() {
             }
         tab_off_[5] = t.size();
         for (int k = 0; k < ne * ne; ++k) t.push_back((T)r0ab[k]);
+        tab_off_[6] = t.size();
+        for (double v : h_c6g_) t.push_back((T)v);
         dfree(d_tab);
         d_tab = dalloc(t.size() * sizeof(T));
         HIPCHK(hipMemcpyAsync(d_tab, t.data(), t.size() * sizeof(T), hipMemcpyHostToDevice, stream_));
@@ -836,8 +881,9 @@ std::string Engine::tune_key() const {
         if (c.set) dens = std::max(dens, (double)c.n / std::fabs(det3(c.H)));
     const int db = dens > 0 ? (int)std::lround(4.0 * std::log2(dens)) : -999;
     char buf[160];
-    std::snprintf(buf, sizeof(buf), "%s|rc%.2f|cn%.2f|atm%.2f|d%d|ne%d|cells%s",
-                  precision_ == D3X_FP64 ? "fp64" : "fp32", rc2_, rccn_, rc3_, db, ne_, sys_.size() > 1 ? "N" : "1");
+    std::snprintf(buf, sizeof(buf), "%s|rc%.2f|cn%.2f|atm%.2f|d%d|ne%d|cells%s%s",
+                  precision_ == D3X_FP64 ? "fp64" : "fp32", rc2_, rccn_, rc3_, db, ne_, sys_.size() > 1 ? "N" : "1",
+                  c6_pairwise_ ? "|c6pair" : "");
     return buf;
 }
 
@@ -925,8 +971,7 @@ void Engine::apply_tune_cache() {
     if (cache_lookup(tune_cache_, tune_key(), w, b, v)) {
         bin_width_ = w;
         atm_blocks_req_ = b;
-        const size_t ts = precision_ == D3X_FP64 ? sizeof(double) : sizeof(float);
-        if (v >= 0 && v <= 3 && atm_lds_bytes(ne_, nref_, yrows_, ts, v) <= 65536) set_atm_variant(v);
+        if (atm_variant_ok(v)) set_atm_variant(v);
     }
 }
 
@@ -985,12 +1030,11 @@ double Engine::autotune(const std::string &cache_path) {
     geom_dirty_ = true;
     if (rc3_ > 0) {
         // three-body kernel variants (KL, DR) that fit, each at its own resident block count
-        const size_t ts = precision_ == D3X_FP64 ? sizeof(double) : sizeof(float);
         const int req = atm_blocks_req_;
         int best_v = atm_var();
         best = 1e300;
         for (int v = 0; v < 4; ++v) {
-            if (atm_lds_bytes(ne_, nref_, yrows_, ts, v) > 65536) continue;
+            if (!atm_variant_ok(v)) continue;
             set_atm_variant(v);
             atm_blocks_req_ = 0;
             const double t = timed_evaluation();
@@ -1043,6 +1087,8 @@ void Engine::set_atm_mode(int mode, int judge_every, double tol_ev_per_atom) {
     if (mode != D3X_ATM_EXACT && mode != D3X_ATM_SURROGATE && mode != D3X_ATM_EXACT_FEATURES)
         fail(D3X_ERR_INPUT, "unknown three-body mode");
     if (mode != D3X_ATM_EXACT && rc3_ <= 0) fail(D3X_ERR_INPUT, "the handle has no three-body term (atm_cutoff <= 0)");
+    if (mode != D3X_ATM_EXACT && c6_pairwise_)  // its features weight neighbours by per-element C6 rows
+        fail(D3X_ERR_INPUT, "the three-body network needs CN references that factorise per element");
     if (mode == D3X_ATM_SURROGATE && h_sur_P_.empty()) fail(D3X_ERR_STATE, "no network parameters loaded");
     if (judge_every < 0) fail(D3X_ERR_INPUT, "judge_every must be >= 0");
     if (!(tol_ev_per_atom > 0)) fail(D3X_ERR_INPUT, "tolerance must be positive");

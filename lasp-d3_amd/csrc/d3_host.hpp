@@ -106,8 +106,17 @@ class Engine {
     int atm_blocks_effective() { return atm_blocks_req_ > 0 ? atm_blocks_req_ : atm_resident(); }
     int atm_var() {
         if (atm_var_ < 0)
-            atm_var_ = atm_choose_variant(ne_, nref_, yrows_, precision_ == 0 ? sizeof(double) : sizeof(float));
+            atm_var_ = atm_choose_variant(ne_, knr(), yrows_, precision_ == 0 ? sizeof(double) : sizeof(float));
         return atm_var_;
+    }
+    // Reference count the pair kernels are instantiated for: 0 = pairwise C6 interpolation
+    // (tables whose CN references do not factorise per element).
+    int knr() const { return c6_pairwise_ ? 0 : nref_; }
+    bool c6_pairwise() const { return c6_pairwise_; }
+    // A three-body variant this handle can run: its LDS fits, and no DR rows with pairwise C6.
+    bool atm_variant_ok(int v) const {
+        return v >= 0 && v <= 3 && !(c6_pairwise_ && (v & ATM_DR)) &&
+               atm_lds_bytes(ne_, knr(), yrows_, precision_ == 0 ? sizeof(double) : sizeof(float), v) <= 65536;
     }
 
     // Three-body stand-in (modes of include/d3x_sur.h: 0 exact, 1 network with judge, 2 exact +
@@ -186,6 +195,10 @@ class Engine {
     int ne_ = 0, nref_ = 0;
     std::vector<int> atm_roff_;  // three-body j-tile row offsets per type (Params::atm_roff)
     int yrows_ = 0;
+    // CN references that do not factorise per element (or D3_C6_PAIRWISE=1): C6 interpolated per
+    // pair from h_c6g_ = {C6ref, CN_A ref, CN_B ref}[ne][ne][nref][nref] (Params::c6g)
+    bool c6_pairwise_ = false;
+    std::vector<double> h_c6g_;
     uint64_t max_len_ = 0;
     double rc2_, rccn_, rc3_;
     int damping_, functional_, precision_;
@@ -223,7 +236,7 @@ class Engine {
          *d_partial = nullptr, *d_out_sys = nullptr;
     size_t chunk_cap_ = 0, sysred_cap_ = 0;
     void *d_tab = nullptr;
-    size_t tab_off_[6] = {0};
+    size_t tab_off_[7] = {0};  // c6 | cnref | rcov | q4 | rdamp | r0 | c6g (pairwise only)
     void *d_status = nullptr;
     void *d_atm_i32 = nullptr, *d_atm_T = nullptr, *d_atm_acc = nullptr;
     void *d_atm_pa = nullptr;  // per-atom ATM results: fa[3N] | ga[N] | eas[N] | eat[N] (double)
@@ -237,7 +250,7 @@ class Engine {
     int atm_resident() {
         if (atm_resident_ <= 0)
             atm_resident_ =
-                atm_resident_blocks(ne_, nref_, yrows_, precision_ == 0 ? sizeof(double) : sizeof(float), atm_var());
+                atm_resident_blocks(ne_, knr(), yrows_, precision_ == 0 ? sizeof(double) : sizeof(float), atm_var());
         return atm_resident_;
     }
     size_t atm_alloc_entries_ = 0;  // allocated scratch entries (>= atm_blocks_ * atm_cap_)

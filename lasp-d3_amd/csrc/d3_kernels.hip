@@ -322,19 +322,25 @@ __device__ __forceinline__ float prsqrt(float x) { return __builtin_amdgcn_rsqf(
 __device__ __forceinline__ double prsqrt(double x) { return rsqrt(x); }
 
 // NR = the handle's reference count: the weight gathers of a pair are unrolled and issued
-// together.
+// together.  NR = 0: tables that do not factorise per element; C6 and dC6/dCN_i are interpolated
+// per pair from the pair's reference entries (c6_pairwise) at the CNs of i (cni) and j.
 template <class T, int NR>
 __device__ __forceinline__ void pair_2b(const Params<T> &P, int j, T dx, T dy, T dz, const T *y, const T *dy_,
-                                        const T *rq, const T *rR, const T *rX, Acc2b<T> &A) {
+                                        const T *rq, const T *rR, const T *rX, int ti, T cni, Acc2b<T> &A) {
     const int tj = P.type_s[j];
-    T wj[NR];
-#pragma unroll
-    for (int b = 0; b < NR; ++b) wj[b] = P.w[b * P.n + j];
     T c6 = T(0), dc6 = T(0);
+    if constexpr (NR == 0) {
+        T dc6j;
+        c6_pairwise(P.c6g, P.ne, P.nref, ti, tj, cni, P.cn[j], c6, dc6, dc6j);
+    } else {
+        T wj[NR];
 #pragma unroll
-    for (int b = 0; b < NR; ++b) {
-        c6 += y[tj * NR + b] * wj[b];
-        dc6 += dy_[tj * NR + b] * wj[b];
+        for (int b = 0; b < NR; ++b) wj[b] = P.w[b * P.n + j];
+#pragma unroll
+        for (int b = 0; b < NR; ++b) {
+            c6 += y[tj * NR + b] * wj[b];
+            dc6 += dy_[tj * NR + b] * wj[b];
+        }
     }
     const T r2 = dx * dx + dy * dy + dz * dz;
     T e, g, phi;  // E+_i share (half the pair), dE+/dCN_i, (1/r) dE+/dr
@@ -391,21 +397,24 @@ __global__ __launch_bounds__(BLOCK) void k_2b(Params<T> P) {
 
     Centre<T> C;
     C.load(P, i);
-    T wi[NR], dwi[NR];
-    for (int a = 0; a < NR; ++a) {
-        wi[a] = P.w[a * P.n + i];
-        dwi[a] = P.dw[a * P.n + i];
-    }
-    for (int idx = lane; idx < ynr; idx += WAVE) {
-        const int B = idx / NR, bb = idx % NR;
-        T sy = T(0), sdy = T(0);
+    const T cni = NR == 0 ? P.cn[i] : T(0);
+    if constexpr (NR > 0) {
+        T wi[NR], dwi[NR];
         for (int a = 0; a < NR; ++a) {
-            const T c = P.c6[((C.ti * P.ne + B) * NR + a) * NR + bb];
-            sy += c * wi[a];
-            sdy += c * dwi[a];
+            wi[a] = P.w[a * P.n + i];
+            dwi[a] = P.dw[a * P.n + i];
         }
-        y[idx] = sy;
-        dyv[idx] = sdy;
+        for (int idx = lane; idx < ynr; idx += WAVE) {
+            const int B = idx / NR, bb = idx % NR;
+            T sy = T(0), sdy = T(0);
+            for (int a = 0; a < NR; ++a) {
+                const T c = P.c6[((C.ti * P.ne + B) * NR + a) * NR + bb];
+                sy += c * wi[a];
+                sdy += c * dwi[a];
+            }
+            y[idx] = sy;
+            dyv[idx] = sdy;
+        }
     }
     for (int B = lane; B < P.ne; B += WAVE) {
         const T q = P.q4[C.ti * P.ne + B], R = P.rdamp[C.ti * P.ne + B];
@@ -458,7 +467,7 @@ __global__ __launch_bounds__(BLOCK) void k_2b(Params<T> P) {
                 qn += __popcll(m);
                 if (qn >= WAVE) {
                     wave_sync();
-                    pair_2b<T, NR>(P, Q.j[lane], Q.dx[lane], Q.dy[lane], Q.dz[lane], y, dyv, rq, rR, rX, A);
+                    pair_2b<T, NR>(P, Q.j[lane], Q.dx[lane], Q.dy[lane], Q.dz[lane], y, dyv, rq, rR, rX, C.ti, cni, A);
                     cnt += 1.0f;
                     const int rest = qn - WAVE;
                     int mj = 0;
@@ -484,7 +493,7 @@ __global__ __launch_bounds__(BLOCK) void k_2b(Params<T> P) {
     }
     wave_sync();
     if (lane < qn) {
-        pair_2b<T, NR>(P, Q.j[lane], Q.dx[lane], Q.dy[lane], Q.dz[lane], y, dyv, rq, rR, rX, A);
+        pair_2b<T, NR>(P, Q.j[lane], Q.dx[lane], Q.dy[lane], Q.dz[lane], y, dyv, rq, rR, rX, C.ti, cni, A);
         cnt += 1.0f;
     }
     A.fold();
@@ -629,8 +638,9 @@ const void *twobody_pick(int nref, std::integer_sequence<int, NR...>) {
 template <class T>
 void Launch<T>::twobody(const Params<T> &P, hipStream_t s) {
     if (P.n == 0) return;
-    const size_t lds = WPB * twobody_wave_lds<T>(P.ne, P.nref);
-    const void *fn = twobody_pick<T>(P.nref, std::make_integer_sequence<int, MAX_NREF>{});
+    const size_t lds = WPB * twobody_wave_lds<T>(P.ne, P.c6_pairwise ? 0 : P.nref);
+    const void *fn = P.c6_pairwise ? reinterpret_cast<const void *>(&k_2b<T, 0>)
+                                   : twobody_pick<T>(P.nref, std::make_integer_sequence<int, MAX_NREF>{});
     if (!fn) return;  // not reached: a handle's reference count is 1..MAX_NREF
     Params<T> arg = P;
     void *args[] = {&arg};

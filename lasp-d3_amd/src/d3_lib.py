@@ -67,6 +67,7 @@ SIGNATURES = {
     "d3x_set_tuning": (C.c_int, [_vp, C.c_double, C.c_int]),
     "d3x_get_tuning": (C.c_int, [_vp, _dp, _ip]),
     "d3x_get_atm_variant": (C.c_int, [_vp]),
+    "d3x_get_c6_pairwise": (C.c_int, [_vp]),
     "d3x_get_atm_capacity": (C.c_int, [_vp]),
     "d3x_set_cn_smooth": (C.c_int, [_vp, C.c_int]),
     "d3x_set_skin": (C.c_int, [_vp, C.c_double]),
@@ -258,6 +259,11 @@ class Handle:
         if v < 0:
             raise D3Error("d3x_get_atm_variant", v)
         return v
+
+    def c6_pairwise(self) -> bool:
+        """True when C6 is interpolated per pair (a table whose CN references do not factorise
+        per element, or D3_C6_PAIRWISE=1 at init)."""
+        return self.lib.d3x_get_c6_pairwise(self.h) == 1
 
     def atm_capacity(self) -> int:
         """Forward-neighbour capacity per centre of the three-body scratch."""

@@ -29,7 +29,7 @@ def declared_functions():
 
 def test_every_declared_symbol_is_exported():
     names = declared_functions()
-    assert len(names) == 56  # 9 reference entry points + 30 extensions + 17 of the three-body stand-in
+    assert len(names) == 57  # 9 reference entry points + 31 extensions + 17 of the three-body stand-in
     out = subprocess.run(["nm", "-D", "--defined-only", d3_lib.LIB_PATH], capture_output=True, text=True,
                          check=True).stdout
     exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
@@ -129,10 +129,11 @@ def test_table_file_round_trip(tmp_path):
     assert d3_lib.table()["name"] == "synthetic-original"
 
 
-def test_non_factorising_table_is_rejected(tmp_path):
-    """The kernels hold one set of CN references per element (entry [1] of c6ab[A][B][a][b] equal
-    for every partner B, as in the D3 reference tables); a table that breaks this is refused when
-    a handle over the affected elements is made, before any device use."""
+def test_non_factorising_table_passes_the_table_checks(tmp_path):
+    """A table whose CN references are not one set per element (entry [1] of c6ab[A][B][a][b]
+    differing between partners B) switches the handle to per-pair C6 interpolation instead of
+    being refused: here, without a device, init gets past every table check and fails only at
+    the first device call.  (The GPU tests evaluate such tables against the oracle.)"""
     import pack_table
     t0 = d3_lib.table()
     c6 = t0["c6ab"].copy()
@@ -144,8 +145,11 @@ def test_non_factorising_table_is_rejected(tmp_path):
         d3_lib.load_table(path)
         el = np.array([6, 8], np.uint16)
         h = lib.d3x_init(d3_lib.ptr(el, C.c_uint16), 2, 16, 40.0, 40.0, 0.0, 1, 0, 0)
-        assert not h
-        assert "factorise" in d3_lib.last_error()
+        if not h:  # no device here
+            err = d3_lib.last_error()
+            assert "factorise" not in err and "table" not in err, err
+        else:
+            lib.free_d3_handle(h)
     finally:
         d3_lib.load_table(None)
 
