@@ -14,7 +14,8 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(_HERE, "libd3_oracle.so")
+# D3_ORACLE_LIBRARY: another build of the same source (the sanitizer build of oracle/Makefile)
+LIB = os.environ.get("D3_ORACLE_LIBRARY", os.path.join(_HERE, "libd3_oracle.so"))
 _lib = None
 
 # {s6, s8_zero, s8_bj, sr6, sr8, a1, a2} of the reference's FUNCTIONAL_PARAMS (src/d3_types.h:46-54),

@@ -13,7 +13,7 @@ import d3_lib
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 GOLDEN = os.path.join(ROOT, "tests", "golden", "tio2_192.extxyz")
-CLI = os.path.join(ROOT, "lasp-d3_amd", "build", "d3_cpp")
+CLI = os.environ.get("D3_CLI", os.path.join(ROOT, "lasp-d3_amd", "build", "d3_cpp"))
 
 
 def test_reads_reference_example():
