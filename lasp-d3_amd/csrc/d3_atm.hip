@@ -238,8 +238,9 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
             }
         } else {
             const int s0 = C.S->st_off[CUT_ATM], s1 = s0 + C.S->st_cnt[CUT_ATM];
+            const T rc3 = xsqrt(rc32);
             for (int sb = s0; sb < s1; sb += AW) {
-                const auto Bn = C.bins(P, sb, s1);
+                const auto Bn = C.bins(P, sb, s1, rc3);
                 const int nsb = s1 - sb < AW ? s1 - sb : AW;
                 for (int sk = 0; sk < nsb; ++sk) {
                     int j0, j1;
@@ -688,8 +689,9 @@ __global__ __launch_bounds__(256) void k_atm_fwd(Params<T> P) {
     const T rcf = P.rc_fl2;
     int count = 0;
     const int s0 = C.S->st_off[CUT_ATM], s1 = s0 + C.S->st_cnt[CUT_ATM];
+    const T rcfl = xsqrt(rcf);
     for (int sb = s0; sb < s1; sb += AW) {
-        const auto Bn = C.bins(P, sb, s1);
+        const auto Bn = C.bins(P, sb, s1, rcfl);
         const int nsb = s1 - sb < AW ? s1 - sb : AW;
         for (int sk = 0; sk < nsb; ++sk) {
             int j0, j1;

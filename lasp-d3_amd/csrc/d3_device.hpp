@@ -211,10 +211,31 @@ struct Centre {
             pz = readlane_t(oz, k);
         }
     };
-    __device__ Bins bins(const Params<T> &P, int s, int s_end) const {
+    // rc: the cutoff the caller tests pairs against.  A stencil entry whose bin no atom within rc
+    // of this centre can occupy comes back empty (j1 = j0): the stencil holds every bin that some
+    // point of the centre's bin can reach, this test the bins the centre atom itself can.  Lower
+    // bound of |r_ij| for j in the bin at image offset O = r_i + D from the bin centre: |O . n_k|
+    // - hh_k along each lattice-plane normal n_k (in quadrature for an orthorhombic cell, the
+    // largest otherwise), and |O| - hdiag; pad covers atoms that moved out of their bin within
+    // the skin, and rounding.
+    __device__ Bins bins(const Params<T> &P, int s, int s_end, T rc) const {
         Bins r{0, 0, T(0), T(0), T(0)};
         const int t = s + lane_id();
-        if (t < s_end) bin(P, t, r.j0, r.j1, r.ox, r.oy, r.oz);
+        if (t < s_end) {
+            bin(P, t, r.j0, r.j1, r.ox, r.oy, r.oz);
+            const double ox = (double)r.ox, oy = (double)r.oy, oz = (double)r.oz;
+            double q2 = 0.0, qm = 0.0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const double p = fabs(ox * S->nrm[3 * k] + oy * S->nrm[3 * k + 1] + oz * S->nrm[3 * k + 2]) - S->hh[k];
+                const double g = p > 0.0 ? p : 0.0;
+                q2 += g * g;
+                qm = fmax(qm, g);
+            }
+            const double sph = sqrt(ox * ox + oy * oy + oz * oz) - S->hdiag;
+            const double lb = S->ortho ? sqrt(q2) : fmax(qm, sph);
+            if (lb > (double)rc + S->pad) r.j1 = r.j0;
+        }
         return r;
     }
 };

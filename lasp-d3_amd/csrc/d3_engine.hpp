@@ -48,6 +48,16 @@ struct SysDev {
     int n_atoms;
     int st_off[NCUT];  // stencil ranges per cutoff
     int st_cnt[NCUT];
+    // Bin shape, for skipping stencil bins that no atom within a cutoff of the centre atom can
+    // occupy: unit normals of the lattice planes (nrm[3 k .. 3 k + 2] for axis k), half the bin
+    // height along each, half the longest bin diagonal, and pad = how far an atom may sit outside
+    // its bin (half the neighbour skin) plus a rounding margin.  ortho: the cell is orthorhombic
+    // (normals orthogonal, the per-axis gaps add in quadrature).
+    double nrm[9];
+    double hh[3];
+    double hdiag;
+    double pad;
+    int ortho;
 };
 
 // Device pointers and scalars of one evaluation; passed by value to every kernel.

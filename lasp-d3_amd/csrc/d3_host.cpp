@@ -508,6 +508,30 @@ void Engine::build_geometry() {
             nbt *= d.nb[k];
         }
         if (nbt > (1 << 22)) fail(D3X_ERR_INPUT, "cell too large for the bin grid");
+        {
+            // bin shape for the per-atom bin skip (Centre::bins)
+            bool ortho = true;
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c)
+                    if (r != c && C.H[3 * r + c] != 0.0) ortho = false;
+            for (int k = 0; k < 3; ++k) {
+                const double nk = std::sqrt(d.Hinv[0 * 3 + k] * d.Hinv[0 * 3 + k] + d.Hinv[1 * 3 + k] * d.Hinv[1 * 3 + k] +
+                                            d.Hinv[2 * 3 + k] * d.Hinv[2 * 3 + k]);
+                for (int c = 0; c < 3; ++c) d.nrm[3 * k + c] = d.Hinv[c * 3 + k] / nk;
+                d.hh[k] = 0.5 / (nk * d.nb[k]);
+            }
+            double diag = 0;
+            for (int s1 = -1; s1 <= 1; s1 += 2)
+                for (int s2 = -1; s2 <= 1; s2 += 2) {
+                    double v[3];
+                    for (int c = 0; c < 3; ++c)
+                        v[c] = C.H[c] / d.nb[0] + s1 * C.H[3 + c] / d.nb[1] + s2 * C.H[6 + c] / d.nb[2];
+                    diag = std::max(diag, std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
+                }
+            d.hdiag = 0.5 * diag;
+            d.pad = 0.5 * skin_ + 1e-3;
+            d.ortho = ortho ? 1 : 0;
+        }
         d.bin_base = bin_base;
         d.atom_base = atom_base;
         d.n_atoms = (int)C.n;

@@ -187,7 +187,7 @@ __global__ __launch_bounds__(BLOCK) void k_cn(Params<T> P) {
     float cnt = 0.0f;
     const int s0 = C.S->st_off[CUT_CN], s1 = s0 + C.S->st_cnt[CUT_CN];
     for (int sb = s0; sb < s1; sb += WAVE) {
-        const auto Bn = C.bins(P, sb, s1);
+        const auto Bn = C.bins(P, sb, s1, P.rc_cn);
         const int nsb = s1 - sb < WAVE ? s1 - sb : WAVE;
         for (int sk = 0; sk < nsb; ++sk) {
             int j0, j1;
@@ -437,8 +437,9 @@ __global__ __launch_bounds__(BLOCK) void k_2b(Params<T> P) {
     // this wave's share of the stencil (all of it with one part per centre)
     const int sa = C.S->st_off[CUT_2B], sz = C.S->st_cnt[CUT_2B], per = (sz + nparts - 1) / nparts;
     const int s0 = sa + min(sz, part * per), s1 = sa + min(sz, (part + 1) * per);
+    const T rc2b = xsqrt(P.rc_22);
     for (int sb = s0; sb < s1; sb += WAVE) {
-        const auto Bn = C.bins(P, sb, s1);
+        const auto Bn = C.bins(P, sb, s1, rc2b);
         const int nsb = s1 - sb < WAVE ? s1 - sb : WAVE;
         for (int sk = 0; sk < nsb; ++sk) {
             int j0, j1;
@@ -558,7 +559,7 @@ __global__ __launch_bounds__(BLOCK) void k_chain(Params<T> P) {
     for (int k = 0; k < 6; ++k) V[k].init();
     const int s0 = C.S->st_off[CUT_CN], s1 = s0 + C.S->st_cnt[CUT_CN];
     for (int sb = s0; sb < s1; sb += WAVE) {
-        const auto Bn = C.bins(P, sb, s1);
+        const auto Bn = C.bins(P, sb, s1, P.rc_cn);
         const int nsb = s1 - sb < WAVE ? s1 - sb : WAVE;
         for (int sk = 0; sk < nsb; ++sk) {
             int j0, j1;

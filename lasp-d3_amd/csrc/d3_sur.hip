@@ -116,7 +116,7 @@ __device__ void scan_neighbours(const Params<T> &P, const Centre<T> &C, int lane
     const T rc2 = T(RCF * RCF);
     const int s0 = C.S->st_off[CUT_SUR], s1 = s0 + C.S->st_cnt[CUT_SUR];
     for (int sb = s0; sb < s1; sb += WAVE) {
-        const auto Bn = C.bins(P, sb, s1);
+        const auto Bn = C.bins(P, sb, s1, T(RCF));
         const int nsb = s1 - sb < WAVE ? s1 - sb : WAVE;
         for (int sk = 0; sk < nsb; ++sk) {
             int j0, j1;
@@ -315,7 +315,7 @@ __global__ __launch_bounds__(BLOCK) void k_sur_dcn(Params<T> P, const float *__r
     float cacc = 0.f;
     const int s0 = C.S->st_off[CUT_CN], s1 = s0 + C.S->st_cnt[CUT_CN];
     for (int sb = s0; sb < s1; sb += WAVE) {
-        const auto Bn = C.bins(P, sb, s1);
+        const auto Bn = C.bins(P, sb, s1, P.rc_cn);
         const int nsb = s1 - sb < WAVE ? s1 - sb : WAVE;
         for (int sk = 0; sk < nsb; ++sk) {
             int j0, j1;
