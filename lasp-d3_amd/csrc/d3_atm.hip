@@ -454,8 +454,12 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                 }
                 wave_sync();
 
-                // tile-local sums of neighbour k (folded into FP64 per tile)
+                // tile-local sums of neighbour k (folded into FP64 per tile).  The CN term of k
+                // is kept per reference b, kGb[b] = sum (e/2C6) y_b(j), and contracted with k's
+                // dw_b once per tile: one FMA per pair fewer than forming dC6_jk/dCN_k per pair.
                 T kA = T(0), kBx = T(0), kBy = T(0), kBz = T(0), kH = T(0), kG = T(0);
+                T kGb[NR > 0 ? NR : 1];
+                for (int bb = 0; bb < NR; ++bb) kGb[bb] = T(0);
                 // On the diagonal tile (jt == kb) every unordered pair {j, k} comes up at two
                 // steps, q and 64 - q; the terms are symmetric in j and k, so steps 1..31 and the
                 // lower half of the lanes at step 32 take each pair once (32 steps, not 64).  The
@@ -464,12 +468,24 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                 auto steps = [&](auto diag) {
                 constexpr bool DIAG = decltype(diag)::value;
                 constexpr int Q0 = DIAG ? 1 : 0, Q1 = DIAG ? AW / 2 + 1 : AW;
+                // slot (lane + q) mod 64 as a byte offset into any row of 64 slots: one add and
+                // one mask per step, and every row (the C6 rows of type(k) too) is that offset
+                // plus a base
+                constexpr int SB = (int)sizeof(T2<T>);
+                // fixed rows by index from the LDS base (J0 = row 0 ... Jt = row 7, as AtmLds lays
+                // them out), so that the offset needs no base register
+                auto at = [&](int row, int jb) -> T2<T> & {
+                    return *reinterpret_cast<T2<T> *>(smem + row * AW * SB + jb);
+                };
+                auto cat = [](const T2<T> *row, int jb) -> const T2<T> & {
+                    return *reinterpret_cast<const T2<T> *>(reinterpret_cast<const unsigned char *>(row) + jb);
+                };
 #pragma unroll 1
                 for (int q = Q0; q < Q1; ++q) {
                     // FP32: lane l updates the j slot that lane l+1 updated one step earlier
                     if constexpr (F32) wave_sync();
-                    const int jl = (lane + q) & (AW - 1);
-                    const T2<T> j0 = S.J0[jl], j1 = S.J1[jl];
+                    const int jb = (lane * SB + q * SB) & (AW * SB - 1);
+                    const T2<T> j0 = at(0, jb), j1 = at(1, jb);
                     const T jx = j0.x, jy = j0.y, jz = j1.x;
                     const T vx = kx - jx, vy = ky - jy, vz = kz - jz;  // r_j - r_k
                     const T c = vx * vx + vy * vy + vz * vz;
@@ -488,25 +504,28 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                     cnt += __popcll(m);
                     if (!ok) continue;
                     const T a = j1.y, b = kb2;
-                    const T2<T> j2 = S.J2[jl], j3 = S.J3[jl];
+                    const T2<T> j2 = at(2, jb), j3 = at(3, jb);
                     T lamjk;  // damping term of the pair's types (with lam_j)
                     if constexpr (DR) {
-                        lamjk = Yk[jl].x;
+                        lamjk = cat(Yk, jb).x;
                     } else {
-                        const T rt = S.Rt[S.Jt[jl * S.TI] + tk];
+                        const T rt = S.Rt[*reinterpret_cast<const int *>(&at(7, jb)) + tk];
                         lamjk = F32 ? j3.y + rt : j3.y * rt;
                     }
-                    // C6_jk and its CN derivatives from the j row for type(k)
+                    // C6_jk and dC6_jk/dCN_j from the j row for type(k) (the sums start from
+                    // their first product: an FMA onto +0 is not a multiply in IEEE arithmetic)
                     T c6jk = T(0), dk6 = T(0), dj6 = T(0);
+                    T ydx[NR > 0 ? NR : 1];
                     if constexpr (NR == 0) {
+                        const int jl = jb / SB;
                         c6_pairwise(P.c6g, ne, P.nref, S.Jt[jl * S.TI + 1], tk, S.Jyd[jl].x, kcn, c6jk, dj6, dk6);
                     } else {
 #pragma unroll
                         for (int bb = 0; bb < NR; ++bb) {
-                            const T2<T> yd = Yk[(DRO + bb) * AW + jl];
-                            c6jk += yd.x * kw[bb];
-                            dk6 += yd.x * kdw[bb];
-                            dj6 += yd.y * kw[bb];
+                            const T2<T> yd = cat(Yk + (DRO + bb) * AW, jb);
+                            ydx[bb] = yd.x;
+                            c6jk = bb == 0 ? yd.x * kw[0] : c6jk + yd.x * kw[bb];
+                            dj6 = bb == 0 ? yd.y * kw[0] : dj6 + yd.y * kw[bb];
                         }
                     }
                     const T rs = prsqrt(c6jk);
@@ -546,23 +565,25 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                     // read-modify-write (LDS float atomics measured ~6x slower on gfx950); FP64:
                     // LDS atomics (no per-step barrier, measured ~4% faster than the barrier form).
                     if constexpr (F32) {
-                        T2<T> p0 = S.Jp[jl], p1 = S.Jp[AW + jl], p2 = S.Jp[2 * AW + jl];
+                        T2<T> &r0 = at(4, jb), &r1 = at(5, jb), &r2 = at(6, jb);
+                        T2<T> p0 = r0, p1 = r1, p2 = r2;
                         p0.x += tij + tjk;
                         p0.y += tjk * kx;
                         p1.x += tjk * ky;
                         p1.y += tjk * kz;
                         p2.x += hC;
                         p2.y += hc6 * dj6;
-                        S.Jp[jl] = p0;
-                        S.Jp[AW + jl] = p1;
-                        S.Jp[2 * AW + jl] = p2;
+                        r0 = p0;
+                        r1 = p1;
+                        r2 = p2;
                     } else {
-                        atomicAdd(&S.Jp[jl].x, tij + tjk);
-                        atomicAdd(&S.Jp[jl].y, tjk * kx);
-                        atomicAdd(&S.Jp[AW + jl].x, tjk * ky);
-                        atomicAdd(&S.Jp[AW + jl].y, tjk * kz);
-                        atomicAdd(&S.Jp[2 * AW + jl].x, hC);
-                        atomicAdd(&S.Jp[2 * AW + jl].y, hc6 * dj6);
+                        T2<T> &r0 = at(4, jb), &r1 = at(5, jb), &r2 = at(6, jb);
+                        atomicAdd(&r0.x, tij + tjk);
+                        atomicAdd(&r0.y, tjk * kx);
+                        atomicAdd(&r1.x, tjk * ky);
+                        atomicAdd(&r1.y, tjk * kz);
+                        atomicAdd(&r2.x, hC);
+                        atomicAdd(&r2.y, hc6 * dj6);
                     }
                     // neighbour k: registers
                     kA += tik + tjk;
@@ -570,7 +591,12 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                     kBy += tjk * jy;
                     kBz += tjk * jz;
                     kH += hC;
-                    kG += hc6 * dk6;
+                    if constexpr (NR == 0) {
+                        kG += hc6 * dk6;
+                    } else {
+#pragma unroll
+                        for (int bb = 0; bb < NR; ++bb) kGb[bb] += hc6 * ydx[bb];
+                    }
                 }
                 };
                 if (jt == kb)
@@ -579,6 +605,8 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                     steps(std::false_type{});
                 {
                     // fold the tile sums of neighbour k (exact two-sum in FP32, plain in FP64)
+                    if constexpr (NR > 0)
+                        for (int bb = 0; bb < NR; ++bb) kG += kGb[bb] * kdw[bb];
                     const T v[A_NF] = {kA, kBx, kBy, kBz, kH, kG};
                     if constexpr (KL) {
                         for (int q = 0; q < A_NF; ++q) {
