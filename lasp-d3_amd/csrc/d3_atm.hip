@@ -24,7 +24,8 @@
 //   f = 1 / (1 + 6 x^16),  x^6 = y = (4/3)^6 R0_ij^2 R0_ik^2 R0_jk^2 / (abc),  x^16 = y^(8/3)
 //   t_xy = 2 de/d(r_xy^2) = 2 C9 f [(P1 + (8/3) A (1 - f)) / r_xy^2 + (3/8) iR5 dQ/d(r_xy^2)],
 //     P1 = -(3/2)(abc)^(-3/2) - (15/16) Q (abc)^(-5/2),  (6 x^16 f = 1 - f)
-//   grad_j e = -(t_ij + t_jk) d_j + t_jk d_k,  grad_k e = -(t_ik + t_jk) d_k + t_jk d_j,
+//   grad_j e = -(t_ij + t_jk) d_j + t_jk d_k = -t_ij d_j + t_jk v,
+//   grad_k e = -(t_ik + t_jk) d_k + t_jk d_j = -t_ik d_k - t_jk v,  v = d_k - d_j,
 //   grad_i e = -(grad_j e + grad_k e)
 //   de/dCN_x = (C9/2)(dlnC6_xy/dCN_x + dlnC6_xz/dCN_x) A f
 //   each vertex's atom takes e/3 of the energy (the reference's split, src/d3_kernel.cu:1394-1399)
@@ -49,8 +50,9 @@ constexpr float FAR = 1e18f;                    // coordinate of padding lanes (
 // cost a few instructions per slot and tile, against twice the record bytes read back from HBM)
 enum { F_DX, F_DY, F_DZ, F_DLI, F_DLJ };
 static_assert(F_DLJ + 1 == ATM_NF, "ATM_NF record fields per neighbour slot");
-// per-neighbour FP64 accumulators: A_p = sum (t_ip + t_pq), B_p = sum t_pq d_q, H_p = sum e/2,
-// Gc_p = sum (e/2) dlnC6_pq/dCN_p over the triangles {i, p, q} of the centre
+// per-neighbour FP64 accumulators over the triangles {i, p, q} of the centre: A_p = sum t'_ip,
+// B_p = sum t'_pq (d_q - d_p), H_p = sum e/2, Gc_p = sum (e/2) dlnC6_pq/dCN_p, with t' = (2/3) t,
+// so that grad_p = (3/2) (B_p - A_p d_p)
 enum { A_A, A_BX, A_BY, A_BZ, A_H, A_G, A_NF };
 static_assert(A_NF == 6, "atm_acc_stride holds 6 doubles per neighbour slot");
 
@@ -354,6 +356,7 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
             const bool kval = k < nf;
             // padding lanes sit far away, so every pair with them fails the cutoff test
             T kx = T(-FAR), ky = T(-FAR), kz = T(-FAR), kb2 = T(1), kib = T(1), krb = T(1), ksc6 = T(0), klam = T(0);
+            T kfac = T(0);  // FP32: 6 exp2(lam_k)
             T kw[NR > 0 ? NR : 1], kdw[NR > 0 ? NR : 1];
             T kcn = T(0);  // pairwise C6 only
             int tk = 0;
@@ -371,8 +374,19 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                 if constexpr (NR == 0) kcn = P.cn[jk];
                 T sc6, lam;
                 pair_terms(kx, ky, kz, tk, kw, kcn, kb2, kib, krb, sc6, lam);
-                ksc6 = T(-0.5) * sc6;  // -sqrt(C6_ik) / 2: C9 / 2 below
+                ksc6 = T(-0.5) * sc6;  // -sqrt(C6_ik) / 2: C9 / 2 below (pairwise C6 only)
                 klam = F32 ? lam + T(L2K6_83) : T(K6) * lam;
+                // The pair steps form c6'' = sum_b y'_b(j) kw'_b = (C6_ik / 4) C6_ij C6_jk from
+                // j rows staged times C6_ij and k weights times C6_ik / 4, so that C9 / 2 =
+                // -sqrt(c6'') and the log-derivative ratios (dC6_jk/dCN) / C6_jk are unchanged;
+                // kdw (used once per tile) takes the same factor.
+                const T sk = T(0.25) * sc6 * sc6;
+                for (int a = 0; a < NR; ++a) {
+                    kw[a] *= sk;
+                    kdw[a] *= sk;
+                }
+                // FP32: 6 x^16 = exp2(lam_jk - (8/3) log2 r_jk^2) * 6 exp2(lam_k)
+                if constexpr (F32) kfac = T(6) * T(__builtin_amdgcn_exp2f((float)klam));
             }
             // bounding box of the k wave (valid lanes), wave-uniform
             const float kminx = uni((float)LB[8 * kb + 0]), kmaxx = uni((float)LB[8 * kb + 1]);
@@ -439,6 +453,8 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                             const T rt = S.Rt[tj * ne + B];
                             S.Jyd[base * AW + lane] = T2<T>{F32 ? lam + rt : lam * rt, T(0)};
                         }
+                        // rows times C6_ij = sc6^2 (see the k weights)
+                        const T c6ij = sc6 * sc6;
 #pragma unroll
                         for (int bb = 0; bb < NR; ++bb) {
                             T sy = T(0), sdy = T(0);
@@ -447,7 +463,7 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                                 sy += c * wj[q];
                                 sdy += c * dwj[q];
                             }
-                            S.Jyd[(base + DRO + bb) * AW + lane] = T2<T>{sy, sdy};
+                            S.Jyd[(base + DRO + bb) * AW + lane] = T2<T>{sy * c6ij, sdy * c6ij};
                         }
                     }
                     for (int q = 0; q < 3; ++q) S.Jp[q * AW + lane] = T2<T>{T(0), T(0)};
@@ -504,32 +520,38 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                     cnt += __popcll(m);
                     if (!ok) continue;
                     const T a = j1.y, b = kb2;
-                    const T2<T> j2 = at(2, jb), j3 = at(3, jb);
+                    const T2<T> j2 = at(2, jb);
                     T lamjk;  // damping term of the pair's types (with lam_j)
+                    T sc6j = T(0);  // sqrt C6_ij (pairwise C6 only)
                     if constexpr (DR) {
                         lamjk = cat(Yk, jb).x;
+                        if constexpr (NR == 0) sc6j = at(3, jb).x;
                     } else {
+                        const T2<T> j3 = at(3, jb);
                         const T rt = S.Rt[*reinterpret_cast<const int *>(&at(7, jb)) + tk];
                         lamjk = F32 ? j3.y + rt : j3.y * rt;
+                        sc6j = j3.x;
                     }
-                    // C6_jk and dC6_jk/dCN_j from the j row for type(k) (the sums start from
-                    // their first product: an FMA onto +0 is not a multiply in IEEE arithmetic)
-                    T c6jk = T(0), dk6 = T(0), dj6 = T(0);
+                    // C6 and dC6/dCN_j of the pair {j, k} from the j row for type(k) (the sums
+                    // start from their first product: an FMA onto +0 is not a multiply in IEEE
+                    // arithmetic).  With per-element references the rows and weights carry C6_ij
+                    // and C6_ik / 4: c6 = (C6_ik / 4) C6_ij C6_jk, and C9 / 2 = -sqrt(c6).
+                    T c6 = T(0), dk6 = T(0), dj6 = T(0);
                     T ydx[NR > 0 ? NR : 1];
                     if constexpr (NR == 0) {
                         const int jl = jb / SB;
-                        c6_pairwise(P.c6g, ne, P.nref, S.Jt[jl * S.TI + 1], tk, S.Jyd[jl].x, kcn, c6jk, dj6, dk6);
+                        c6_pairwise(P.c6g, ne, P.nref, S.Jt[jl * S.TI + 1], tk, S.Jyd[jl].x, kcn, c6, dj6, dk6);
                     } else {
 #pragma unroll
                         for (int bb = 0; bb < NR; ++bb) {
                             const T2<T> yd = cat(Yk + (DRO + bb) * AW, jb);
                             ydx[bb] = yd.x;
-                            c6jk = bb == 0 ? yd.x * kw[0] : c6jk + yd.x * kw[bb];
+                            c6 = bb == 0 ? yd.x * kw[0] : c6 + yd.x * kw[bb];
                             dj6 = bb == 0 ? yd.y * kw[0] : dj6 + yd.y * kw[bb];
                         }
                     }
-                    const T rs = prsqrt(c6jk);
-                    const T C9h = (j3.x * ksc6) * (c6jk * rs);  // C9 / 2
+                    const T rs = prsqrt(c6);
+                    const T C9h = NR == 0 ? (sc6j * ksc6) * (c6 * rs) : -(c6 * rs);  // C9 / 2
                     const T ic6 = rs * rs;
                     const T rcq = prsqrt(c);
                     const T ic = rcq * rcq;
@@ -544,33 +566,38 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                     const T dQa = yz + x2 * d, dQb = yz - x2 * d, dQc = x2 * c - yz;
                     const T QiR5 = Q * iR5;
                     const T A = iR3 + T(0.375) * QiR5;
-                    const T P4 = T(-6) * iR3 - T(3.75) * QiR5;  // 4 P1
-                    T x16;
+                    T f;
                     if constexpr (F32) {
-                        x16 = __builtin_amdgcn_exp2f(lamjk + klam - (8.0f / 3.0f) * __builtin_amdgcn_logf(c));
+                        // 6 x^16 = exp2(lam_jk - (8/3) log2 c) * 6 exp2(lam_k)
+                        const T e = __builtin_amdgcn_exp2f(lamjk - (8.0f / 3.0f) * __builtin_amdgcn_logf(c));
+                        f = prcp(T(1) + e * kfac);
                     } else {
                         const T y = lamjk * klam * ic;
-                        x16 = y * y * y * inv_cbrt(y);
+                        f = prcp(T(1) + T(6) * (y * y * y * inv_cbrt(y)));
                     }
-                    const T f = prcp(T(1) + T(6) * x16);
                     const T cfh = C9h * f;
                     const T hC = cfh * A;  // e / 2
-                    const T G2 = cfh * (P4 + T(32.0 / 3.0) * A * (T(1) - f));
-                    const T H2 = T(1.5) * cfh * iR5;
+                    // t' = (2/3) t_xy: the 3/2 of H2 is taken out of every t and restored where the
+                    // per-neighbour sums are read (phase 4): G2' = cfh (P4 + (32/3) A (1 - f)) / 1.5
+                    const T P4 = T(-4) * iR3 - T(2.5) * QiR5;
+                    const T G2 = cfh * (P4 + T(64.0 / 9.0) * A * (T(1) - f));
+                    const T H2 = cfh * iR5;
                     const T tij = G2 * j2.x + H2 * dQa;
                     const T tik = G2 * kib + H2 * dQb;
                     const T tjk = G2 * ic + H2 * dQc;
                     const T hc6 = hC * ic6;
-                    // neighbour j: LDS slot jl, distinct across lanes at each step.  FP32: plain
+                    // Per-neighbour sums (F_p = (3/2) (B_p - A_p d_p), phase 4) with the {j, k} edge along
+                    // v = d_k - d_j: grad_j e = -t_ij d_j + t_jk v, grad_k e = -t_ik d_k - t_jk v.
+                    // Neighbour j: LDS slot, distinct across lanes at each step.  FP32: plain
                     // read-modify-write (LDS float atomics measured ~6x slower on gfx950); FP64:
                     // LDS atomics (no per-step barrier, measured ~4% faster than the barrier form).
                     if constexpr (F32) {
                         T2<T> &r0 = at(4, jb), &r1 = at(5, jb), &r2 = at(6, jb);
                         T2<T> p0 = r0, p1 = r1, p2 = r2;
-                        p0.x += tij + tjk;
-                        p0.y += tjk * kx;
-                        p1.x += tjk * ky;
-                        p1.y += tjk * kz;
+                        p0.x += tij;
+                        p0.y += tjk * vx;
+                        p1.x += tjk * vy;
+                        p1.y += tjk * vz;
                         p2.x += hC;
                         p2.y += hc6 * dj6;
                         r0 = p0;
@@ -578,18 +605,18 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                         r2 = p2;
                     } else {
                         T2<T> &r0 = at(4, jb), &r1 = at(5, jb), &r2 = at(6, jb);
-                        atomicAdd(&r0.x, tij + tjk);
-                        atomicAdd(&r0.y, tjk * kx);
-                        atomicAdd(&r1.x, tjk * ky);
-                        atomicAdd(&r1.y, tjk * kz);
+                        atomicAdd(&r0.x, tij);
+                        atomicAdd(&r0.y, tjk * vx);
+                        atomicAdd(&r1.x, tjk * vy);
+                        atomicAdd(&r1.y, tjk * vz);
                         atomicAdd(&r2.x, hC);
                         atomicAdd(&r2.y, hc6 * dj6);
                     }
                     // neighbour k: registers
-                    kA += tik + tjk;
-                    kBx += tjk * jx;
-                    kBy += tjk * jy;
-                    kBz += tjk * jz;
+                    kA += tik;
+                    kBx -= tjk * vx;
+                    kBy -= tjk * vy;
+                    kBz -= tjk * vz;
                     kH += hC;
                     if constexpr (NR == 0) {
                         kG += hc6 * dk6;
@@ -635,7 +662,7 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                         unsafeAtomicAdd(&La[q * cap + k], (double)S.Khi[q * AW + lane] + (double)S.Klo[q * AW + lane]);
         }
 
-        // ---- phase 4.  Per neighbour slot p: F_p = grad_p E = -d_p A_p + B_p, H_p = sum of e/2,
+        // ---- phase 4.  Per neighbour slot p: F_p = grad_p E = (3/2) (B_p - d_p A_p), H_p = sum of e/2,
         // Gc_p = sum of (e/2) dlnC6_pq/dCN_p over the triangles it belongs to.  Then
         //   dE/dCN_p = Gc_p + (dlnC6_ip/dCN_p) H_p,  E/3 share of p = (2/3) H_p,
         //   centre: E = sum H_p (each triangle feeds two slots), dE/dCN_i = sum (dlnC6_ip/dCN_i) H_p,
@@ -653,8 +680,10 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
             double acc[A_NF];
             for (int q = 0; q < A_NF; ++q)
                 acc[q] = __hip_atomic_load(&La[q * cap + p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            // the pair steps sum t' = (2/3) t
             const double Ap = acc[A_A];
-            const double fx = acc[A_BX] - dx * Ap, fy = acc[A_BY] - dy * Ap, fz = acc[A_BZ] - dz * Ap;
+            const double fx = 1.5 * (acc[A_BX] - dx * Ap), fy = 1.5 * (acc[A_BY] - dy * Ap),
+                         fz = 1.5 * (acc[A_BZ] - dz * Ap);
             const double H = acc[A_H], Gc = acc[A_G];
             atomicAdd(&P.fa[3 * j + 0], fx);
             atomicAdd(&P.fa[3 * j + 1], fy);
