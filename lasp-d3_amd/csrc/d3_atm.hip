@@ -48,8 +48,8 @@ constexpr float FAR = 1e18f;                    // coordinate of padding lanes (
 // record fields (global scratch, one array of cap entries per field)
 // (r^2, 1/r^2, 1/r, sqrt C6 and the damping term are recomputed where a tile is staged: they
 // cost a few instructions per slot and tile, against twice the record bytes read back from HBM)
-enum { F_DX, F_DY, F_DZ, F_DLI, F_DLJ };
-static_assert(F_DLJ + 1 == ATM_NF, "ATM_NF record fields per neighbour slot");
+enum { F_DX, F_DY, F_DZ, F_DLI, F_DLJ, F_SC6 };  // offset, CN log-derivatives of C6_ip, sqrt C6_ip
+static_assert(F_SC6 + 1 == ATM_NF, "ATM_NF record fields per neighbour slot");
 // per-neighbour FP64 accumulators over the triangles {i, p, q} of the centre: A_p = sum t'_ip,
 // B_p = sum t'_pq (d_q - d_p), H_p = sum e/2, Gc_p = sum (e/2) dlnC6_pq/dCN_p, with t' = (2/3) t,
 // so that grad_p = (3/2) (B_p - A_p d_p)
@@ -95,42 +95,66 @@ __device__ __forceinline__ float wmax(float v) {
 template <class T>
 struct AtmLds {
     // Rows of 64 pair-sized slots (T2), all at fixed offsets before the C6 rows, so that a pair
-    // step addresses every per-j value from one register (jl * sizeof(T2)) plus immediates:
-    //   J0..J3  j tile: (dx, dy) (dz, a) (1/a, a^-1/2) (sqrt C6_ij, lam_j)
-    //   Jp      j-side tile sums as pairs (A, Bx) (By, Bz) (H, G)       [3 rows]
-    //   Jt      without DR: the j type x ne, one int per slot            [1 row]
+    // step addresses every per-j value from one register (the slot's byte offset) plus immediates:
+    //   J0..J2  j tile: (dx, dy) (dz, a) (1/a, a^-1/2)
+    //   J3      (sqrt C6_ij, lam_j): without DR, or with pairwise C6 (the steps of the DR form
+    //           with per-element references read neither)                  [0 or 1 row]
+    //   Jp      j-side tile sums as pairs (A, Bx) (By, Bz) (H, G)        [3 rows]
+    //   Jt      without DR: the j type x ne, one int per slot (and the type) [0 or 1 row]
     //   Jyd     per type B of k: with DR, one damping row (FP32: lam_j + (8/3) log2
     //           R0_{A_j B}^2, FP64: lam_j R0_{A_j B}^2; 0), then the type's reference count of
     //           rows (y, dy); then padding rows.  A pair step reads NR rows from the start of
     //           type(k)'s C6 rows: for a type with fewer references the extra rows are the next
     //           type's (finite values) or padding, met by k weights that are exactly zero.
+    // The fixed rows (J0..Jt) are a static LDS array of the kernel, so their addresses are
+    // link-time constants and a step's slot offset needs no base register; the rest is dynamic
+    // LDS: Jyd, then one region that holds the centre's C6 rows Cy, Cdy [ne * nref] in phase 2
+    // and, with KL, the k-side sums Khi, Klo [A_NF][64] in phase 3 (the pair terms read sqrt C6
+    // from the neighbour records, not from Cy), then without DR the per-type-pair damping terms
+    // Rt [ne * ne] (with DR the staging computes them from r0).  Fcc Cu, FP32, KL + DR: 6 static
+    // rows + 4 C6 rows + the sums = 8192 bytes, 20 blocks per CU.
     T2<T> *J0, *J1, *J2, *J3, *Jp, *Jyd;
     int *Jt;
-    T *Khi, *Klo;                    // [A_NF][64] k-side sums over the j tiles (two-sum pair in FP32; KL only)
-    T *Cy, *Cdy;                     // centre rows [ne * nref]
-    T *Rt;                           // per type pair: FP32 (8/3) log2 R0^2, FP64 R0^2
+    T *Khi, *Klo;  // k-side sums over the j tiles (two-sum pair in FP32; KL only)
+    T *Cy, *Cdy;   // centre rows (phase 2)
+    T *Rt;         // per type pair: FP32 (8/3) log2 R0^2, FP64 R0^2 (without DR)
     static constexpr int TI = sizeof(T2<T>) / sizeof(int);  // ints per slot
-    __host__ __device__ static constexpr int fixed_rows(bool dr) { return dr ? 7 : 8; }
-    __host__ __device__ static int jy_rows(int ne, int yrows, bool dr) { return yrows + (dr ? ne : 0); }
-    __host__ __device__ static size_t bytes(int ne, int nref, int yrows, bool kl, bool dr) {
-        const size_t ynr = (size_t)ne * nref;
-        return (fixed_rows(dr) + (size_t)jy_rows(ne, yrows, dr)) * AW * sizeof(T2<T>) +
-               (size_t)(kl ? 2 : 0) * A_NF * AW * sizeof(T) + 2 * ynr * sizeof(T) + (size_t)ne * ne * sizeof(T);
+    __host__ __device__ static constexpr bool has_j3(bool dr, bool pairwise) { return !dr || pairwise; }
+    __host__ __device__ static constexpr int jp_row(bool j3) { return j3 ? 4 : 3; }
+    __host__ __device__ static constexpr int jt_row(bool j3) { return jp_row(j3) + 3; }
+    __host__ __device__ static constexpr int fixed_rows(bool dr, bool j3) { return jt_row(j3) + (dr ? 0 : 1); }
+    __host__ __device__ static constexpr size_t fixed_bytes(bool dr, bool j3) {
+        return (size_t)fixed_rows(dr, j3) * AW * sizeof(T2<T>);
     }
-    __device__ AtmLds(unsigned char *s, int ne, int nref, int yrows, bool kl, bool dr) {
-        const size_t ynr = (size_t)ne * nref;
-        J0 = reinterpret_cast<T2<T> *>(s);
+    __host__ __device__ static int jy_rows(int ne, int yrows, bool dr) { return yrows + (dr ? ne : 0); }
+    __host__ __device__ static size_t region_bytes(int ne, int nref, bool kl) {
+        const size_t sums = kl ? (size_t)2 * A_NF * AW * sizeof(T) : 0, rows = (size_t)2 * ne * nref * sizeof(T);
+        return sums > rows ? sums : rows;
+    }
+    // dynamic LDS bytes of a launch (nref 0: pairwise C6)
+    __host__ __device__ static size_t dyn_bytes(int ne, int nref, int yrows, bool kl, bool dr) {
+        return (size_t)jy_rows(ne, yrows, dr) * AW * sizeof(T2<T>) + region_bytes(ne, nref, kl) +
+               (dr ? 0 : (size_t)ne * ne * sizeof(T));
+    }
+    // all LDS of a block (static + dynamic)
+    __host__ __device__ static size_t bytes(int ne, int nref, int yrows, bool kl, bool dr) {
+        return fixed_bytes(dr, has_j3(dr, nref == 0)) + dyn_bytes(ne, nref, yrows, kl, dr);
+    }
+    __device__ AtmLds(T2<T> *fixed, unsigned char *s, int ne, int nref, int yrows, bool kl, bool dr) {
+        const bool j3 = has_j3(dr, nref == 0);
+        J0 = fixed;
         J1 = J0 + AW;
         J2 = J0 + 2 * AW;
-        J3 = J0 + 3 * AW;
-        Jp = J0 + 4 * AW;
-        Jt = reinterpret_cast<int *>(J0 + 7 * AW);
-        Jyd = J0 + fixed_rows(dr) * AW;
-        Khi = reinterpret_cast<T *>(Jyd + (size_t)jy_rows(ne, yrows, dr) * AW);
+        J3 = j3 ? J0 + 3 * AW : nullptr;
+        Jp = J0 + jp_row(j3) * AW;
+        Jt = dr ? nullptr : reinterpret_cast<int *>(J0 + jt_row(j3) * AW);
+        Jyd = reinterpret_cast<T2<T> *>(s);
+        unsigned char *region = reinterpret_cast<unsigned char *>(Jyd + (size_t)jy_rows(ne, yrows, dr) * AW);
+        Khi = reinterpret_cast<T *>(region);
         Klo = Khi + A_NF * AW;
-        Cy = kl ? Klo + A_NF * AW : Khi;
-        Cdy = Cy + ynr;
-        Rt = Cdy + ynr;
+        Cy = reinterpret_cast<T *>(region);
+        Cdy = Cy + (size_t)ne * nref;
+        Rt = dr ? nullptr : reinterpret_cast<T *>(region + region_bytes(ne, nref, kl));
     }
 };
 
@@ -151,12 +175,15 @@ constexpr int atm_min_waves() {
 template <class T, int NR, bool KL, bool DR>
 __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> P) {
     extern __shared__ __align__(16) unsigned char smem[];
+    constexpr bool J3 = AtmLds<T>::has_j3(DR, NR == 0);  // the (sqrt C6_ij, lam_j) row is staged
+    constexpr int JP = AtmLds<T>::jp_row(J3), JT = AtmLds<T>::jt_row(J3);  // rows of Jp, Jt
+    __shared__ T2<T> fixed[AtmLds<T>::fixed_rows(DR, J3) * AW];
     constexpr bool F32 = sizeof(T) == 4;
     static_assert(NR > 0 || !DR, "the pairwise-C6 form (NR = 0) stages no C6 or damping rows");
     constexpr int DRO = DR ? 1 : 0;  // C6 rows of a type start after its damping row
     const int lane = threadIdx.x;
     const int ne = P.ne, ynr = ne * NR;
-    AtmLds<T> S(smem, ne, NR, P.atm_yrows, KL, DR);
+    AtmLds<T> S(fixed, smem, ne, NR, P.atm_yrows, KL, DR);
 
     const int cap = P.atm_cap;
     int *Lj = P.atm_i32 + (size_t)blockIdx.x * cap * 2;
@@ -166,10 +193,13 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
     double *La = P.atm_acc + (size_t)blockIdx.x * atm_acc_stride(cap);
     const T rc32 = P.rc_32;
 
-    for (int idx = lane; idx < ne * ne; idx += AW) {
+    // per-type-pair damping terms, FP32 (8/3) log2 R0^2, FP64 R0^2 (in LDS without DR)
+    auto rt_of = [&](int idx) -> T {
         const T r0sq = P.r0[idx] * P.r0[idx];
-        S.Rt[idx] = F32 ? T((8.0f / 3.0f) * log2f((float)r0sq)) : r0sq;
-    }
+        return F32 ? T((8.0f / 3.0f) * log2f((float)r0sq)) : r0sq;
+    };
+    if constexpr (!DR)
+        for (int idx = lane; idx < ne * ne; idx += AW) S.Rt[idx] = rt_of(idx);
     // lane B (< ne <= 64): first C6 row of type B, read with readlane / a lane shuffle instead of
     // dependent loads in the tile loops
     const int roff_l = lane < ne ? P.atm_roff[lane] : 0;
@@ -304,6 +334,7 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                 Lt[p] = tj;
                 LT[F_DLI * cap + p] = dci * ic6;
                 LT[F_DLJ * cap + p] = dcj * ic6;
+                LT[F_SC6 * cap + p] = xsqrt(c6);
                 // the accumulators take FP64 atomic adds in phase 3 (no returned value, so the wave
                 // never waits on them) and are read back with atomic loads (L2, not a stale L1 line)
                 for (int q = 0; q < A_NF; ++q)
@@ -336,18 +367,11 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
         // Pair (i, p) terms of a forward neighbour p from its offset, type and weights, the same
         // expressions for the k and the j side: r^2, 1/r^2, 1/r, sqrt C6_ip, damping term
         // (FP32: (8/3) log2(R0^2 / r^2), FP64: R0^2 / r^2).
-        auto pair_terms = [&](T dx, T dy, T dz, int tp, const T *wp, T cnp, T &a, T &ia, T &ra, T &sc6, T &lam) {
+        auto pair_terms = [&](T dx, T dy, T dz, int tp, int slot, T &a, T &ia, T &ra, T &sc6, T &lam) {
             a = dx * dx + dy * dy + dz * dz;
             ia = T(1) / a;
             ra = T(1) / xsqrt(a);
-            T c6 = T(0);
-            if constexpr (NR == 0) {
-                T d1, d2;
-                c6_pairwise(P.c6g, ne, P.nref, ti, tp, cni, cnp, c6, d1, d2);
-            } else {
-                for (int b = 0; b < NR; ++b) c6 += S.Cy[tp * NR + b] * wp[b];
-            }
-            sc6 = xsqrt(c6);
+            sc6 = LT[F_SC6 * cap + slot];  // from phase 2
             const T r0 = P.r0[ti * ne + tp];
             lam = F32 ? T((8.0f / 3.0f) * log2f((float)(r0 * r0 * ia))) : r0 * r0 * ia;
         };
@@ -373,7 +397,7 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                 }
                 if constexpr (NR == 0) kcn = P.cn[jk];
                 T sc6, lam;
-                pair_terms(kx, ky, kz, tk, kw, kcn, kb2, kib, krb, sc6, lam);
+                pair_terms(kx, ky, kz, tk, k, kb2, kib, krb, sc6, lam);
                 ksc6 = T(-0.5) * sc6;  // -sqrt(C6_ik) / 2: C9 / 2 below (pairwise C6 only)
                 klam = F32 ? lam + T(L2K6_83) : T(K6) * lam;
                 // The pair steps form c6'' = sum_b y'_b(j) kw'_b = (C6_ik / 4) C6_ij C6_jk from
@@ -431,12 +455,12 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                             dwj[q] = P.dw[q * P.n + jj];
                         }
                         if constexpr (NR == 0) jcn = P.cn[jj];
-                        pair_terms(jxs, jys, jzs, tj, wj, jcn, a, ia, ra, sc6, lam);
+                        pair_terms(jxs, jys, jzs, tj, pj, a, ia, ra, sc6, lam);
                     }
                     S.J0[lane] = T2<T>{jxs, jys};
                     S.J1[lane] = T2<T>{jzs, a};
                     S.J2[lane] = T2<T>{ia, ra};
-                    S.J3[lane] = T2<T>{sc6, lam};
+                    if constexpr (J3) S.J3[lane] = T2<T>{sc6, lam};
                     if constexpr (!DR) S.Jt[lane * S.TI] = tj * ne;  // row of the per-type-pair table
                     if constexpr (NR == 0) {
                         // pairwise C6: the j type and CN, one row
@@ -450,7 +474,7 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                     for (int B = 0; B < ne; ++B) {
                         const int base = __builtin_amdgcn_readlane(roff_l, B) + (DR ? B : 0);
                         if constexpr (DR) {
-                            const T rt = S.Rt[tj * ne + B];
+                            const T rt = rt_of(tj * ne + B);
                             S.Jyd[base * AW + lane] = T2<T>{F32 ? lam + rt : lam * rt, T(0)};
                         }
                         // rows times C6_ij = sc6^2 (see the k weights)
@@ -488,10 +512,10 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                 // one mask per step, and every row (the C6 rows of type(k) too) is that offset
                 // plus a base
                 constexpr int SB = (int)sizeof(T2<T>);
-                // fixed rows by index from the LDS base (J0 = row 0 ... Jt = row 7, as AtmLds lays
-                // them out), so that the offset needs no base register
+                // fixed rows by index in the static array (J0..J2 = rows 0-2, then J3, Jp and Jt
+                // as AtmLds lays them out): the row offsets are immediates of the LDS instructions
                 auto at = [&](int row, int jb) -> T2<T> & {
-                    return *reinterpret_cast<T2<T> *>(smem + row * AW * SB + jb);
+                    return *reinterpret_cast<T2<T> *>(reinterpret_cast<unsigned char *>(fixed) + row * AW * SB + jb);
                 };
                 auto cat = [](const T2<T> *row, int jb) -> const T2<T> & {
                     return *reinterpret_cast<const T2<T> *>(reinterpret_cast<const unsigned char *>(row) + jb);
@@ -528,7 +552,7 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                         if constexpr (NR == 0) sc6j = at(3, jb).x;
                     } else {
                         const T2<T> j3 = at(3, jb);
-                        const T rt = S.Rt[*reinterpret_cast<const int *>(&at(7, jb)) + tk];
+                        const T rt = S.Rt[*reinterpret_cast<const int *>(&at(JT, jb)) + tk];
                         lamjk = F32 ? j3.y + rt : j3.y * rt;
                         sc6j = j3.x;
                     }
@@ -579,8 +603,9 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                     const T hC = cfh * A;  // e / 2
                     // t' = (2/3) t_xy: the 3/2 of H2 is taken out of every t and restored where the
                     // per-neighbour sums are read (phase 4): G2' = cfh (P4 + (32/3) A (1 - f)) / 1.5
-                    const T P4 = T(-4) * iR3 - T(2.5) * QiR5;
-                    const T G2 = cfh * (P4 + T(64.0 / 9.0) * A * (T(1) - f));
+                    // = cfh (iR3 (28/9 - (64/9) f) + QiR5 (1/6 - (8/3) f)), P4 = 4 P1
+                    const T G2 = cfh * (iR3 * (T(28.0 / 9.0) - T(64.0 / 9.0) * f) +
+                                        QiR5 * (T(1.0 / 6.0) - T(8.0 / 3.0) * f));
                     const T H2 = cfh * iR5;
                     const T tij = G2 * j2.x + H2 * dQa;
                     const T tik = G2 * kib + H2 * dQb;
@@ -592,7 +617,7 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                     // read-modify-write (LDS float atomics measured ~6x slower on gfx950); FP64:
                     // LDS atomics (no per-step barrier, measured ~4% faster than the barrier form).
                     if constexpr (F32) {
-                        T2<T> &r0 = at(4, jb), &r1 = at(5, jb), &r2 = at(6, jb);
+                        T2<T> &r0 = at(JP, jb), &r1 = at(JP + 1, jb), &r2 = at(JP + 2, jb);
                         T2<T> p0 = r0, p1 = r1, p2 = r2;
                         p0.x += tij;
                         p0.y += tjk * vx;
@@ -604,7 +629,7 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                         r1 = p1;
                         r2 = p2;
                     } else {
-                        T2<T> &r0 = at(4, jb), &r1 = at(5, jb), &r2 = at(6, jb);
+                        T2<T> &r0 = at(JP, jb), &r1 = at(JP + 1, jb), &r2 = at(JP + 2, jb);
                         atomicAdd(&r0.x, tij);
                         atomicAdd(&r0.y, tjk * vx);
                         atomicAdd(&r1.x, tjk * vy);
@@ -831,8 +856,15 @@ const void *atm_kernel(int nref, int var) {
     }
 }
 
+// dynamic LDS of a launch (the fixed rows are the kernel's static LDS)
+size_t atm_dyn_lds_bytes(int ne, int nref, int yrows, size_t ts, int var) {
+    const bool kl = var & ATM_KL, dr = var & ATM_DR;
+    return ts == sizeof(double) ? AtmLds<double>::dyn_bytes(ne, nref, yrows, kl, dr)
+                                : AtmLds<float>::dyn_bytes(ne, nref, yrows, kl, dr);
+}
+
 int blocks_per_cu(int ne, int nref, int yrows, size_t ts, int var) {
-    const size_t lds = atm_lds_bytes(ne, nref, yrows, ts, var);
+    const size_t lds = atm_dyn_lds_bytes(ne, nref, yrows, ts, var);
     const void *fn = ts == sizeof(double) ? atm_kernel<double>(nref, var) : atm_kernel<float>(nref, var);
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, AW, lds) != hipSuccess) per_cu = 0;
@@ -884,7 +916,7 @@ void launch_atm(const Params<T> &P, int nblocks, hipStream_t s) {
     if (nz) hipLaunchKernelGGL(k_fill_d, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, s, P.fa, nz, P.atm_next);
     if (P.n == 0 || !(P.rc_32 > T(0))) return;
     const int knr = P.c6_pairwise ? 0 : P.nref;
-    const size_t lds = atm_lds_bytes(P.ne, knr, P.atm_yrows, sizeof(T), P.atm_var);
+    const size_t lds = atm_dyn_lds_bytes(P.ne, knr, P.atm_yrows, sizeof(T), P.atm_var);
     const void *fn = atm_kernel<T>(knr, P.atm_var);
     if (!fn) return;  // not reached: 1..MAX_NREF references, or pairwise C6 without DR
     Params<T> arg = P;

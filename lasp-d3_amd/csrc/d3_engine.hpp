@@ -102,7 +102,7 @@ struct Params {
     const T *c6g;
     int c6_pairwise;
     // ATM: per-block scratch for one centre's forward neighbours (capacity atm_cap):
-    //   atm_i32: j, type (2 x cap); atm_T: record fields (ATM_NF x cap: offset, CN log-derivatives)
+    //   atm_i32: j, type (2 x cap); atm_T: record fields (ATM_NF x cap: offset, CN log-derivatives, sqrt C6)
     //   and per tile of 64 slots its bounding box (8 values: min, max per axis, 2 unused);
     //   atm_acc: per-neighbour accumulators (6 x cap, double)
     // and per-atom results (sorted order): fa[3N], ga[N], eas[N] filled with atomics (every
@@ -158,7 +158,7 @@ struct Params {
     unsigned *status;
 };
 
-constexpr int ATM_NF = 5;  // record fields per ATM neighbour (offset, CN log-derivatives)
+constexpr int ATM_NF = 6;  // record fields per ATM neighbour (offset, CN log-derivatives, sqrt C6)
 constexpr int TB_PART_W = 12;  // doubles per two-body part sum: E, dE/dCN, F[3], virial[6], pairs
 // doubles of ATM accumulator scratch per block: 6 per neighbour slot
 __host__ __device__ constexpr size_t atm_acc_stride(int cap) { return 6 * (size_t)cap; }
