@@ -1,12 +1,18 @@
 // Axilrod-Teller-Muto three-body term, replacing the reference's atm_kernel
 // (src/d3_kernel.cu:1199-1509).  Every periodic triangle is evaluated exactly once.
 //
-// Enumeration.  The anchor of a triangle is its vertex with the smallest atom index (sorted
-// order); among vertices of the same atom, the one whose other copies lie at lattice offsets
-// t >_lex 0.  A centre i keeps only its "forward" neighbours within rc3 (atom index > i, or the
-// atom i itself at an image t >_lex 0), so each triangle appears once, as a pair {j, k} of
-// forward neighbours of its anchor with r_jk <= rc3.  (The reference enumerates i >= j >= k over
-// every image and rescales index-degenerate triplets by 1/2 or 1/6.)
+// Enumeration.  Every vertex of a periodic triangle lies in a bin of the unbounded (unwrapped)
+// bin grid; the anchor of a triangle is its vertex whose unwrapped bin comes first in
+// lexicographic (z, y, x) order, and among vertices in the same unwrapped bin (distinct atoms,
+// since one atom occupies one point per bin image) the one with the smallest atom index.  A centre
+// i therefore keeps only its "forward" neighbours within rc3: atoms in the bins at stencil
+// offsets d >_lex 0 (any atom, i itself at another image included), and atoms j > i of its own
+// bin (d = 0); each triangle appears once, as a pair {j, k} of forward neighbours of its anchor
+// with r_jk <= rc3.  The test is on integers (offsets, indices), so it is exact, and every centre
+// keeps the upper half of its stencil (an anchor rule by atom index alone would give the centres
+// in the first bin layers of the cell, whose lower neighbours wrap to the top layers, nearly
+// whole spheres).  (The reference enumerates i >= j >= k over every image and rescales
+// index-degenerate triplets by 1/2 or 1/6.)
 //
 // Work layout.  One wave per centre; persistent one-wave blocks take centres from an atomic
 // counter.  Phase 1 compacts the forward neighbours (ballot/mbcnt) into per-block scratch;
@@ -131,16 +137,17 @@ struct AtmLds {
         const size_t sums = kl ? (size_t)2 * A_NF * AW * sizeof(T) : 0, rows = (size_t)2 * ne * nref * sizeof(T);
         return sums > rows ? sums : rows;
     }
-    // dynamic LDS bytes of a launch (nref 0: pairwise C6)
-    __host__ __device__ static size_t dyn_bytes(int ne, int nref, int yrows, bool kl, bool dr) {
-        return (size_t)jy_rows(ne, yrows, dr) * AW * sizeof(T2<T>) + region_bytes(ne, nref, kl) +
+    // dynamic LDS bytes of a launch (nref 0: pairwise C6).  one: a single element type, whose
+    // rows Jyd are part of the kernel's static LDS (their offsets then are immediates too)
+    __host__ __device__ static size_t dyn_bytes(int ne, int nref, int yrows, bool kl, bool dr, bool one = false) {
+        return (one ? 0 : (size_t)jy_rows(ne, yrows, dr) * AW * sizeof(T2<T>)) + region_bytes(ne, nref, kl) +
                (dr ? 0 : (size_t)ne * ne * sizeof(T));
     }
     // all LDS of a block (static + dynamic)
     __host__ __device__ static size_t bytes(int ne, int nref, int yrows, bool kl, bool dr) {
         return fixed_bytes(dr, has_j3(dr, nref == 0)) + dyn_bytes(ne, nref, yrows, kl, dr);
     }
-    __device__ AtmLds(T2<T> *fixed, unsigned char *s, int ne, int nref, int yrows, bool kl, bool dr) {
+    __device__ AtmLds(T2<T> *fixed, unsigned char *s, int ne, int nref, int yrows, bool kl, bool dr, bool one) {
         const bool j3 = has_j3(dr, nref == 0);
         J0 = fixed;
         J1 = J0 + AW;
@@ -148,8 +155,8 @@ struct AtmLds {
         J3 = j3 ? J0 + 3 * AW : nullptr;
         Jp = J0 + jp_row(j3) * AW;
         Jt = dr ? nullptr : reinterpret_cast<int *>(J0 + jt_row(j3) * AW);
-        Jyd = reinterpret_cast<T2<T> *>(s);
-        unsigned char *region = reinterpret_cast<unsigned char *>(Jyd + (size_t)jy_rows(ne, yrows, dr) * AW);
+        Jyd = one ? J0 + fixed_rows(dr, j3) * AW : reinterpret_cast<T2<T> *>(s);
+        unsigned char *region = one ? s : reinterpret_cast<unsigned char *>(Jyd + (size_t)jy_rows(ne, yrows, dr) * AW);
         Khi = reinterpret_cast<T *>(region);
         Klo = Khi + A_NF * AW;
         Cy = reinterpret_cast<T *>(region);
@@ -157,6 +164,11 @@ struct AtmLds {
         Rt = dr ? nullptr : reinterpret_cast<T *>(region + region_bytes(ne, nref, kl));
     }
 };
+
+// First forward stencil entry of a cell's three-body stencil: its middle entry, the offset d = 0
+// (the stencil is symmetric under d -> -d and ordered lexicographically by (dz, dy, dx); the host
+// checks that the middle entry is d = 0).
+__device__ __forceinline__ int fwd_first(const SysDev *S) { return S->st_off[CUT_ATM] + (S->st_cnt[CUT_ATM] - 1) / 2; }
 
 // Waves per SIMD the compiler must allow: FP32 with 3 references fits 5 waves (96 VGPRs) without
 // spilling and runs 2.6 % faster on the fcc Cu cell; under that cap 2 references spill (1 %
@@ -171,19 +183,22 @@ constexpr int atm_min_waves() {
 // accumulators once per k tile; otherwise they are added to them after every tile.  DR: the
 // per-type-pair damping term is staged as an extra row per type of k (two dependent LDS reads and
 // three VALU fewer per pair, ne rows more LDS).  The host picks them from the resident block
-// counts (atm_choose_variant); the autotuner times all four.
-template <class T, int NR, bool KL, bool DR>
+// counts (atm_choose_variant); the autotuner times all four.  ONE: a handle of one element type,
+// whose rows of type(k) start at a fixed LDS offset (an immediate of the step's LDS reads instead
+// of a per-lane row pointer added at every step).
+template <class T, int NR, bool KL, bool DR, bool ONE>
 __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> P) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr bool J3 = AtmLds<T>::has_j3(DR, NR == 0);  // the (sqrt C6_ij, lam_j) row is staged
     constexpr int JP = AtmLds<T>::jp_row(J3), JT = AtmLds<T>::jt_row(J3);  // rows of Jp, Jt
-    __shared__ T2<T> fixed[AtmLds<T>::fixed_rows(DR, J3) * AW];
+    // the fixed rows, and with a single element type its (damping,) C6 rows
+    __shared__ T2<T> fixed[(AtmLds<T>::fixed_rows(DR, J3) + (ONE ? NR + (DR ? 1 : 0) : 0)) * AW];
     constexpr bool F32 = sizeof(T) == 4;
     static_assert(NR > 0 || !DR, "the pairwise-C6 form (NR = 0) stages no C6 or damping rows");
     constexpr int DRO = DR ? 1 : 0;  // C6 rows of a type start after its damping row
     const int lane = threadIdx.x;
     const int ne = P.ne, ynr = ne * NR;
-    AtmLds<T> S(fixed, smem, ne, NR, P.atm_yrows, KL, DR);
+    AtmLds<T> S(fixed, smem, ne, NR, P.atm_yrows, KL, DR, ONE);
 
     const int cap = P.atm_cap;
     int *Lj = P.atm_i32 + (size_t)blockIdx.x * cap * 2;
@@ -269,7 +284,10 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                 take(j, dx, dy, dz, in);
             }
         } else {
-            const int s0 = C.S->st_off[CUT_ATM], s1 = s0 + C.S->st_cnt[CUT_ATM];
+            // forward neighbours: the stencil entries after the centre's own bin (offset d = 0, the
+            // middle entry of the symmetric stencil, whose entries run in lexicographic (dz, dy,
+            // dx) order), and the atoms of the own bin after i
+            const int s0 = fwd_first(C.S), s1 = C.S->st_off[CUT_ATM] + C.S->st_cnt[CUT_ATM];
             const T rc3 = xsqrt(rc32);
             for (int sb = s0; sb < s1; sb += AW) {
                 const auto Bn = C.bins(P, sb, s1, rc3);
@@ -278,14 +296,12 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                     int j0, j1;
                     T ox, oy, oz;
                     Bn.get(sk, j0, j1, ox, oy, oz);
-                    if (j1 <= i) continue;  // bins are sorted: every atom here has index < i
-                    const StEntry d = P.st_d[sb + sk];
-                    const bool dlex = d.dx > 0 || (d.dx == 0 && (d.dy > 0 || (d.dy == 0 && d.dz > 0)));
+                    if (sb + sk == s0) j0 = i + 1;  // own bin, no image shift
                     for (int jb = j0; jb < j1; jb += AW) {
                         const int j = jb + lane;
                         bool in = false;
                         T dx = T(0), dy = T(0), dz = T(0);
-                        if (j < j1 && (j > i || (j == i && dlex))) {
+                        if (j < j1) {
                             dx = ox - P.lx[j];
                             dy = oy - P.ly[j];
                             dz = oz - P.lz[j];
@@ -363,7 +379,8 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
         // accumulators La, which lane l alone touches for its slots, as atomic adds: one lane's
         // adds to one slot reach L2 in issue order, so the sums stay bitwise repeatable, and
         // they cost no read latency (a read-modify-write here took 7.6 % of the kernel on Cu).
-        uint64_t cnt = 0;
+        unsigned cnt = 0;  // pairs within rc3 (at most 2^31 per centre)
+        const unsigned rc32b = __builtin_bit_cast(unsigned, (float)rc32);
         // Pair (i, p) terms of a forward neighbour p from its offset, type and weights, the same
         // expressions for the k and the j side: r^2, 1/r^2, 1/r, sqrt C6_ip, damping term
         // (FP32: (8/3) log2(R0^2 / r^2), FP64: R0^2 / r^2).
@@ -417,7 +434,7 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
             const float kminy = uni((float)LB[8 * kb + 2]), kmaxy = uni((float)LB[8 * kb + 3]);
             const float kminz = uni((float)LB[8 * kb + 4]), kmaxz = uni((float)LB[8 * kb + 5]);
             // rows of type(k) in the j tile: its damping row (DR), then its C6 rows
-            const T2<T> *Yk = S.Jyd + (__shfl(roff_l, tk, AW) + (DR ? tk : 0)) * AW;
+            const T2<T> *Yk = ONE ? S.Jyd : S.Jyd + (__shfl(roff_l, tk, AW) + (DR ? tk : 0)) * AW;
             // k-side sums over all j tiles: lane-private LDS columns, folded to FP64 once per kb
             if constexpr (KL)
                 for (int q = 0; q < A_NF; ++q) S.Khi[q * AW + lane] = S.Klo[q * AW + lane] = T(0);
@@ -520,20 +537,41 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                 auto cat = [](const T2<T> *row, int jb) -> const T2<T> & {
                     return *reinterpret_cast<const T2<T> *>(reinterpret_cast<const unsigned char *>(row) + jb);
                 };
+                // the lane's own slot offset, opaque to the optimiser: (l SB) xor (q SB) would
+                // otherwise be rewritten as (l xor q) SB, two instructions per step instead of one
+                int lsb = lane * SB;
+                asm volatile("" : "+v"(lsb));
 #pragma unroll 1
                 for (int q = Q0; q < Q1; ++q) {
-                    // FP32: lane l updates the j slot that lane l+1 updated one step earlier
+                    // FP32: lane l updates a j slot that another lane updated one step earlier
                     if constexpr (F32) wave_sync();
-                    const int jb = (lane * SB + q * SB) & (AW * SB - 1);
+                    // the diagonal tile pairs lane l with slot (l + q) mod 64 (each unordered pair
+                    // comes up twice, at q and 64 - q); the others with slot l xor q, one
+                    // instruction per step (also a permutation of the slots at every step, and
+                    // every (j, k) pair of the tile pair comes up once over the 64 steps)
+                    const int jb = DIAG ? ((lane * SB + q * SB) & (AW * SB - 1)) : (lsb ^ (q * SB));
+                    __builtin_assume(jb >= 0 && jb < AW * SB);
                     const T2<T> j0 = at(0, jb), j1 = at(1, jb);
                     const T jx = j0.x, jy = j0.y, jz = j1.x;
                     const T vx = kx - jx, vy = ky - jy, vz = kz - jz;  // r_j - r_k
-                    const T c = vx * vx + vy * vy + vz * vz;
-                    const bool okc = c <= rc32, okz = c >= T(1e-12);
-                    bool ok = okc && okz;
-                    // one ballot per compare: the ballot of the combined flag would be re-derived
-                    // from a VGPR copy of it
-                    uint64_t m = __builtin_amdgcn_ballot_w64(okc) & __builtin_amdgcn_ballot_w64(okz);
+                    T c;
+                    bool ok;
+                    uint64_t m;
+                    if constexpr (F32) {
+                        // 1e-12 <= r_jk^2 <= rc3^2 as one unsigned compare of the bits of
+                        // c = r_jk^2 - 1e-12 (a negative c has its sign bit set); the 1e-12 is far
+                        // below the rounding error of any r_jk^2 past the floor, so c stands for r_jk^2
+                        c = fmaf(vz, vz, fmaf(vy, vy, fmaf(vx, vx, -1e-12f)));
+                        ok = __builtin_bit_cast(unsigned, (float)c) <= rc32b;
+                        m = __builtin_amdgcn_ballot_w64(ok);
+                    } else {
+                        c = vx * vx + vy * vy + vz * vz;
+                        const bool okc = c <= rc32, okz = c >= T(1e-12);
+                        ok = okc && okz;
+                        // one ballot per compare: the ballot of the combined flag would be
+                        // re-derived from a VGPR copy of it
+                        m = __builtin_amdgcn_ballot_w64(okc) & __builtin_amdgcn_ballot_w64(okz);
+                    }
                     if constexpr (DIAG) {
                         if (q == AW / 2) {  // lanes l and l + 32 share the pair
                             m &= 0xffffffffull;
@@ -575,21 +613,18 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                         }
                     }
                     const T rs = prsqrt(c6);
-                    const T C9h = NR == 0 ? (sc6j * ksc6) * (c6 * rs) : -(c6 * rs);  // C9 / 2
-                    const T ic6 = rs * rs;
                     const T rcq = prsqrt(c);
                     const T ic = rcq * rcq;
-                    const T iR = j2.y * krb * rcq;
-                    const T iR2 = iR * iR, iR3 = iR2 * iR, iR5 = iR3 * iR2;
-                    // Q = x y z, x = a + b - c, y = c - d, z = c + d with d = b - a:
+                    const T iR = (j2.y * krb) * rcq;  // (abc)^(-1/2)
+                    const T iR2 = iR * iR, iR3 = iR2 * iR;
+                    // Q = x y z, x = a + b - c, y = c - d, z = c + d with d = b - a, so yz = c^2 - d^2:
                     // dQ/da = yz + 2 x d, dQ/db = yz - 2 x d, dQ/dc = 2 x c - yz
                     const T d = b - a, x = (a + b) - c;
-                    const T yz = (c - d) * (c + d);
+                    const T yz = c * c - d * d;
                     const T Q = x * yz;
+                    const T z = Q * iR2;  // Q / (abc): A = iR3 (1 + (3/8) z)
                     const T x2 = x + x;
                     const T dQa = yz + x2 * d, dQb = yz - x2 * d, dQc = x2 * c - yz;
-                    const T QiR5 = Q * iR5;
-                    const T A = iR3 + T(0.375) * QiR5;
                     T f;
                     if constexpr (F32) {
                         // 6 x^16 = exp2(lam_jk - (8/3) log2 c) * 6 exp2(lam_k)
@@ -599,18 +634,21 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                         const T y = lamjk * klam * ic;
                         f = prcp(T(1) + T(6) * (y * y * y * inv_cbrt(y)));
                     }
-                    const T cfh = C9h * f;
-                    const T hC = cfh * A;  // e / 2
+                    // rf3 = (C9 / 2) f iR3 / c6 (per-element references: C9 / 2 = -sqrt(c6) =
+                    // -c6 rs); every term below is a multiple of it
+                    const T rf3 = (NR == 0 ? (sc6j * ksc6) * rs : -rs) * f * iR3;
+                    const T Ap = T(1) + T(0.375) * z;  // A / iR3
+                    const T cf3 = c6 * rf3;            // (C9 / 2) f iR3
+                    const T hC = cf3 * Ap;             // e / 2
+                    const T hc6 = rf3 * Ap;            // e / (2 c6)
                     // t' = (2/3) t_xy: the 3/2 of H2 is taken out of every t and restored where the
                     // per-neighbour sums are read (phase 4): G2' = cfh (P4 + (32/3) A (1 - f)) / 1.5
-                    // = cfh (iR3 (28/9 - (64/9) f) + QiR5 (1/6 - (8/3) f)), P4 = 4 P1
-                    const T G2 = cfh * (iR3 * (T(28.0 / 9.0) - T(64.0 / 9.0) * f) +
-                                        QiR5 * (T(1.0 / 6.0) - T(8.0 / 3.0) * f));
-                    const T H2 = cfh * iR5;
+                    // = cf3 ((28/9 - (64/9) f) + z (1/6 - (8/3) f)), P4 = 4 P1, cfh = (C9 / 2) f
+                    const T G2 = cf3 * ((T(28.0 / 9.0) - T(64.0 / 9.0) * f) + z * (T(1.0 / 6.0) - T(8.0 / 3.0) * f));
+                    const T H2 = cf3 * iR2;  // cfh iR5
                     const T tij = G2 * j2.x + H2 * dQa;
                     const T tik = G2 * kib + H2 * dQb;
                     const T tjk = G2 * ic + H2 * dQc;
-                    const T hc6 = hC * ic6;
                     // Per-neighbour sums (F_p = (3/2) (B_p - A_p d_p), phase 4) with the {j, k} edge along
                     // v = d_k - d_j: grad_j e = -t_ij d_j + t_jk v, grad_k e = -t_ik d_k - t_jk v.
                     // Neighbour j: LDS slot, distinct across lanes at each step.  FP32: plain
@@ -770,7 +808,7 @@ __global__ __launch_bounds__(256) void k_atm_fwd(Params<T> P) {
     FwdEntry *FL = P.atm_fl + (size_t)i * cap;
     const T rcf = P.rc_fl2;
     int count = 0;
-    const int s0 = C.S->st_off[CUT_ATM], s1 = s0 + C.S->st_cnt[CUT_ATM];
+    const int s0 = fwd_first(C.S), s1 = C.S->st_off[CUT_ATM] + C.S->st_cnt[CUT_ATM];
     const T rcfl = xsqrt(rcf);
     for (int sb = s0; sb < s1; sb += AW) {
         const auto Bn = C.bins(P, sb, s1, rcfl);
@@ -779,13 +817,11 @@ __global__ __launch_bounds__(256) void k_atm_fwd(Params<T> P) {
             int j0, j1;
             T ox, oy, oz;
             Bn.get(sk, j0, j1, ox, oy, oz);
-            if (j1 <= i) continue;
-            const StEntry d = P.st_d[sb + sk];
-            const bool dlex = d.dx > 0 || (d.dx == 0 && (d.dy > 0 || (d.dy == 0 && d.dz > 0)));
+            if (sb + sk == s0) j0 = i + 1;  // own bin, no image shift
             for (int jb = j0; jb < j1; jb += AW) {
                 const int j = jb + lane;
                 bool in = false;
-                if (j < j1 && (j > i || (j == i && dlex))) {
+                if (j < j1) {
                     const T dx = ox - P.lx[j], dy = oy - P.ly[j], dz = oz - P.lz[j];
                     in = dx * dx + dy * dy + dz * dz <= rcf;
                 }
@@ -832,40 +868,47 @@ __global__ void k_merge(Params<T> P) {
 }
 
 // One instantiation per reference count (1..MAX_NREF): the C6 dot products and the LDS rows
-// are sized by the references the handle's elements actually have.  var: bit 0 = KL, bit 1 = DR.
-// nref 0: the pairwise-C6 form for tables that do not factorise (without DR).
-template <class T, bool KL, bool DR, int... NR>
+// are sized by the references the handle's elements actually have.  var: bit 0 = KL, bit 1 = DR;
+// one: the handle has a single element type.  nref 0: the pairwise-C6 form for tables that do
+// not factorise (without DR).
+template <class T, bool KL, bool DR, bool ONE, int... NR>
 const void *atm_pick(int nref, std::integer_sequence<int, NR...>) {
     const void *fn = nullptr;
-    ((fn = (nref == NR + 1) ? reinterpret_cast<const void *>(&k_atm<T, NR + 1, KL, DR>) : fn), ...);
+    ((fn = (nref == NR + 1) ? reinterpret_cast<const void *>(&k_atm<T, NR + 1, KL, DR, ONE>) : fn), ...);
     return fn;
 }
 
-template <class T>
-const void *atm_kernel(int nref, int var) {
+template <class T, bool ONE>
+const void *atm_kernel_of(int nref, int var) {
     using S = std::make_integer_sequence<int, MAX_NREF>;
+    switch (var & 3) {
+        case 0: return atm_pick<T, false, false, ONE>(nref, S{});
+        case 1: return atm_pick<T, true, false, ONE>(nref, S{});
+        case 2: return atm_pick<T, false, true, ONE>(nref, S{});
+        default: return atm_pick<T, true, true, ONE>(nref, S{});
+    }
+}
+
+template <class T>
+const void *atm_kernel(int ne, int nref, int var) {
     if (nref == 0)
         return (var & ATM_DR) ? nullptr
-                              : (var & ATM_KL) ? reinterpret_cast<const void *>(&k_atm<T, 0, true, false>)
-                                               : reinterpret_cast<const void *>(&k_atm<T, 0, false, false>);
-    switch (var & 3) {
-        case 0: return atm_pick<T, false, false>(nref, S{});
-        case 1: return atm_pick<T, true, false>(nref, S{});
-        case 2: return atm_pick<T, false, true>(nref, S{});
-        default: return atm_pick<T, true, true>(nref, S{});
-    }
+                              : (var & ATM_KL) ? reinterpret_cast<const void *>(&k_atm<T, 0, true, false, false>)
+                                               : reinterpret_cast<const void *>(&k_atm<T, 0, false, false, false>);
+    return ne == 1 ? atm_kernel_of<T, true>(nref, var) : atm_kernel_of<T, false>(nref, var);
 }
 
 // dynamic LDS of a launch (the fixed rows are the kernel's static LDS)
 size_t atm_dyn_lds_bytes(int ne, int nref, int yrows, size_t ts, int var) {
     const bool kl = var & ATM_KL, dr = var & ATM_DR;
-    return ts == sizeof(double) ? AtmLds<double>::dyn_bytes(ne, nref, yrows, kl, dr)
-                                : AtmLds<float>::dyn_bytes(ne, nref, yrows, kl, dr);
+    const bool one = ne == 1 && nref > 0;  // atm_kernel's choice
+    return ts == sizeof(double) ? AtmLds<double>::dyn_bytes(ne, nref, yrows, kl, dr, one)
+                                : AtmLds<float>::dyn_bytes(ne, nref, yrows, kl, dr, one);
 }
 
 int blocks_per_cu(int ne, int nref, int yrows, size_t ts, int var) {
     const size_t lds = atm_dyn_lds_bytes(ne, nref, yrows, ts, var);
-    const void *fn = ts == sizeof(double) ? atm_kernel<double>(nref, var) : atm_kernel<float>(nref, var);
+    const void *fn = ts == sizeof(double) ? atm_kernel<double>(ne, nref, var) : atm_kernel<float>(ne, nref, var);
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, AW, lds) != hipSuccess) per_cu = 0;
     return per_cu;
@@ -917,7 +960,7 @@ void launch_atm(const Params<T> &P, int nblocks, hipStream_t s) {
     if (P.n == 0 || !(P.rc_32 > T(0))) return;
     const int knr = P.c6_pairwise ? 0 : P.nref;
     const size_t lds = atm_dyn_lds_bytes(P.ne, knr, P.atm_yrows, sizeof(T), P.atm_var);
-    const void *fn = atm_kernel<T>(knr, P.atm_var);
+    const void *fn = atm_kernel<T>(P.ne, knr, P.atm_var);
     if (!fn) return;  // not reached: 1..MAX_NREF references, or pairwise C6 without DR
     Params<T> arg = P;
     void *args[] = {&arg};

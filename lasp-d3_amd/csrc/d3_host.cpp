@@ -543,6 +543,11 @@ void Engine::build_geometry() {
             d.st_off[cut] = (int)st_d_.size();
             if (rcs[cut] > 0) make_stencil(C.H, d.nb, rcs[cut] + skin_, st_d_, st_D_);
             d.st_cnt[cut] = (int)st_d_.size() - d.st_off[cut];
+            if (cut == CUT_ATM && d.st_cnt[cut] > 0) {
+                // the three-body kernel takes the entries after the middle one, d = 0, as forward
+                const StEntry &m = st_d_[d.st_off[cut] + (d.st_cnt[cut] - 1) / 2];
+                if (d.st_cnt[cut] % 2 == 0 || m.dx || m.dy || m.dz) fail(0, "three-body stencil not centred on d = 0");
+            }
         }
         for (int a = 0; a < (int)C.n; a += CHUNK) {
             chunk_b_.push_back(atom_base + a);

@@ -92,20 +92,23 @@ def test_length_above_max_length_is_rejected(table):
     _assert_parity(out, d3_oracle.compute(pos, Z, cell, table, **KW))
 
 
-def hub_and_shell(n_shell=1100, radius=20.0, seed=2):
-    """A hub atom in the first bin of a 200 A cell and n_shell atoms on a shell of the given radius
-    (A) around it: every other atom has a larger sorted index, so all n_shell are forward
-    three-body neighbours of the hub, while a shell atom sees about a quarter of the shell."""
-    # Fibonacci sphere: near-uniform spacing (about 2.1 A here), then a random rotation
-    k = np.arange(n_shell) + 0.5
-    zc = 1.0 - 2.0 * k / n_shell
-    ph = np.pi * (1.0 + 5.0 ** 0.5) * k
-    rr = np.sqrt(1.0 - zc * zc)
-    v = np.stack([rr * np.cos(ph), rr * np.sin(ph), zc], axis=1)
-    q, _ = np.linalg.qr(np.random.Generator(np.random.PCG64(seed)).normal(size=(3, 3)))
-    v = v @ q.T
+def hub_and_cap(radii=(12.0, 16.0, 20.0), zmin=8.0, per_a2=0.459, seed=2):
+    """A hub atom in a 200 A cell and about 1200 atoms on the parts of three spheres around it (radii
+    in A) that lie at least zmin A above it: zmin exceeds the bin width (15 bohr, 7.9 A), so every
+    such atom sits in a bin layer above the hub's and is a forward three-body neighbour of it,
+    while a cap atom has far fewer forward neighbours."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    pts = []
+    for R in radii:
+        n = int(per_a2 * 4.0 * np.pi * R * R)
+        k = np.arange(n) + 0.5
+        zc = 1.0 - 2.0 * k / n
+        ph = np.pi * (1.0 + 5.0 ** 0.5) * k + rng.uniform(0.0, 2.0 * np.pi)  # turned about z
+        rr = np.sqrt(1.0 - zc * zc)
+        v = R * np.stack([rr * np.cos(ph), rr * np.sin(ph), zc], axis=1)
+        pts.append(v[v[:, 2] >= zmin])
     hub = np.array([0.3, 0.3, 0.3])
-    pos = np.vstack([hub, hub + radius * v])
+    pos = np.vstack([hub, hub + np.vstack(pts)])
     Z = np.full(len(pos), 6, np.uint16)
     return pos, Z, np.eye(3) * 200.0
 
@@ -113,12 +116,13 @@ def hub_and_shell(n_shell=1100, radius=20.0, seed=2):
 def test_three_body_capacity_overflow_is_retried(table):
     print("case three-body capacity overflow", flush=True)
     kw = dict(KW, atm_cutoff=40.0)
-    pos, Z, cell = hub_and_shell()
+    pos, Z, cell = hub_and_cap()
+    nfwd = len(Z) - 1
     # the first capacity estimate (1.5 x the mean forward count at this density, + 1024, rounded
-    # up to 64) is below the hub's 1100 forward neighbours, so the first pass overflows
+    # up to 64) is below the hub's forward neighbours, so the first pass overflows
     dens = len(Z) / (200.0 / 0.52917721067) ** 3
     first = int(np.ceil((1.5 * 0.5 * dens * 4.18879020478639 * 40.0 ** 3 + 1024) / 64) * 64)
-    assert first < 1100
+    assert first < nfwd
     o = d3_oracle.compute(pos, Z, cell, table, **kw)
     h = d3_lib.Handle(Z, len(Z), **kw)
     h.set_atoms(pos, Z)
@@ -126,7 +130,7 @@ def test_three_body_capacity_overflow_is_retried(table):
     out = h.compute()
     cap = h.atm_capacity()
     h.close()
-    assert cap > 1100
+    assert cap >= nfwd
     _assert_parity(out, o)
 
 
