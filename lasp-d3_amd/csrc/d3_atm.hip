@@ -18,9 +18,11 @@
 // counter.  Phase 1 compacts the forward neighbours (ballot/mbcnt) into per-block scratch;
 // phase 2 turns them into records (C6_ij and its logarithmic CN derivatives, 1/a, a^-1/2,
 // sqrt C6_ij, damping term); phase 3 visits the pairs in 64 x 64 tiles: lane l owns neighbour
-// k = 64 kb + l in registers, the j tile sits in LDS with its C6 rows y_j[B] = C6ref_{A_j B}^T w_j
-// and dy_j[B] = C6ref^T dw_j, and at step q lane l takes j = (l + q) mod 64.  Tile pairs whose
-// bounding boxes are farther apart than rc3 are skipped.  Phase 4 turns the per-neighbour sums
+// k = 64 kb + l in registers (FP32: two k tiles, kb and kb + 1), the j tile sits in LDS with its
+// C6 rows y_j[B] = C6ref_{A_j B}^T w_j and dy_j[B] = C6ref^T dw_j, and at step q lane l takes
+// j = l xor q (on a tile paired with itself, j = (l + q) mod 64); with two k tiles a step reads
+// the j slot and writes its sums back once for two pairs.  Tile pairs whose bounding boxes are
+// farther apart than rc3 are skipped.  Phase 4 turns the per-neighbour sums
 // into forces, dE/dCN, energies and the virial and adds them to the per-atom arrays with FP64
 // atomics.
 //
@@ -117,8 +119,9 @@ struct AtmLds {
     // LDS: Jyd, then one region that holds the centre's C6 rows Cy, Cdy [ne * nref] in phase 2
     // and, with KL, the k-side sums Khi, Klo [A_NF][64] in phase 3 (the pair terms read sqrt C6
     // from the neighbour records, not from Cy), then without DR the per-type-pair damping terms
-    // Rt [ne * ne] (with DR the staging computes them from r0).  Fcc Cu, FP32, KL + DR: 6 static
-    // rows + 4 C6 rows + the sums = 8192 bytes, 20 blocks per CU.
+    // Rt [ne * ne] (with DR the staging computes them from r0).  Fcc Cu, FP32, KL + DR: 6 fixed
+    // and 4 C6 rows (static, one element type) + the sums of two k tiles = 11,264 bytes; DR alone
+    // 5,120 bytes.
     T2<T> *J0, *J1, *J2, *J3, *Jp, *Jyd;
     int *Jt;
     T *Khi, *Klo;  // k-side sums over the j tiles (two-sum pair in FP32; KL only)
@@ -133,8 +136,12 @@ struct AtmLds {
         return (size_t)fixed_rows(dr, j3) * AW * sizeof(T2<T>);
     }
     __host__ __device__ static int jy_rows(int ne, int yrows, bool dr) { return yrows + (dr ? ne : 0); }
+    // k tiles a pass of phase 3 holds: two in FP32 (each j slot read and written once for two
+    // pairs), one in FP64 (two would need the registers of a wave of the SIMD's two or three)
+    static constexpr int KTILES = sizeof(T) == 4 ? 2 : 1;
     __host__ __device__ static size_t region_bytes(int ne, int nref, bool kl) {
-        const size_t sums = kl ? (size_t)2 * A_NF * AW * sizeof(T) : 0, rows = (size_t)2 * ne * nref * sizeof(T);
+        // KL: (hi, lo) columns for each k tile of a pass
+        const size_t sums = kl ? (size_t)2 * KTILES * A_NF * AW * sizeof(T) : 0, rows = (size_t)2 * ne * nref * sizeof(T);
         return sums > rows ? sums : rows;
     }
     // dynamic LDS bytes of a launch (nref 0: pairwise C6).  one: a single element type, whose
@@ -170,13 +177,14 @@ struct AtmLds {
 // checks that the middle entry is d = 0).
 __device__ __forceinline__ int fwd_first(const SysDev *S) { return S->st_off[CUT_ATM] + (S->st_cnt[CUT_ATM] - 1) / 2; }
 
-// Waves per SIMD the compiler must allow: FP32 with 3 references fits 5 waves (96 VGPRs) without
-// spilling and runs 2.6 % faster on the fcc Cu cell; under that cap 2 references spill (1 %
-// slower on diamond Si) and 5 spill more (12 % slower on the water box), so they keep the
-// default.
+// Waves per SIMD the compiler must allow: FP32 with 3 or more references at 4 waves (128 VGPRs;
+// with 5 references some spilling, which still leaves the water box 13 % and the six-element
+// batch 4 % faster than at 3 waves); 1 or 2 references keep the default (diamond Si, 2
+// references: 8 % faster than forced to 4 waves); FP64 with 3 references at 3 waves (167 VGPRs,
+// one spilled pointer outside the loops).
 template <class T, int NR>
 constexpr int atm_min_waves() {
-    return sizeof(T) == 4 && NR == 3 ? 5 : 1;
+    return sizeof(T) == 4 ? (NR >= 3 ? 4 : 1) : (NR == 3 ? 3 : 1);
 }
 
 // KL: the k side's tile sums go to lane-private LDS (two-sum pairs) and reach the FP64
@@ -320,11 +328,13 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
         const int nf = count > cap ? 0 : count;
         const int ntile = (nf + AW - 1) / AW;
         // this work item's k tiles: chunk c of nch takes [ntile sqrt(c / nch), ntile sqrt((c + 1) / nch)),
-        // about the same number of tile pairs each (tile kb pairs with jt = 0..kb).  It touches
-        // the slots of tiles 0..kb_hi-1 only: phases 2 and 4 stop there (the other slots' sums
-        // are zero).
-        const int kb_lo = chunk == 0 ? 0 : min(ntile, (int)(ntile * sqrtf((float)chunk / nch) + 0.5f));
-        const int kb_hi = chunk == nch - 1 ? ntile : min(ntile, (int)(ntile * sqrtf((float)(chunk + 1) / nch) + 0.5f));
+        // about the same number of tile pairs each (tile kb pairs with jt = 0..kb), with the bounds
+        // rounded to even tiles: phase 3 takes k tiles in pairs (2m, 2m + 1) whatever the chunks,
+        // so every FP32 tile sum, and hence every result, is the same for any number of work items.
+        // It touches the slots of tiles 0..kb_hi-1 only: phases 2 and 4 stop there (the other
+        // slots' sums are zero).
+        auto even_tile = [&](int c) { return c <= 0 ? 0 : c >= nch ? ntile : min(ntile, 2 * (int)(0.5f * ntile * sqrtf((float)c / nch) + 0.5f)); };
+        const int kb_lo = even_tile(chunk), kb_hi = even_tile(chunk + 1);
         const int np = min(nf, kb_hi * AW);
         __syncthreads();  // phase-1 slots were written by arbitrary lanes
 
@@ -392,65 +402,371 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
             const T r0 = P.r0[ti * ne + tp];
             lam = F32 ? T((8.0f / 3.0f) * log2f((float)(r0 * r0 * ia))) : r0 * r0 * ia;
         };
-        for (int kb = kb_lo; kb < kb_hi; ++kb) {
-            const int k = kb * AW + lane;
-            const bool kval = k < nf;
+        // One k tile: lane l owns neighbour k = 64 kt + l, its pair terms with the centre and its
+        // tile sums (per reference b for the CN term: Gb[b] = sum (e / 2 c6) y_b(j), contracted
+        // with k's dw_b once per tile, one FMA per pair fewer than forming dC6_jk/dCN_k per pair).
+        // (Only what the steps read stays in registers through them: the slot index, validity and
+        // LDS columns of a tile are recomputed from its index where the sums are folded.)
+        struct KTile {
+            T x, y, z, b2, ib, rb, sc6h, lam, fac, cn;
+            T w[NR > 0 ? NR : 1], dw[NR > 0 ? NR : 1];
+            int t;
+            const T2<T> *Yk;  // rows of type(k) in the j tile: its damping row (DR), then its C6 rows
+            float lo[3], hi[3];  // bounding box of the tile's valid lanes (wave-uniform)
+            T A, Bx, By, Bz, H, G, Gb[NR > 0 ? NR : 1];
+        };
+        // kl: which of the two k tiles of a pass (LDS columns of its k-side sums with KL)
+        auto load_k = [&](KTile &K, int kt, int kl) __attribute__((always_inline)) {
+            const int k = kt * AW + lane;
             // padding lanes sit far away, so every pair with them fails the cutoff test
-            T kx = T(-FAR), ky = T(-FAR), kz = T(-FAR), kb2 = T(1), kib = T(1), krb = T(1), ksc6 = T(0), klam = T(0);
-            T kfac = T(0);  // FP32: 6 exp2(lam_k)
-            T kw[NR > 0 ? NR : 1], kdw[NR > 0 ? NR : 1];
-            T kcn = T(0);  // pairwise C6 only
-            int tk = 0;
-            for (int a = 0; a < NR; ++a) kw[a] = kdw[a] = T(0);
-            if (kval) {
+            K.x = K.y = K.z = T(-FAR);
+            K.b2 = K.ib = K.rb = T(1);
+            K.sc6h = K.lam = K.fac = K.cn = T(0);
+            K.t = 0;
+            for (int a = 0; a < NR; ++a) K.w[a] = K.dw[a] = T(0);
+            if (k < nf) {
                 const int jk = Lj[k];
-                kx = LT[F_DX * cap + k];
-                ky = LT[F_DY * cap + k];
-                kz = LT[F_DZ * cap + k];
-                tk = Lt[k];
+                K.x = LT[F_DX * cap + k];
+                K.y = LT[F_DY * cap + k];
+                K.z = LT[F_DZ * cap + k];
+                K.t = Lt[k];
                 for (int a = 0; a < NR; ++a) {
-                    kw[a] = P.w[a * P.n + jk];
-                    kdw[a] = P.dw[a * P.n + jk];
+                    K.w[a] = P.w[a * P.n + jk];
+                    K.dw[a] = P.dw[a * P.n + jk];
                 }
-                if constexpr (NR == 0) kcn = P.cn[jk];
+                if constexpr (NR == 0) K.cn = P.cn[jk];
                 T sc6, lam;
-                pair_terms(kx, ky, kz, tk, k, kb2, kib, krb, sc6, lam);
-                ksc6 = T(-0.5) * sc6;  // -sqrt(C6_ik) / 2: C9 / 2 below (pairwise C6 only)
-                klam = F32 ? lam + T(L2K6_83) : T(K6) * lam;
+                pair_terms(K.x, K.y, K.z, K.t, k, K.b2, K.ib, K.rb, sc6, lam);
+                K.sc6h = T(-0.5) * sc6;  // -sqrt(C6_ik) / 2: C9 / 2 below (pairwise C6 only)
+                K.lam = F32 ? lam + T(L2K6_83) : T(K6) * lam;
                 // The pair steps form c6'' = sum_b y'_b(j) kw'_b = (C6_ik / 4) C6_ij C6_jk from
                 // j rows staged times C6_ij and k weights times C6_ik / 4, so that C9 / 2 =
                 // -sqrt(c6'') and the log-derivative ratios (dC6_jk/dCN) / C6_jk are unchanged;
-                // kdw (used once per tile) takes the same factor.
+                // dw (used once per tile) takes the same factor.
                 const T sk = T(0.25) * sc6 * sc6;
                 for (int a = 0; a < NR; ++a) {
-                    kw[a] *= sk;
-                    kdw[a] *= sk;
+                    K.w[a] *= sk;
+                    K.dw[a] *= sk;
                 }
                 // FP32: 6 x^16 = exp2(lam_jk - (8/3) log2 r_jk^2) * 6 exp2(lam_k)
-                if constexpr (F32) kfac = T(6) * T(__builtin_amdgcn_exp2f((float)klam));
+                if constexpr (F32) K.fac = T(6) * T(__builtin_amdgcn_exp2f((float)K.lam));
             }
-            // bounding box of the k wave (valid lanes), wave-uniform
-            const float kminx = uni((float)LB[8 * kb + 0]), kmaxx = uni((float)LB[8 * kb + 1]);
-            const float kminy = uni((float)LB[8 * kb + 2]), kmaxy = uni((float)LB[8 * kb + 3]);
-            const float kminz = uni((float)LB[8 * kb + 4]), kmaxz = uni((float)LB[8 * kb + 5]);
-            // rows of type(k) in the j tile: its damping row (DR), then its C6 rows
-            const T2<T> *Yk = ONE ? S.Jyd : S.Jyd + (__shfl(roff_l, tk, AW) + (DR ? tk : 0)) * AW;
-            // k-side sums over all j tiles: lane-private LDS columns, folded to FP64 once per kb
+            for (int c = 0; c < 3; ++c) {
+                K.lo[c] = uni((float)LB[8 * kt + 2 * c]);
+                K.hi[c] = uni((float)LB[8 * kt + 2 * c + 1]);
+            }
+            K.Yk = ONE ? S.Jyd : S.Jyd + (__shfl(roff_l, K.t, AW) + (DR ? K.t : 0)) * AW;
+            // k-side sums over all j tiles: lane-private LDS columns, folded to FP64 once per tile
+            if constexpr (KL) {
+                T *Khi = S.Khi + kl * 2 * A_NF * AW, *Klo = Khi + A_NF * AW;
+                for (int q = 0; q < A_NF; ++q) Khi[q * AW + lane] = Klo[q * AW + lane] = T(0);
+            }
+        };
+        // the bounding boxes of a k tile and of the j tile (lo, hi) are within rc3 of each other
+        // (skip only when clearly apart: a 1e-3 relative margin covers float rounding)
+        auto near = [&](const KTile &K, const float *jlo, const float *jhi) __attribute__((always_inline)) {
+            float g2 = 0.f;
+            for (int c = 0; c < 3; ++c) {
+                const float g = fmaxf(0.f, fmaxf(jlo[c] - K.hi[c], K.lo[c] - jhi[c]));
+                g2 += g * g;
+            }
+            return g2 <= (float)rc32 * 1.001f;
+        };
+        auto zero_k = [&](KTile &K) __attribute__((always_inline)) {
+            K.A = K.Bx = K.By = K.Bz = K.H = K.G = T(0);
+            for (int bb = 0; bb < NR; ++bb) K.Gb[bb] = T(0);
+        };
+        // fold the tile sums of neighbour k (exact two-sum in FP32, plain in FP64)
+        auto fold_k = [&](KTile &K, int kt, int kl) __attribute__((always_inline)) {
+            if constexpr (NR > 0)
+                for (int bb = 0; bb < NR; ++bb) K.G += K.Gb[bb] * K.dw[bb];
+            const T v[A_NF] = {K.A, K.Bx, K.By, K.Bz, K.H, K.G};
+            const int k = kt * AW + lane;
+            if constexpr (KL) {
+                T *Khi = S.Khi + kl * 2 * A_NF * AW, *Klo = Khi + A_NF * AW;
+                for (int q = 0; q < A_NF; ++q) {
+                    const T hi = Khi[q * AW + lane], s = hi + v[q];
+                    if constexpr (F32) {
+                        const T bp = s - hi;
+                        Klo[q * AW + lane] += (hi - (s - bp)) + (v[q] - bp);
+                    }
+                    Khi[q * AW + lane] = s;
+                }
+            } else if (k < nf) {
+                for (int q = 0; q < A_NF; ++q) unsafeAtomicAdd(&La[q * cap + k], (double)v[q]);
+            }
+        };
+        auto close_k = [&](int kt, int kl) __attribute__((always_inline)) {
+            const int k = kt * AW + lane;
             if constexpr (KL)
-                for (int q = 0; q < A_NF; ++q) S.Khi[q * AW + lane] = S.Klo[q * AW + lane] = T(0);
+                if (k < nf) {
+                    const T *Khi = S.Khi + kl * 2 * A_NF * AW, *Klo = Khi + A_NF * AW;
+                    for (int q = 0; q < A_NF; ++q)
+                        unsafeAtomicAdd(&La[q * cap + k], (double)Khi[q * AW + lane] + (double)Klo[q * AW + lane]);
+                }
+        };
 
-            for (int jt = 0; jt <= kb; ++jt) {
+        // slot offsets: a j slot as a byte offset into any row of 64 slots, every row (the C6
+        // rows of type(k) too) that offset plus a base
+        constexpr int SB = (int)sizeof(T2<T>);
+        // fixed rows by index in the static array (J0..J2 = rows 0-2, then J3, Jp and Jt as
+        // AtmLds lays them out): the row offsets are immediates of the LDS instructions
+        auto at = [&](int row, int jb) -> T2<T> & {
+            return *reinterpret_cast<T2<T> *>(reinterpret_cast<unsigned char *>(fixed) + row * AW * SB + jb);
+        };
+        auto cat = [](const T2<T> *row, int jb) -> const T2<T> & {
+            return *reinterpret_cast<const T2<T> *>(reinterpret_cast<const unsigned char *>(row) + jb);
+        };
+        // the lane's own slot offset, opaque to the optimiser: (l SB) xor (q SB) would otherwise
+        // be rewritten as (l xor q) SB, two instructions per step instead of one
+        int lsb = lane * SB;
+        asm volatile("" : "+v"(lsb));
+
+        // Cutoff test of the pair {j, k} at slot offset jb: v = r_j - r_k, c = r_jk^2.
+        auto test = [&](const KTile &K, const T2<T> &j0, const T2<T> &j1, T &vx, T &vy, T &vz, T &c, bool &ok,
+                        uint64_t &m) __attribute__((always_inline)) {
+            vx = K.x - j0.x;
+            vy = K.y - j0.y;
+            vz = K.z - j1.x;
+            if constexpr (F32) {
+                // 1e-12 <= r_jk^2 <= rc3^2 as one unsigned compare of the bits of c = r_jk^2 - 1e-12
+                // (a negative c has its sign bit set); the 1e-12 is far below the rounding error of
+                // any r_jk^2 past the floor, so c stands for r_jk^2
+                c = fmaf(vz, vz, fmaf(vy, vy, fmaf(vx, vx, -1e-12f)));
+                ok = __builtin_bit_cast(unsigned, (float)c) <= rc32b;
+                m = __builtin_amdgcn_ballot_w64(ok);
+            } else {
+                c = vx * vx + vy * vy + vz * vz;
+                const bool okc = c <= rc32, okz = c >= T(1e-12);
+                ok = okc && okz;
+                // one ballot per compare: the ballot of the combined flag would be re-derived from
+                // a VGPR copy of it
+                m = __builtin_amdgcn_ballot_w64(okc) & __builtin_amdgcn_ballot_w64(okz);
+            }
+        };
+        // The damping term of the pair's types (with lam_j) and the C6 rows y_b(j) of type(k) at
+        // slot jb (with per-element references).
+        auto rows = [&](const KTile &K, int jb, T &lamjk, T &sc6j, T *yx, T *yy) __attribute__((always_inline)) {
+            sc6j = T(0);  // sqrt C6_ij (pairwise C6 only)
+            if constexpr (DR) {
+                lamjk = cat(K.Yk, jb).x;
+                if constexpr (NR == 0) sc6j = at(3, jb).x;
+            } else {
+                const T2<T> j3 = at(3, jb);
+                const T rt = S.Rt[*reinterpret_cast<const int *>(&at(JT, jb)) + K.t];
+                lamjk = F32 ? j3.y + rt : j3.y * rt;
+                sc6j = j3.x;
+            }
+            if constexpr (NR > 0)
+#pragma unroll
+                for (int bb = 0; bb < NR; ++bb) {
+                    const T2<T> yd = cat(K.Yk + (DRO + bb) * AW, jb);
+                    yx[bb] = yd.x;
+                    yy[bb] = yd.y;
+                }
+        };
+        // One pair {j, k}: its terms added to k's tile sums and to the j-side sums p0..p2 (FP32:
+        // registers the caller reads from and writes back to the j slot; FP64: the slot itself,
+        // by LDS atomics).
+        auto body = [&](KTile &K, int jb, T a, const T2<T> &j2, T vx, T vy, T vz, T c, T lamjk, T sc6j, const T *yx,
+                        const T *yy, T2<T> &p0, T2<T> &p1, T2<T> &p2) __attribute__((always_inline)) {
+            const T b = K.b2;
+            // C6 and dC6/dCN_j of the pair {j, k} from the j row for type(k) (the sums start from
+            // their first product: an FMA onto +0 is not a multiply in IEEE arithmetic).  With
+            // per-element references the rows and weights carry C6_ij and C6_ik / 4: c6 = (C6_ik
+            // / 4) C6_ij C6_jk, and C9 / 2 = -sqrt(c6).
+            T c6 = T(0), dk6 = T(0), dj6 = T(0);
+            if constexpr (NR == 0) {
+                const int jl = jb / SB;
+                c6_pairwise(P.c6g, ne, P.nref, S.Jt[jl * S.TI + 1], K.t, S.Jyd[jl].x, K.cn, c6, dj6, dk6);
+            } else {
+#pragma unroll
+                for (int bb = 0; bb < NR; ++bb) {
+                    c6 = bb == 0 ? yx[0] * K.w[0] : c6 + yx[bb] * K.w[bb];
+                    dj6 = bb == 0 ? yy[0] * K.w[0] : dj6 + yy[bb] * K.w[bb];
+                }
+            }
+            const T rs = prsqrt(c6);
+            const T rcq = prsqrt(c);
+            const T ic = rcq * rcq;
+            const T iR = (j2.y * K.rb) * rcq;  // (abc)^(-1/2)
+            const T iR2 = iR * iR, iR3 = iR2 * iR;
+            // Q = x y z, x = a + b - c, y = c - d, z = c + d with d = b - a, so yz = c^2 - d^2:
+            // dQ/da = yz + 2 x d, dQ/db = yz - 2 x d, dQ/dc = 2 x c - yz
+            const T d = b - a, x = (a + b) - c;
+            const T yz = c * c - d * d;
+            const T Q = x * yz;
+            const T z = Q * iR2;  // Q / (abc): A = iR3 (1 + (3/8) z)
+            const T x2 = x + x;
+            const T dQa = yz + x2 * d, dQb = yz - x2 * d, dQc = x2 * c - yz;
+            T f;
+            if constexpr (F32) {
+                // 6 x^16 = exp2(lam_jk - (8/3) log2 c) * 6 exp2(lam_k)
+                const T e = __builtin_amdgcn_exp2f(lamjk - (8.0f / 3.0f) * __builtin_amdgcn_logf(c));
+                f = prcp(T(1) + e * K.fac);
+            } else {
+                const T y = lamjk * K.lam * ic;
+                f = prcp(T(1) + T(6) * (y * y * y * inv_cbrt(y)));
+            }
+            // rf3 = (C9 / 2) f iR3 / c6 (per-element references: C9 / 2 = -sqrt(c6) = -c6 rs);
+            // every term below is a multiple of it
+            const T rf3 = (NR == 0 ? (sc6j * K.sc6h) * rs : -rs) * f * iR3;
+            const T Ap = T(1) + T(0.375) * z;  // A / iR3
+            const T cf3 = c6 * rf3;            // (C9 / 2) f iR3
+            const T hC = cf3 * Ap;             // e / 2
+            const T hc6 = rf3 * Ap;            // e / (2 c6)
+            // t' = (2/3) t_xy: the 3/2 of H2 is taken out of every t and restored where the
+            // per-neighbour sums are read (phase 4): G2' = cfh (P4 + (32/3) A (1 - f)) / 1.5
+            // = cf3 ((28/9 - (64/9) f) + z (1/6 - (8/3) f)), P4 = 4 P1, cfh = (C9 / 2) f
+            const T G2 = cf3 * ((T(28.0 / 9.0) - T(64.0 / 9.0) * f) + z * (T(1.0 / 6.0) - T(8.0 / 3.0) * f));
+            const T H2 = cf3 * iR2;  // cfh iR5
+            const T tij = G2 * j2.x + H2 * dQa;
+            const T tik = G2 * K.ib + H2 * dQb;
+            const T tjk = G2 * ic + H2 * dQc;
+            // Per-neighbour sums (F_p = (3/2) (B_p - A_p d_p), phase 4) with the {j, k} edge along
+            // v = d_k - d_j: grad_j e = -t_ij d_j + t_jk v, grad_k e = -t_ik d_k - t_jk v.
+            // Neighbour j: FP32 plain read-modify-write of its slot (distinct across lanes at each
+            // step; LDS float atomics measured ~6x slower on gfx950); FP64: LDS atomics (no
+            // per-step barrier, measured ~4% faster than the barrier form).
+            if constexpr (F32) {
+                p0.x += tij;
+                p0.y += tjk * vx;
+                p1.x += tjk * vy;
+                p1.y += tjk * vz;
+                p2.x += hC;
+                p2.y += hc6 * dj6;
+            } else {
+                T2<T> &r0 = at(JP, jb), &r1 = at(JP + 1, jb), &r2 = at(JP + 2, jb);
+                atomicAdd(&r0.x, tij);
+                atomicAdd(&r0.y, tjk * vx);
+                atomicAdd(&r1.x, tjk * vy);
+                atomicAdd(&r1.y, tjk * vz);
+                atomicAdd(&r2.x, hC);
+                atomicAdd(&r2.y, hc6 * dj6);
+            }
+            // neighbour k: registers
+            K.A += tik;
+            K.Bx -= tjk * vx;
+            K.By -= tjk * vy;
+            K.Bz -= tjk * vz;
+            K.H += hC;
+            if constexpr (NR == 0) {
+                K.G += hc6 * dk6;
+            } else {
+#pragma unroll
+                for (int bb = 0; bb < NR; ++bb) K.Gb[bb] += hc6 * yx[bb];
+            }
+        };
+        // The steps of one k tile against the staged j tile.  On the diagonal tile pair (the j tile
+        // is the k tile) every unordered pair {j, k} comes up at two steps, q and 64 - q, of the
+        // rotation that pairs lane l with slot (l + q) mod 64; the terms are symmetric in j and k,
+        // so steps 1..31 and the lower half of the lanes at step 32 take each pair once (32 steps,
+        // not 64).  The other tile pairs pair lane l with slot l xor q, one instruction per step
+        // (a permutation of the slots at every step, and every (j, k) pair comes up once over the
+        // 64 steps), and need no order test.
+        auto steps1 = [&](KTile &K, auto diag) __attribute__((always_inline)) {
+            constexpr bool DIAG = decltype(diag)::value;
+            constexpr int Q0 = DIAG ? 1 : 0, Q1 = DIAG ? AW / 2 + 1 : AW;
+#pragma unroll 1
+            for (int q = Q0; q < Q1; ++q) {
+                // FP32: lane l updates a j slot that another lane updated one step earlier
+                if constexpr (F32) wave_sync();
+                const int jb = DIAG ? ((lane * SB + q * SB) & (AW * SB - 1)) : (lsb ^ (q * SB));
+                __builtin_assume(jb >= 0 && jb < AW * SB);
+                const T2<T> j0 = at(0, jb), j1 = at(1, jb);
+                T vx, vy, vz, c;
+                bool ok;
+                uint64_t m;
+                test(K, j0, j1, vx, vy, vz, c, ok, m);
+                if constexpr (DIAG) {
+                    if (q == AW / 2) {  // lanes l and l + 32 share the pair
+                        m &= 0xffffffffull;
+                        ok = ok && lane < AW / 2;
+                    }
+                }
+                if (m == 0) continue;  // uniform
+                cnt += __popcll(m);
+                if (!ok) continue;
+                const T2<T> j2 = at(2, jb);
+                T lamjk, sc6j, yx[NR > 0 ? NR : 1], yy[NR > 0 ? NR : 1];
+                rows(K, jb, lamjk, sc6j, yx, yy);
+                if constexpr (F32) {
+                    T2<T> &r0 = at(JP, jb), &r1 = at(JP + 1, jb), &r2 = at(JP + 2, jb);
+                    T2<T> p0 = r0, p1 = r1, p2 = r2;
+                    body(K, jb, j1.y, j2, vx, vy, vz, c, lamjk, sc6j, yx, yy, p0, p1, p2);
+                    r0 = p0;
+                    r1 = p1;
+                    r2 = p2;
+                } else {
+                    T2<T> p0, p1, p2;
+                    body(K, jb, j1.y, j2, vx, vy, vz, c, lamjk, sc6j, yx, yy, p0, p1, p2);
+                }
+            }
+        };
+        // Two k tiles against the staged j tile (off-diagonal for both): the j slot is read, and
+        // its sums written back, once for two pairs.
+        auto steps2 = [&](KTile &K1, KTile &K2) __attribute__((always_inline)) {
+#pragma unroll 1
+            for (int q = 0; q < AW; ++q) {
+                if constexpr (F32) wave_sync();
+                const int jb = lsb ^ (q * SB);
+                __builtin_assume(jb >= 0 && jb < AW * SB);
+                const T2<T> j0 = at(0, jb), j1 = at(1, jb);
+                T vx1, vy1, vz1, c1, vx2, vy2, vz2, c2;
+                bool ok1, ok2;
+                uint64_t m1, m2;
+                test(K1, j0, j1, vx1, vy1, vz1, c1, ok1, m1);
+                test(K2, j0, j1, vx2, vy2, vz2, c2, ok2, m2);
+                if ((m1 | m2) == 0) continue;  // uniform
+                cnt += __popcll(m1) + __popcll(m2);
+                const T2<T> j2 = at(2, jb);
+                T lamjk, sc6j, yx[NR > 0 ? NR : 1], yy[NR > 0 ? NR : 1];
+                // one element type: the rows do not depend on k
+                if constexpr (ONE) rows(K1, jb, lamjk, sc6j, yx, yy);
+                T2<T> p0, p1, p2;
+                if constexpr (F32) {
+                    p0 = at(JP, jb);
+                    p1 = at(JP + 1, jb);
+                    p2 = at(JP + 2, jb);
+                }
+                if (ok1) {
+                    if constexpr (!ONE) rows(K1, jb, lamjk, sc6j, yx, yy);
+                    body(K1, jb, j1.y, j2, vx1, vy1, vz1, c1, lamjk, sc6j, yx, yy, p0, p1, p2);
+                }
+                if (ok2) {
+                    if constexpr (!ONE) rows(K2, jb, lamjk, sc6j, yx, yy);
+                    body(K2, jb, j1.y, j2, vx2, vy2, vz2, c2, lamjk, sc6j, yx, yy, p0, p1, p2);
+                }
+                if constexpr (F32) {
+                    at(JP, jb) = p0;
+                    at(JP + 1, jb) = p1;
+                    at(JP + 2, jb) = p2;
+                }
+            }
+        };
+
+        // FP32: k tiles two at a time (K1 = tile kb, K2 = tile kb + 1 when this work item has it):
+        // each j tile is staged once for both, and on the tiles below kb both run in the same
+        // steps.  FP64: one at a time.
+        constexpr int KT = AtmLds<T>::KTILES;
+        for (int kb = kb_lo; kb < kb_hi; kb += KT) {
+            const bool two = KT == 2 && kb + 1 < kb_hi;
+            KTile K1, K2;
+            load_k(K1, kb, 0);
+            if (two) load_k(K2, kb + 1, 1);
+            const int jt_end = two ? kb + 1 : kb;
+            for (int jt = 0; jt <= jt_end; ++jt) {
                 const int pj = jt * AW + lane;
                 const bool jval = pj < nf;
                 // bounding box of the j tile (phase 2)
-                const float jminx = uni((float)LB[8 * jt + 0]), jmaxx = uni((float)LB[8 * jt + 1]);
-                const float jminy = uni((float)LB[8 * jt + 2]), jmaxy = uni((float)LB[8 * jt + 3]);
-                const float jminz = uni((float)LB[8 * jt + 4]), jmaxz = uni((float)LB[8 * jt + 5]);
-                const float gx = fmaxf(0.f, fmaxf(jminx - kmaxx, kminx - jmaxx));
-                const float gy = fmaxf(0.f, fmaxf(jminy - kmaxy, kminy - jmaxy));
-                const float gz = fmaxf(0.f, fmaxf(jminz - kmaxz, kminz - jmaxz));
-                // skip only when clearly apart (1e-3 relative margin covers float rounding)
-                if (gx * gx + gy * gy + gz * gz > (float)rc32 * 1.001f) continue;
+                float jlo[3], jhi[3];
+                for (int c = 0; c < 3; ++c) {
+                    jlo[c] = uni((float)LB[8 * jt + 2 * c]);
+                    jhi[c] = uni((float)LB[8 * jt + 2 * c + 1]);
+                }
+                const bool u1 = jt <= kb && near(K1, jlo, jhi);
+                const bool u2 = two && near(K2, jlo, jhi);
+                if (!u1 && !u2) continue;
 
                 // stage the j tile (padding lanes: far coordinates, zero rows)
                 wave_sync();
@@ -492,7 +808,15 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                         const int base = __builtin_amdgcn_readlane(roff_l, B) + (DR ? B : 0);
                         if constexpr (DR) {
                             const T rt = rt_of(tj * ne + B);
-                            S.Jyd[base * AW + lane] = T2<T>{F32 ? lam + rt : lam * rt, T(0)};
+                            T lr;
+                            {
+                                // rt rounded on its own, as the per-type-pair table of the variant
+                                // without DR holds it (a fused lam + (8/3) log2 R0^2 would differ
+                                // in the last bit, and with it every result of that variant)
+#pragma clang fp contract(off)
+                                lr = F32 ? lam + rt : lam * rt;
+                            }
+                            S.Jyd[base * AW + lane] = T2<T>{lr, T(0)};
                         }
                         // rows times C6_ij = sc6^2 (see the k weights)
                         const T c6ij = sc6 * sc6;
@@ -511,205 +835,37 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                 }
                 wave_sync();
 
-                // tile-local sums of neighbour k (folded into FP64 per tile).  The CN term of k
-                // is kept per reference b, kGb[b] = sum (e/2C6) y_b(j), and contracted with k's
-                // dw_b once per tile: one FMA per pair fewer than forming dC6_jk/dCN_k per pair.
-                T kA = T(0), kBx = T(0), kBy = T(0), kBz = T(0), kH = T(0), kG = T(0);
-                T kGb[NR > 0 ? NR : 1];
-                for (int bb = 0; bb < NR; ++bb) kGb[bb] = T(0);
-                // On the diagonal tile (jt == kb) every unordered pair {j, k} comes up at two
-                // steps, q and 64 - q; the terms are symmetric in j and k, so steps 1..31 and the
-                // lower half of the lanes at step 32 take each pair once (32 steps, not 64).  The
-                // other tiles take every step and need no order test; the step loop is
-                // instantiated for both.
-                auto steps = [&](auto diag) {
-                constexpr bool DIAG = decltype(diag)::value;
-                constexpr int Q0 = DIAG ? 1 : 0, Q1 = DIAG ? AW / 2 + 1 : AW;
-                // slot (lane + q) mod 64 as a byte offset into any row of 64 slots: one add and
-                // one mask per step, and every row (the C6 rows of type(k) too) is that offset
-                // plus a base
-                constexpr int SB = (int)sizeof(T2<T>);
-                // fixed rows by index in the static array (J0..J2 = rows 0-2, then J3, Jp and Jt
-                // as AtmLds lays them out): the row offsets are immediates of the LDS instructions
-                auto at = [&](int row, int jb) -> T2<T> & {
-                    return *reinterpret_cast<T2<T> *>(reinterpret_cast<unsigned char *>(fixed) + row * AW * SB + jb);
-                };
-                auto cat = [](const T2<T> *row, int jb) -> const T2<T> & {
-                    return *reinterpret_cast<const T2<T> *>(reinterpret_cast<const unsigned char *>(row) + jb);
-                };
-                // the lane's own slot offset, opaque to the optimiser: (l SB) xor (q SB) would
-                // otherwise be rewritten as (l xor q) SB, two instructions per step instead of one
-                int lsb = lane * SB;
-                asm volatile("" : "+v"(lsb));
-#pragma unroll 1
-                for (int q = Q0; q < Q1; ++q) {
-                    // FP32: lane l updates a j slot that another lane updated one step earlier
-                    if constexpr (F32) wave_sync();
-                    // the diagonal tile pairs lane l with slot (l + q) mod 64 (each unordered pair
-                    // comes up twice, at q and 64 - q); the others with slot l xor q, one
-                    // instruction per step (also a permutation of the slots at every step, and
-                    // every (j, k) pair of the tile pair comes up once over the 64 steps)
-                    const int jb = DIAG ? ((lane * SB + q * SB) & (AW * SB - 1)) : (lsb ^ (q * SB));
-                    __builtin_assume(jb >= 0 && jb < AW * SB);
-                    const T2<T> j0 = at(0, jb), j1 = at(1, jb);
-                    const T jx = j0.x, jy = j0.y, jz = j1.x;
-                    const T vx = kx - jx, vy = ky - jy, vz = kz - jz;  // r_j - r_k
-                    T c;
-                    bool ok;
-                    uint64_t m;
-                    if constexpr (F32) {
-                        // 1e-12 <= r_jk^2 <= rc3^2 as one unsigned compare of the bits of
-                        // c = r_jk^2 - 1e-12 (a negative c has its sign bit set); the 1e-12 is far
-                        // below the rounding error of any r_jk^2 past the floor, so c stands for r_jk^2
-                        c = fmaf(vz, vz, fmaf(vy, vy, fmaf(vx, vx, -1e-12f)));
-                        ok = __builtin_bit_cast(unsigned, (float)c) <= rc32b;
-                        m = __builtin_amdgcn_ballot_w64(ok);
+                if (jt < kb) {
+                    if (u1 && u2) {
+                        zero_k(K1);
+                        zero_k(K2);
+                        steps2(K1, K2);
+                        fold_k(K1, kb, 0);
+                        fold_k(K2, kb + 1, 1);
+                    } else if (u1) {
+                        zero_k(K1);
+                        steps1(K1, std::false_type{});
+                        fold_k(K1, kb, 0);
                     } else {
-                        c = vx * vx + vy * vy + vz * vz;
-                        const bool okc = c <= rc32, okz = c >= T(1e-12);
-                        ok = okc && okz;
-                        // one ballot per compare: the ballot of the combined flag would be
-                        // re-derived from a VGPR copy of it
-                        m = __builtin_amdgcn_ballot_w64(okc) & __builtin_amdgcn_ballot_w64(okz);
+                        zero_k(K2);
+                        steps1(K2, std::false_type{});
+                        fold_k(K2, kb + 1, 1);
                     }
-                    if constexpr (DIAG) {
-                        if (q == AW / 2) {  // lanes l and l + 32 share the pair
-                            m &= 0xffffffffull;
-                            ok = ok && lane < AW / 2;
-                        }
+                } else if (jt == kb) {
+                    if (u1) {
+                        zero_k(K1);
+                        steps1(K1, std::true_type{});
+                        fold_k(K1, kb, 0);
                     }
-                    if (m == 0) continue;  // uniform
-                    cnt += __popcll(m);
-                    if (!ok) continue;
-                    const T a = j1.y, b = kb2;
-                    const T2<T> j2 = at(2, jb);
-                    T lamjk;  // damping term of the pair's types (with lam_j)
-                    T sc6j = T(0);  // sqrt C6_ij (pairwise C6 only)
-                    if constexpr (DR) {
-                        lamjk = cat(Yk, jb).x;
-                        if constexpr (NR == 0) sc6j = at(3, jb).x;
-                    } else {
-                        const T2<T> j3 = at(3, jb);
-                        const T rt = S.Rt[*reinterpret_cast<const int *>(&at(JT, jb)) + tk];
-                        lamjk = F32 ? j3.y + rt : j3.y * rt;
-                        sc6j = j3.x;
+                    if (u2) {
+                        zero_k(K2);
+                        steps1(K2, std::false_type{});
+                        fold_k(K2, kb + 1, 1);
                     }
-                    // C6 and dC6/dCN_j of the pair {j, k} from the j row for type(k) (the sums
-                    // start from their first product: an FMA onto +0 is not a multiply in IEEE
-                    // arithmetic).  With per-element references the rows and weights carry C6_ij
-                    // and C6_ik / 4: c6 = (C6_ik / 4) C6_ij C6_jk, and C9 / 2 = -sqrt(c6).
-                    T c6 = T(0), dk6 = T(0), dj6 = T(0);
-                    T ydx[NR > 0 ? NR : 1];
-                    if constexpr (NR == 0) {
-                        const int jl = jb / SB;
-                        c6_pairwise(P.c6g, ne, P.nref, S.Jt[jl * S.TI + 1], tk, S.Jyd[jl].x, kcn, c6, dj6, dk6);
-                    } else {
-#pragma unroll
-                        for (int bb = 0; bb < NR; ++bb) {
-                            const T2<T> yd = cat(Yk + (DRO + bb) * AW, jb);
-                            ydx[bb] = yd.x;
-                            c6 = bb == 0 ? yd.x * kw[0] : c6 + yd.x * kw[bb];
-                            dj6 = bb == 0 ? yd.y * kw[0] : dj6 + yd.y * kw[bb];
-                        }
-                    }
-                    const T rs = prsqrt(c6);
-                    const T rcq = prsqrt(c);
-                    const T ic = rcq * rcq;
-                    const T iR = (j2.y * krb) * rcq;  // (abc)^(-1/2)
-                    const T iR2 = iR * iR, iR3 = iR2 * iR;
-                    // Q = x y z, x = a + b - c, y = c - d, z = c + d with d = b - a, so yz = c^2 - d^2:
-                    // dQ/da = yz + 2 x d, dQ/db = yz - 2 x d, dQ/dc = 2 x c - yz
-                    const T d = b - a, x = (a + b) - c;
-                    const T yz = c * c - d * d;
-                    const T Q = x * yz;
-                    const T z = Q * iR2;  // Q / (abc): A = iR3 (1 + (3/8) z)
-                    const T x2 = x + x;
-                    const T dQa = yz + x2 * d, dQb = yz - x2 * d, dQc = x2 * c - yz;
-                    T f;
-                    if constexpr (F32) {
-                        // 6 x^16 = exp2(lam_jk - (8/3) log2 c) * 6 exp2(lam_k)
-                        const T e = __builtin_amdgcn_exp2f(lamjk - (8.0f / 3.0f) * __builtin_amdgcn_logf(c));
-                        f = prcp(T(1) + e * kfac);
-                    } else {
-                        const T y = lamjk * klam * ic;
-                        f = prcp(T(1) + T(6) * (y * y * y * inv_cbrt(y)));
-                    }
-                    // rf3 = (C9 / 2) f iR3 / c6 (per-element references: C9 / 2 = -sqrt(c6) =
-                    // -c6 rs); every term below is a multiple of it
-                    const T rf3 = (NR == 0 ? (sc6j * ksc6) * rs : -rs) * f * iR3;
-                    const T Ap = T(1) + T(0.375) * z;  // A / iR3
-                    const T cf3 = c6 * rf3;            // (C9 / 2) f iR3
-                    const T hC = cf3 * Ap;             // e / 2
-                    const T hc6 = rf3 * Ap;            // e / (2 c6)
-                    // t' = (2/3) t_xy: the 3/2 of H2 is taken out of every t and restored where the
-                    // per-neighbour sums are read (phase 4): G2' = cfh (P4 + (32/3) A (1 - f)) / 1.5
-                    // = cf3 ((28/9 - (64/9) f) + z (1/6 - (8/3) f)), P4 = 4 P1, cfh = (C9 / 2) f
-                    const T G2 = cf3 * ((T(28.0 / 9.0) - T(64.0 / 9.0) * f) + z * (T(1.0 / 6.0) - T(8.0 / 3.0) * f));
-                    const T H2 = cf3 * iR2;  // cfh iR5
-                    const T tij = G2 * j2.x + H2 * dQa;
-                    const T tik = G2 * kib + H2 * dQb;
-                    const T tjk = G2 * ic + H2 * dQc;
-                    // Per-neighbour sums (F_p = (3/2) (B_p - A_p d_p), phase 4) with the {j, k} edge along
-                    // v = d_k - d_j: grad_j e = -t_ij d_j + t_jk v, grad_k e = -t_ik d_k - t_jk v.
-                    // Neighbour j: LDS slot, distinct across lanes at each step.  FP32: plain
-                    // read-modify-write (LDS float atomics measured ~6x slower on gfx950); FP64:
-                    // LDS atomics (no per-step barrier, measured ~4% faster than the barrier form).
-                    if constexpr (F32) {
-                        T2<T> &r0 = at(JP, jb), &r1 = at(JP + 1, jb), &r2 = at(JP + 2, jb);
-                        T2<T> p0 = r0, p1 = r1, p2 = r2;
-                        p0.x += tij;
-                        p0.y += tjk * vx;
-                        p1.x += tjk * vy;
-                        p1.y += tjk * vz;
-                        p2.x += hC;
-                        p2.y += hc6 * dj6;
-                        r0 = p0;
-                        r1 = p1;
-                        r2 = p2;
-                    } else {
-                        T2<T> &r0 = at(JP, jb), &r1 = at(JP + 1, jb), &r2 = at(JP + 2, jb);
-                        atomicAdd(&r0.x, tij);
-                        atomicAdd(&r0.y, tjk * vx);
-                        atomicAdd(&r1.x, tjk * vy);
-                        atomicAdd(&r1.y, tjk * vz);
-                        atomicAdd(&r2.x, hC);
-                        atomicAdd(&r2.y, hc6 * dj6);
-                    }
-                    // neighbour k: registers
-                    kA += tik;
-                    kBx -= tjk * vx;
-                    kBy -= tjk * vy;
-                    kBz -= tjk * vz;
-                    kH += hC;
-                    if constexpr (NR == 0) {
-                        kG += hc6 * dk6;
-                    } else {
-#pragma unroll
-                        for (int bb = 0; bb < NR; ++bb) kGb[bb] += hc6 * ydx[bb];
-                    }
-                }
-                };
-                if (jt == kb)
-                    steps(std::true_type{});
-                else
-                    steps(std::false_type{});
-                {
-                    // fold the tile sums of neighbour k (exact two-sum in FP32, plain in FP64)
-                    if constexpr (NR > 0)
-                        for (int bb = 0; bb < NR; ++bb) kG += kGb[bb] * kdw[bb];
-                    const T v[A_NF] = {kA, kBx, kBy, kBz, kH, kG};
-                    if constexpr (KL) {
-                        for (int q = 0; q < A_NF; ++q) {
-                            const T hi = S.Khi[q * AW + lane], s = hi + v[q];
-                            if constexpr (F32) {
-                                const T bp = s - hi;
-                                S.Klo[q * AW + lane] += (hi - (s - bp)) + (v[q] - bp);
-                            }
-                            S.Khi[q * AW + lane] = s;
-                        }
-                    } else if (kval) {
-                        for (int q = 0; q < A_NF; ++q) unsafeAtomicAdd(&La[q * cap + k], (double)v[q]);
-                    }
+                } else {  // jt == kb + 1: K2's own tile
+                    zero_k(K2);
+                    steps1(K2, std::true_type{});
+                    fold_k(K2, kb + 1, 1);
                 }
                 wave_sync();
                 if (jval)
@@ -719,10 +875,8 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                         unsafeAtomicAdd(&La[(2 * q + 1) * cap + pj], (double)pq.y);
                     }
             }
-            if constexpr (KL)
-                if (kval)
-                    for (int q = 0; q < A_NF; ++q)
-                        unsafeAtomicAdd(&La[q * cap + k], (double)S.Khi[q * AW + lane] + (double)S.Klo[q * AW + lane]);
+            close_k(kb, 0);
+            if (two) close_k(kb + 1, 1);
         }
 
         // ---- phase 4.  Per neighbour slot p: F_p = grad_p E = (3/2) (B_p - d_p A_p), H_p = sum of e/2,
