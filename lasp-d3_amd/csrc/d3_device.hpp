@@ -37,6 +37,15 @@ __device__ __forceinline__ double readlane_t(double v, int k) {
     return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 
+// Element i of a per-atom array through a 32-bit byte offset: the load takes the array base in
+// scalar registers and one 32-bit offset register instead of 64-bit address arithmetic per lane.
+// Handles hold at most 2^28 atoms (max_length), so every offset of an array of 8-byte elements
+// indexed by atom fits.
+template <class U>
+__device__ __forceinline__ U at32(const U *base, unsigned i) {
+    return *reinterpret_cast<const U *>(reinterpret_cast<const char *>(base) + i * (unsigned)sizeof(U));
+}
+
 __device__ __forceinline__ int wrapi(int x, int n) {
     int r = x % n;
     return r < 0 ? r + n : r;

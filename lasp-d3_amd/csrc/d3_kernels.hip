@@ -165,8 +165,97 @@ __global__ void k_regather(Params<T> P) {
 }
 
 // ---------------------------------------------------------------------------------------------
+// Pair walks.  A wave walks its centre's stencil entries [s0, s1); lanes test 64 atoms of a bin
+// at a time against the cutoff (0 < r^2 <= rc^2, or < rc^2 when strict), the accepted pairs go to
+// the wave's ring queue in LDS (ballot + mbcnt), and every 64 queued pairs are evaluated by the
+// whole wave, one pair per lane (the remainder at the end by the lanes that have one).  The
+// loads of a candidate are clamped to the bin (no lane-dependent branch) and addressed by 32-bit
+// offsets.
+// ---------------------------------------------------------------------------------------------
+template <class T>
+struct Queue {
+    int j[QCAP];
+    T dx[QCAP], dy[QCAP], dz[QCAP];
+};
+
+template <class T, bool STRICT, class Eval>
+__device__ __forceinline__ void walk_pairs(const Params<T> &P, const Centre<T> &C, int s0, int s1, T rc, T rc2,
+                                           Queue<T> &Q, Eval &&eval) {
+    const int lane = lane_id();
+    int qt = 0, qh = 0;  // pairs queued, pairs evaluated (ring positions modulo QCAP)
+    for (int sb = s0; sb < s1; sb += WAVE) {
+        const auto Bn = C.bins(P, sb, s1, rc);
+        const int nsb = s1 - sb < WAVE ? s1 - sb : WAVE;
+        for (int sk = 0; sk < nsb; ++sk) {
+            int j0, j1;
+            T ox, oy, oz;
+            Bn.get(sk, j0, j1, ox, oy, oz);
+            for (int jb = j0; jb < j1; jb += WAVE) {
+                const unsigned j = (unsigned)(jb + lane);
+                const unsigned jj = min(j, (unsigned)(j1 - 1));
+                const T dx = ox - at32(P.lx, jj), dy = oy - at32(P.ly, jj), dz = oz - at32(P.lz, jj);
+                const T r2 = dx * dx + dy * dy + dz * dz;
+                const bool in = j < (unsigned)j1 && (STRICT ? r2 < rc2 : r2 <= rc2) && r2 > T(0);
+                const uint64_t m = ballot64(in);
+                if (m == 0) continue;
+                if (in) {
+                    const int pos = (qt + mbcnt64(m)) & (QCAP - 1);
+                    Q.j[pos] = (int)j;
+                    Q.dx[pos] = dx;
+                    Q.dy[pos] = dy;
+                    Q.dz[pos] = dz;
+                }
+                qt += __popcll(m);
+                if (qt - qh >= WAVE) {
+                    wave_sync();
+                    const int pos = (qh + lane) & (QCAP - 1);
+                    eval(Q.j[pos], Q.dx[pos], Q.dy[pos], Q.dz[pos]);
+                    qh += WAVE;
+                    wave_sync();  // the next pairs may go to the slots just read
+                }
+            }
+        }
+    }
+    wave_sync();
+    if (lane < qt - qh) {
+        const int pos = (qh + lane) & (QCAP - 1);
+        eval(Q.j[pos], Q.dx[pos], Q.dy[pos], Q.dz[pos]);
+    }
+}
+
+// Per-lane sums of N terms: plain partial sums over at most FOLD pairs, folded into the
+// compensated totals (two-sum in FP32) -- one compensated add per FOLD pairs instead of per pair.
+template <class T, int N>
+struct AccN {
+    static constexpr int FOLD = 16;
+    Acc<T> t[N];
+    T p[N];
+    int np;
+    __device__ void init() {
+        for (int k = 0; k < N; ++k) t[k].init();
+        clear();
+    }
+    __device__ void clear() {
+        for (int k = 0; k < N; ++k) p[k] = T(0);
+        np = 0;
+    }
+    __device__ void fold() {
+        for (int k = 0; k < N; ++k) t[k].add(p[k]);
+        clear();
+    }
+    __device__ void add(const T v[N]) {
+        if constexpr (sizeof(T) == sizeof(float)) {
+            for (int k = 0; k < N; ++k) p[k] += v[k];
+            if (++np == FOLD) fold();
+        } else {
+            for (int k = 0; k < N; ++k) t[k].add(v[k]);
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------------------------
 // Coordination number, src/d3_kernel.cu:544-574 (calculate_CN) and 582-683: one wave per
-// centre atom, lanes stride over the atoms of each stencil bin.  CN_i = sum over
+// centre atom over the pairs of its CN stencil (walk_pairs).  CN_i = sum over
 // 0 < r^2 <= rc_CN^2 of 1 / (1 + exp(-k1 ((Rcov_i + Rcov_j) / r - 1))).  The same wave then
 // forms the centre's normalised Gaussian weights of the C6 interpolation:
 //   w_a = exp(-k3 (CN - CNref_a)^2) / sum_b exp(-k3 (CN - CNref_b)^2), a valid iff CNref_a > -1
@@ -176,8 +265,9 @@ __global__ void k_regather(Params<T> P) {
 // ---------------------------------------------------------------------------------------------
 template <class T>
 __global__ __launch_bounds__(BLOCK) void k_cn(Params<T> P) {
-    const int lane = lane_id();
-    const int i = blockIdx.x * WPB + (threadIdx.x >> 6);
+    __shared__ Queue<T> Qs[WPB];
+    const int lane = lane_id(), wv = threadIdx.x >> 6;
+    const int i = blockIdx.x * WPB + wv;
     if (i >= P.n) return;
     Centre<T> C;
     C.load(P, i);
@@ -186,30 +276,17 @@ __global__ __launch_bounds__(BLOCK) void k_cn(Params<T> P) {
     acc.init();
     float cnt = 0.0f;
     const int s0 = C.S->st_off[CUT_CN], s1 = s0 + C.S->st_cnt[CUT_CN];
-    for (int sb = s0; sb < s1; sb += WAVE) {
-        const auto Bn = C.bins(P, sb, s1, P.rc_cn);
-        const int nsb = s1 - sb < WAVE ? s1 - sb : WAVE;
-        for (int sk = 0; sk < nsb; ++sk) {
-            int j0, j1;
-            T ox, oy, oz;
-            Bn.get(sk, j0, j1, ox, oy, oz);
-            for (int j = j0 + lane; j < j1; j += WAVE) {
-                const T dx = ox - P.lx[j], dy = oy - P.ly[j], dz = oz - P.lz[j];
-                const T r2 = dx * dx + dy * dy + dz * dz;
-                if (r2 <= P.rc_cn2 && r2 > T(0)) {
-                    const T r = xsqrt(r2);
-                    const T ex = xexp(-T(K1) * ((rci + P.rcov[P.type_s[j]]) / r - T(1)));
-                    T c = T(1) / (T(1) + ex);
-                    if (P.cn_smooth) {
-                        T unused = T(0);
-                        smooth_cn_term(P.rc_cn, r, T(1) / r, c, unused);
-                    }
-                    acc.add(c);
-                    cnt += 1.0f;
-                }
-            }
+    walk_pairs<T, false>(P, C, s0, s1, P.rc_cn, P.rc_cn2, Qs[wv], [&](int j, T dx, T dy, T dz) {
+        const T r = xsqrt(dx * dx + dy * dy + dz * dz);
+        const T ex = xexp(-T(K1) * ((rci + P.rcov[at32(P.type_s, (unsigned)j)]) / r - T(1)));
+        T c = T(1) / (T(1) + ex);
+        if (P.cn_smooth) {
+            T unused = T(0);
+            smooth_cn_term(P.rc_cn, r, T(1) / r, c, unused);
         }
-    }
+        acc.add(c);
+        cnt += 1.0f;
+    });
     const double cnd = wave_total(acc);
     cnt = wave_sumf(cnt);
     if (lane == 0) {
@@ -255,11 +332,6 @@ __global__ __launch_bounds__(BLOCK) void k_cn(Params<T> P) {
 // Damping (src/d3_kernel.cu:359-424): zero f_n = 1/(1 + 6 (s_rn R0/r)^a_n), a = 14, 16, R0 = r0ab;
 // BJ f_n = r^n / (r^n + (a1 sqrt(3 r2r4_A r2r4_B) + a2)^n).
 // ---------------------------------------------------------------------------------------------
-template <class T>
-struct Queue {
-    int j[QCAP];
-    T dx[QCAP], dy[QCAP], dz[QCAP];
-};
 
 // Per-lane sums of the two-body terms: plain partial sums over at most FOLD pairs, folded into
 // the compensated totals (two-sum in FP32) — one compensated add per FOLD pairs instead of per
@@ -327,15 +399,15 @@ __device__ __forceinline__ double prsqrt(double x) { return rsqrt(x); }
 template <class T, int NR>
 __device__ __forceinline__ void pair_2b(const Params<T> &P, int j, T dx, T dy, T dz, const T *y, const T *dy_,
                                         const T *rq, const T *rR, const T *rX, int ti, T cni, Acc2b<T> &A) {
-    const int tj = P.type_s[j];
+    const int tj = at32(P.type_s, (unsigned)j);
     T c6 = T(0), dc6 = T(0);
     if constexpr (NR == 0) {
         T dc6j;
-        c6_pairwise(P.c6g, P.ne, P.nref, ti, tj, cni, P.cn[j], c6, dc6, dc6j);
+        c6_pairwise(P.c6g, P.ne, P.nref, ti, tj, cni, at32(P.cn, (unsigned)j), c6, dc6, dc6j);
     } else {
         T wj[NR];
 #pragma unroll
-        for (int b = 0; b < NR; ++b) wj[b] = P.w[b * P.n + j];
+        for (int b = 0; b < NR; ++b) wj[b] = at32(P.w + (size_t)b * P.n, (unsigned)j);
 #pragma unroll
         for (int b = 0; b < NR; ++b) {
             c6 += y[tj * NR + b] * wj[b];
@@ -432,71 +504,14 @@ __global__ __launch_bounds__(BLOCK) void k_2b(Params<T> P) {
 
     Acc2b<T> A;
     A.init();
-    int qn = 0;
     float cnt = 0.0f;
     // this wave's share of the stencil (all of it with one part per centre)
     const int sa = C.S->st_off[CUT_2B], sz = C.S->st_cnt[CUT_2B], per = (sz + nparts - 1) / nparts;
     const int s0 = sa + min(sz, part * per), s1 = sa + min(sz, (part + 1) * per);
-    const T rc2b = xsqrt(P.rc_22);
-    for (int sb = s0; sb < s1; sb += WAVE) {
-        const auto Bn = C.bins(P, sb, s1, rc2b);
-        const int nsb = s1 - sb < WAVE ? s1 - sb : WAVE;
-        for (int sk = 0; sk < nsb; ++sk) {
-            int j0, j1;
-            T ox, oy, oz;
-            Bn.get(sk, j0, j1, ox, oy, oz);
-            for (int jb = j0; jb < j1; jb += WAVE) {
-                const int j = jb + lane;
-                bool in = false;
-                T dx = T(0), dy = T(0), dz = T(0);
-                if (j < j1) {
-                    dx = ox - P.lx[j];
-                    dy = oy - P.ly[j];
-                    dz = oz - P.lz[j];
-                    const T r2 = dx * dx + dy * dy + dz * dz;
-                    in = r2 < P.rc_22 && r2 > T(0);
-                }
-                const uint64_t m = ballot64(in);
-                if (m == 0) continue;
-                if (in) {
-                    const int pos = qn + mbcnt64(m);
-                    Q.j[pos] = j;
-                    Q.dx[pos] = dx;
-                    Q.dy[pos] = dy;
-                    Q.dz[pos] = dz;
-                }
-                qn += __popcll(m);
-                if (qn >= WAVE) {
-                    wave_sync();
-                    pair_2b<T, NR>(P, Q.j[lane], Q.dx[lane], Q.dy[lane], Q.dz[lane], y, dyv, rq, rR, rX, C.ti, cni, A);
-                    cnt += 1.0f;
-                    const int rest = qn - WAVE;
-                    int mj = 0;
-                    T mx = T(0), my = T(0), mz = T(0);
-                    if (lane < rest) {
-                        mj = Q.j[WAVE + lane];
-                        mx = Q.dx[WAVE + lane];
-                        my = Q.dy[WAVE + lane];
-                        mz = Q.dz[WAVE + lane];
-                    }
-                    wave_sync();
-                    if (lane < rest) {
-                        Q.j[lane] = mj;
-                        Q.dx[lane] = mx;
-                        Q.dy[lane] = my;
-                        Q.dz[lane] = mz;
-                    }
-                    wave_sync();
-                    qn = rest;
-                }
-            }
-        }
-    }
-    wave_sync();
-    if (lane < qn) {
-        pair_2b<T, NR>(P, Q.j[lane], Q.dx[lane], Q.dy[lane], Q.dz[lane], y, dyv, rq, rR, rX, C.ti, cni, A);
+    walk_pairs<T, true>(P, C, s0, s1, xsqrt(P.rc_22), P.rc_22, Q, [&](int j, T dx, T dy, T dz) {
+        pair_2b<T, NR>(P, j, dx, dy, dz, y, dyv, rq, rR, rX, C.ti, cni, A);
         cnt += 1.0f;
-    }
+    });
     A.fold();
     const double E = wave_total(A.e), G = wave_total(A.g);
     double F[3], V[6];
@@ -547,55 +562,38 @@ __global__ void k_2b_parts(Params<T> P) {
 // ---------------------------------------------------------------------------------------------
 template <class T>
 __global__ __launch_bounds__(BLOCK) void k_chain(Params<T> P) {
-    const int lane = lane_id();
-    const int i = blockIdx.x * WPB + (threadIdx.x >> 6);
+    __shared__ Queue<T> Qs[WPB];
+    const int lane = lane_id(), wv = threadIdx.x >> 6;
+    const int i = blockIdx.x * WPB + wv;
     if (i >= P.n) return;
     Centre<T> C;
     C.load(P, i);
     const T rci = P.rcov[C.ti];
     const T gi = (T)P.g[i];
-    Acc<T> F[3], V[6];
-    for (int k = 0; k < 3; ++k) F[k].init();
-    for (int k = 0; k < 6; ++k) V[k].init();
+    AccN<T, 9> A;  // F (3), virial (6)
+    A.init();
     const int s0 = C.S->st_off[CUT_CN], s1 = s0 + C.S->st_cnt[CUT_CN];
-    for (int sb = s0; sb < s1; sb += WAVE) {
-        const auto Bn = C.bins(P, sb, s1, P.rc_cn);
-        const int nsb = s1 - sb < WAVE ? s1 - sb : WAVE;
-        for (int sk = 0; sk < nsb; ++sk) {
-            int j0, j1;
-            T ox, oy, oz;
-            Bn.get(sk, j0, j1, ox, oy, oz);
-            for (int j = j0 + lane; j < j1; j += WAVE) {
-                const T dx = ox - P.lx[j], dy = oy - P.ly[j], dz = oz - P.lz[j];
-                const T r2 = dx * dx + dy * dy + dz * dz;
-                if (r2 <= P.rc_cn2 && r2 > T(0)) {
-                    const T ir = xrsqrt(r2);
-                    const T rr = rci + P.rcov[P.type_s[j]];
-                    const T ex = xexp(-T(K1) * (rr * ir - T(1)));
-                    const T ope = T(1) + ex;
-                    T dcn = -T(K1) * ex * rr * ir * ir * ir / (ope * ope);
-                    if (P.cn_smooth) {
-                        T c = T(1) / ope;
-                        smooth_cn_term(P.rc_cn, r2 * ir, ir, c, dcn);
-                    }
-                    const T sc = finite_or_zero((gi + (T)P.g[j]) * dcn);
-                    F[0].add(sc * dx);
-                    F[1].add(sc * dy);
-                    F[2].add(sc * dz);
-                    const T h = T(-0.5) * sc;
-                    V[0].add(h * dx * dx);
-                    V[1].add(h * dy * dy);
-                    V[2].add(h * dz * dz);
-                    V[3].add(h * dx * dy);
-                    V[4].add(h * dx * dz);
-                    V[5].add(h * dy * dz);
-                }
-            }
+    walk_pairs<T, false>(P, C, s0, s1, P.rc_cn, P.rc_cn2, Qs[wv], [&](int j, T dx, T dy, T dz) {
+        const T r2 = dx * dx + dy * dy + dz * dz;
+        const T ir = xrsqrt(r2);
+        const T rr = rci + P.rcov[at32(P.type_s, (unsigned)j)];
+        const T ex = xexp(-T(K1) * (rr * ir - T(1)));
+        const T ope = T(1) + ex;
+        T dcn = -T(K1) * ex * rr * ir * ir * ir / (ope * ope);
+        if (P.cn_smooth) {
+            T c = T(1) / ope;
+            smooth_cn_term(P.rc_cn, r2 * ir, ir, c, dcn);
         }
-    }
+        const T sc = finite_or_zero((gi + (T)at32(P.g, (unsigned)j)) * dcn);
+        const T h = T(-0.5) * sc;
+        const T hx = h * dx, hy = h * dy, hz = h * dz;
+        const T v[9] = {sc * dx, sc * dy, sc * dz, hx * dx, hy * dy, hz * dz, hx * dy, hx * dz, hy * dz};
+        A.add(v);
+    });
+    if constexpr (sizeof(T) == sizeof(float)) A.fold();
     double Fd[3], Vd[6];
-    for (int k = 0; k < 3; ++k) Fd[k] = wave_total(F[k]);
-    for (int k = 0; k < 6; ++k) Vd[k] = wave_total(V[k]);
+    for (int k = 0; k < 3; ++k) Fd[k] = wave_total(A.t[k]);
+    for (int k = 0; k < 6; ++k) Vd[k] = wave_total(A.t[3 + k]);
     if (lane == 0) {
         const double iv = C.S->inv_vol;
         for (int k = 0; k < 3; ++k) P.f[3 * i + k] += Fd[k];
