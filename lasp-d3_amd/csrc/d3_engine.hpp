@@ -1,4 +1,4 @@
-// Definitions shared by the host engine (d3_engine.cpp) and the HIP kernels (d3_kernels.hip).
+// Definitions shared by the host engine (d3_host.cpp) and the HIP kernels (d3_*.hip).
 //
 // HBM layout (one handle = one batch of periodic cells):
 //   inputs (caller order): pos_in[3N] double bohr, type_in[N] u16, sys_in[N] i32
