@@ -83,7 +83,8 @@ def write_cache(path, data):
 def test_every_three_body_variant(tmp_path):
     """Each three-body kernel variant (k-side sums in LDS or not, damping rows or not), forced
     through the tuning cache, gives the same result: FP64 at the parity bar, FP32 within its error
-    budget of the FP64 result."""
+    budget of the FP64 result, and every FP32 variant the same FP32 tile sums as the first (the
+    results then differ only by the order of FP64 sums, ~1e-16)."""
     s = systems.triclinic_mixed(n=24, seed=9)
     kw = dict(cutoff=s["cutoff"], cn_cutoff=s["cn_cutoff"], atm_cutoff=s["atm"], damping=1, functional=2)
     cache = str(tmp_path / "tune.json")
@@ -98,6 +99,7 @@ def test_every_three_body_variant(tmp_path):
     data = json.load(open(cache))
     assert len(data) == 2
     os.environ["D3_TUNE_CACHE"] = cache
+    ref32 = None
     try:
         for prec in (d3_lib.D3X_FP64, d3_lib.D3X_FP32):
             for v in range(4):
@@ -116,6 +118,12 @@ def test_every_three_body_variant(tmp_path):
                 fp64 = prec == d3_lib.D3X_FP64
                 assert abs(e - ref[0]) <= (1e-12 if fp64 else 2e-6) * abs(ref[0]), (prec, v)
                 assert np.max(np.abs(f - ref[1])) <= (1e-9 if fp64 else 2e-5 * np.max(np.abs(ref[1]))), (prec, v)
+                if not fp64:
+                    if ref32 is None:
+                        ref32 = (e, f, st)
+                    assert abs(e - ref32[0]) <= 1e-13 * abs(ref32[0]), v
+                    assert np.max(np.abs(f - ref32[1])) <= 1e-12 * np.max(np.abs(ref32[1])), v
+                    assert np.max(np.abs(st - ref32[2])) <= 1e-12 * np.max(np.abs(ref32[2])), v
     finally:
         del os.environ["D3_TUNE_CACHE"]
 
@@ -136,3 +144,26 @@ def test_default_bin_width_follows_cn_cutoff():
         w, _ = h.tuning()
         h.close()
         assert w == pytest.approx(0.375 * rccn), (rc2, rccn, w)
+
+
+@pytest.mark.parametrize("system", ["si64", "water_small"])
+def test_fp32_results_do_not_depend_on_work_items(system):
+    """The three-body kernel's work items per centre follow the number of persistent blocks; their
+    k-tile ranges start on even tiles, so the two-tile passes, and every FP32 tile sum, are the
+    same for any number of them: FP32 results agree to the order of FP64 sums."""
+    s = systems.si64(seed=31) if system == "si64" else systems.water_small(n_mol=32, seed=3)
+    ref = None
+    for b in (0, 64, 300, 4096):
+        h = d3_lib.Handle(np.unique(s["Z"]), len(s["Z"]), cutoff=60.0, cn_cutoff=30.0, atm_cutoff=30.0, damping=1,
+                          functional=0, precision=d3_lib.D3X_FP32)
+        if b:
+            h.set_tuning(0.0, b)
+        h.set_atoms(s["pos"], s["Z"])
+        h.set_cell(s["cell"])
+        out = h.compute()
+        h.close()
+        if ref is None:
+            ref = out
+            continue
+        assert abs(out[0] - ref[0]) <= 1e-13 * abs(ref[0]), b
+        assert np.max(np.abs(out[1] - ref[1])) <= 1e-12 * np.max(np.abs(ref[1])), b
