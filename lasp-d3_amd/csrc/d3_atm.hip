@@ -219,7 +219,7 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
     // per-type-pair damping terms, FP32 (8/3) log2 R0^2, FP64 R0^2 (in LDS without DR)
     auto rt_of = [&](int idx) -> T {
         const T r0sq = P.r0[idx] * P.r0[idx];
-        return F32 ? T((8.0f / 3.0f) * log2f((float)r0sq)) : r0sq;
+        return F32 ? T((8.0f / 3.0f) * __builtin_amdgcn_logf((float)r0sq)) : r0sq;
     };
     if constexpr (!DR)
         for (int idx = lane; idx < ne * ne; idx += AW) S.Rt[idx] = rt_of(idx);
@@ -394,13 +394,15 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
         // Pair (i, p) terms of a forward neighbour p from its offset, type and weights, the same
         // expressions for the k and the j side: r^2, 1/r^2, 1/r, sqrt C6_ip, damping term
         // (FP32: (8/3) log2(R0^2 / r^2), FP64: R0^2 / r^2).
+        // FP32 takes the hardware reciprocal, reciprocal square root and log2 (1 ulp, as the pair
+        // steps do), FP64 the correctly rounded forms.
         auto pair_terms = [&](T dx, T dy, T dz, int tp, int slot, T &a, T &ia, T &ra, T &sc6, T &lam) {
             a = dx * dx + dy * dy + dz * dz;
-            ia = T(1) / a;
-            ra = T(1) / xsqrt(a);
-            sc6 = LT[F_SC6 * cap + slot];  // from phase 2
+            ia = prcp(a);
+            ra = prsqrt(a);
+            sc6 = at32(LT + F_SC6 * cap, (unsigned)slot);  // from phase 2
             const T r0 = P.r0[ti * ne + tp];
-            lam = F32 ? T((8.0f / 3.0f) * log2f((float)(r0 * r0 * ia))) : r0 * r0 * ia;
+            lam = F32 ? T((8.0f / 3.0f) * __builtin_amdgcn_logf((float)(r0 * r0 * ia))) : r0 * r0 * ia;
         };
         // One k tile: lane l owns neighbour k = 64 kt + l, its pair terms with the centre and its
         // tile sums (per reference b for the CN term: Gb[b] = sum (e / 2 c6) y_b(j), contracted
@@ -425,16 +427,16 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
             K.t = 0;
             for (int a = 0; a < NR; ++a) K.w[a] = K.dw[a] = T(0);
             if (k < nf) {
-                const int jk = Lj[k];
-                K.x = LT[F_DX * cap + k];
-                K.y = LT[F_DY * cap + k];
-                K.z = LT[F_DZ * cap + k];
-                K.t = Lt[k];
+                const unsigned jk = (unsigned)at32(Lj, (unsigned)k);
+                K.x = at32(LT + F_DX * cap, (unsigned)k);
+                K.y = at32(LT + F_DY * cap, (unsigned)k);
+                K.z = at32(LT + F_DZ * cap, (unsigned)k);
+                K.t = at32(Lt, (unsigned)k);
                 for (int a = 0; a < NR; ++a) {
-                    K.w[a] = P.w[a * P.n + jk];
-                    K.dw[a] = P.dw[a * P.n + jk];
+                    K.w[a] = at32(P.w + (size_t)a * P.n, jk);
+                    K.dw[a] = at32(P.dw + (size_t)a * P.n, jk);
                 }
-                if constexpr (NR == 0) K.cn = P.cn[jk];
+                if constexpr (NR == 0) K.cn = at32(P.cn, jk);
                 T sc6, lam;
                 pair_terms(K.x, K.y, K.z, K.t, k, K.b2, K.ib, K.rb, sc6, lam);
                 K.sc6h = T(-0.5) * sc6;  // -sqrt(C6_ik) / 2: C9 / 2 below (pairwise C6 only)
@@ -493,7 +495,7 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                     Khi[q * AW + lane] = s;
                 }
             } else if (k < nf) {
-                for (int q = 0; q < A_NF; ++q) unsafeAtomicAdd(&La[q * cap + k], (double)v[q]);
+                for (int q = 0; q < A_NF; ++q) unsafeAtomicAdd(ptr32(La + q * cap, (unsigned)k), (double)v[q]);
             }
         };
         auto close_k = [&](int kt, int kl) __attribute__((always_inline)) {
@@ -502,7 +504,7 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                 if (k < nf) {
                     const T *Khi = S.Khi + kl * 2 * A_NF * AW, *Klo = Khi + A_NF * AW;
                     for (int q = 0; q < A_NF; ++q)
-                        unsafeAtomicAdd(&La[q * cap + k], (double)Khi[q * AW + lane] + (double)Klo[q * AW + lane]);
+                        unsafeAtomicAdd(ptr32(La + q * cap, (unsigned)k), (double)Khi[q * AW + lane] + (double)Klo[q * AW + lane]);
                 }
         };
 
@@ -778,16 +780,16 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                     T jcn = T(0);  // pairwise C6 only
                     for (int q = 0; q < NR; ++q) wj[q] = dwj[q] = T(0);
                     if (jval) {
-                        jxs = LT[F_DX * cap + pj];
-                        jys = LT[F_DY * cap + pj];
-                        jzs = LT[F_DZ * cap + pj];
-                        const int jj = Lj[pj];
-                        tj = Lt[pj];
+                        jxs = at32(LT + F_DX * cap, (unsigned)pj);
+                        jys = at32(LT + F_DY * cap, (unsigned)pj);
+                        jzs = at32(LT + F_DZ * cap, (unsigned)pj);
+                        const unsigned jj = (unsigned)at32(Lj, (unsigned)pj);
+                        tj = at32(Lt, (unsigned)pj);
                         for (int q = 0; q < NR; ++q) {
-                            wj[q] = P.w[q * P.n + jj];
-                            dwj[q] = P.dw[q * P.n + jj];
+                            wj[q] = at32(P.w + (size_t)q * P.n, jj);
+                            dwj[q] = at32(P.dw + (size_t)q * P.n, jj);
                         }
-                        if constexpr (NR == 0) jcn = P.cn[jj];
+                        if constexpr (NR == 0) jcn = at32(P.cn, jj);
                         pair_terms(jxs, jys, jzs, tj, pj, a, ia, ra, sc6, lam);
                     }
                     S.J0[lane] = T2<T>{jxs, jys};
@@ -871,8 +873,8 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                 if (jval)
                     for (int q = 0; q < 3; ++q) {
                         const T2<T> pq = S.Jp[q * AW + lane];
-                        unsafeAtomicAdd(&La[(2 * q) * cap + pj], (double)pq.x);
-                        unsafeAtomicAdd(&La[(2 * q + 1) * cap + pj], (double)pq.y);
+                        unsafeAtomicAdd(ptr32(La + (2 * q) * cap, (unsigned)pj), (double)pq.x);
+                        unsafeAtomicAdd(ptr32(La + (2 * q + 1) * cap, (unsigned)pj), (double)pq.y);
                     }
             }
             close_k(kb, 0);

@@ -46,6 +46,12 @@ __device__ __forceinline__ U at32(const U *base, unsigned i) {
     return *reinterpret_cast<const U *>(reinterpret_cast<const char *>(base) + i * (unsigned)sizeof(U));
 }
 
+// Address of element i of such an array, for stores and atomics (the same 32-bit offset form).
+template <class U>
+__device__ __forceinline__ U *ptr32(U *base, unsigned i) {
+    return reinterpret_cast<U *>(reinterpret_cast<char *>(base) + i * (unsigned)sizeof(U));
+}
+
 __device__ __forceinline__ int wrapi(int x, int n) {
     int r = x % n;
     return r < 0 ? r + n : r;
