@@ -237,7 +237,7 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
         if (i >= P.n) break;
         Centre<T> C;
         C.load(P, i);
-        const int ti = C.ti;
+        const int ti = ONE ? 0 : C.ti;
         const T cni = NR == 0 ? P.cn[i] : T(0);  // pairwise C6 only
         __syncthreads();  // previous centre done with LDS and scratch
         if constexpr (NR > 0)
@@ -344,7 +344,7 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
             const int p = t * AW + lane;
             float bb[6] = {3e38f, -3e38f, 3e38f, -3e38f, 3e38f, -3e38f};
             if (p < nf) {
-                const int j = Lj[p], tj = P.type_s[j];
+                const int j = Lj[p], tj = ONE ? 0 : P.type_s[j];
                 T c6 = T(0), dci = T(0), dcj = T(0);
                 if constexpr (NR == 0) {
                     c6_pairwise(P.c6g, ne, P.nref, ti, tj, cni, P.cn[j], c6, dci, dcj);
@@ -431,7 +431,7 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                 K.x = at32(LT + F_DX * cap, (unsigned)k);
                 K.y = at32(LT + F_DY * cap, (unsigned)k);
                 K.z = at32(LT + F_DZ * cap, (unsigned)k);
-                K.t = at32(Lt, (unsigned)k);
+                K.t = ONE ? 0 : at32(Lt, (unsigned)k);  // one element type: every table index is uniform
                 for (int a = 0; a < NR; ++a) {
                     K.w[a] = at32(P.w + (size_t)a * P.n, jk);
                     K.dw[a] = at32(P.dw + (size_t)a * P.n, jk);
@@ -784,7 +784,7 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                         jys = at32(LT + F_DY * cap, (unsigned)pj);
                         jzs = at32(LT + F_DZ * cap, (unsigned)pj);
                         const unsigned jj = (unsigned)at32(Lj, (unsigned)pj);
-                        tj = at32(Lt, (unsigned)pj);
+                        tj = ONE ? 0 : at32(Lt, (unsigned)pj);
                         for (int q = 0; q < NR; ++q) {
                             wj[q] = at32(P.w + (size_t)q * P.n, jj);
                             dwj[q] = at32(P.dw + (size_t)q * P.n, jj);
