@@ -1,4 +1,4 @@
-"""GPU checks at the benchmark workloads' full sizes (BASELINE.json configurations):
+"""GPU checks at the benchmark workloads' full sizes:
 
 * the 512-atom Si diamond cell (the oracle's own CPU case) against the FP64 oracle at the parity bar;
 * the 12,288-atom water box: translation and permutation invariance, sum of forces, stress

@@ -1,6 +1,6 @@
 """GPU parity: the HIP path through the C ABI against the FP64 oracle on the same inputs.
 
-Bars (BASELINE.json north star): plain FP64 path within 1e-12 relative energy and 1e-9
+Bars: plain FP64 path within 1e-12 relative energy and 1e-9
 eV/Angstrom forces of the oracle; stress within 1e-12 relative to its largest component (plus
 1e-12 eV/A^3 absolute); CN and dE/dCN to 1e-10 relative.  The FP32 path is held to 2e-6 relative
 energy and 2e-5 of the largest force component; its measured error is printed per system.
