@@ -16,7 +16,7 @@ constexpr int WPB = BLOCK / WAVE;
 constexpr int QCAP = 2 * WAVE;  // per-wave compaction queue (entries)
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (WAVE - 1); }
-__device__ __forceinline__ uint64_t ballot64(bool p) { return __ballot(p); }
+__device__ __forceinline__ uint64_t ballot64(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 __device__ __forceinline__ int mbcnt64(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
@@ -50,6 +50,15 @@ __device__ __forceinline__ U at32(const U *base, unsigned i) {
 template <class U>
 __device__ __forceinline__ U *ptr32(U *base, unsigned i) {
     return reinterpret_cast<U *>(reinterpret_cast<char *>(base) + i * (unsigned)sizeof(U));
+}
+// A wave-uniform pointer made opaque to the optimiser (it stays in SGPRs as computed), so that
+// at32(p, j) keeps the scalar-base + 32-bit-offset form instead of being re-associated into 64-bit
+// vector address arithmetic when p is itself a sum (a row of a row-major array).
+template <class U>
+__device__ __forceinline__ const U *sbase(const U *p) {
+    auto g = (const __attribute__((address_space(1))) U *)p;  // global memory
+    asm volatile("" : "+s"(g));
+    return (const U *)g;
 }
 
 __device__ __forceinline__ int wrapi(int x, int n) {

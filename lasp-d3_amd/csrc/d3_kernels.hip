@@ -195,8 +195,11 @@ __device__ __forceinline__ void walk_pairs(const Params<T> &P, const Centre<T> &
                 const unsigned jj = min(j, (unsigned)(j1 - 1));
                 const T dx = ox - at32(P.lx, jj), dy = oy - at32(P.ly, jj), dz = oz - at32(P.lz, jj);
                 const T r2 = dx * dx + dy * dy + dz * dz;
-                const bool in = j < (unsigned)j1 && (STRICT ? r2 < rc2 : r2 <= rc2) && r2 > T(0);
-                const uint64_t m = ballot64(in);
+                // one ballot per compare: each folds into its v_cmp (a ballot of the combined
+                // condition costs a select and a compare more per candidate step)
+                const bool c0 = j < (unsigned)j1, c1 = STRICT ? r2 < rc2 : r2 <= rc2, c2 = r2 > T(0);
+                const bool in = c0 && c1 && c2;
+                const uint64_t m = ballot64(c0) & ballot64(c1) & ballot64(c2);
                 if (m == 0) continue;
                 if (in) {
                     const int pos = (qt + mbcnt64(m)) & (QCAP - 1);
@@ -396,10 +399,13 @@ __device__ __forceinline__ double prsqrt(double x) { return rsqrt(x); }
 // NR = the handle's reference count: the weight gathers of a pair are unrolled and issued
 // together.  NR = 0: tables that do not factorise per element; C6 and dC6/dCN_i are interpolated
 // per pair from the pair's reference entries (c6_pairwise) at the CNs of i (cni) and j.
-template <class T, int NR>
-__device__ __forceinline__ void pair_2b(const Params<T> &P, int j, T dx, T dy, T dz, const T *y, const T *dy_,
-                                        const T *rq, const T *rR, const T *rX, int ti, T cni, Acc2b<T> &A) {
-    const int tj = at32(P.type_s, (unsigned)j);
+// ONE: a handle of one element type -- the partner's type is 0 (not loaded) and the centre's C6
+// row and damping constants are registers, not LDS.
+template <class T, int NR, bool ONE>
+__device__ __forceinline__ void pair_2b(const Params<T> &P, const T *const *wr, int j, T dx, T dy, T dz, const T *y,
+                                        const T *dy_, const T *rq, const T *rR, const T *rX, int ti, T cni,
+                                        Acc2b<T> &A) {
+    const int tj = ONE ? 0 : at32(P.type_s, (unsigned)j);
     T c6 = T(0), dc6 = T(0);
     if constexpr (NR == 0) {
         T dc6j;
@@ -407,7 +413,7 @@ __device__ __forceinline__ void pair_2b(const Params<T> &P, int j, T dx, T dy, T
     } else {
         T wj[NR];
 #pragma unroll
-        for (int b = 0; b < NR; ++b) wj[b] = at32(P.w + (size_t)b * P.n, (unsigned)j);
+        for (int b = 0; b < NR; ++b) wj[b] = at32(wr[b], (unsigned)j);
 #pragma unroll
         for (int b = 0; b < NR; ++b) {
             c6 += y[tj * NR + b] * wj[b];
@@ -415,7 +421,9 @@ __device__ __forceinline__ void pair_2b(const Params<T> &P, int j, T dx, T dy, T
         }
     }
     const T r2 = dx * dx + dy * dy + dz * dz;
-    T e, g, phi;  // E+_i share (half the pair), dE+/dCN_i, (1/r) dE+/dr
+    // e = E+ of the whole pair (the centre's share is half of it: the sums are halved once, in
+    // k_2b), dE+/dCN_i, phi = (1/r) dE+/dr; the virial sums collect phi d_a d_b (times -1/2 there)
+    T e, g, phi;
     if (P.damping == 0) {
         const T q = rq[tj], R = rR[tj];
         const T c8 = T(3) * c6 * q, dc8 = T(3) * dc6 * q;
@@ -428,26 +436,31 @@ __device__ __forceinline__ void pair_2b(const Params<T> &P, int j, T dx, T dy, T
         const T b8_2 = b8 * b8, b8_4 = b8_2 * b8_2, b8_8 = b8_4 * b8_4, b8_16 = b8_8 * b8_8;
         const T f6 = finite_or_zero(prcp(T(1) + T(6) * b6_14)), f8 = finite_or_zero(prcp(T(1) + T(6) * b8_16));
         const T df6 = finite_or_zero(T(84) * f6 * f6 * b6_14 * ir), df8 = finite_or_zero(T(96) * f8 * f8 * b8_16 * ir);
-        e = T(0.5) * (P.s6 * c6 * f6 * ir6 + P.s8 * c8 * f8 * ir8);
+        e = P.s6 * c6 * f6 * ir6 + P.s8 * c8 * f8 * ir8;
         g = P.s6 * f6 * dc6 * ir6 + P.s8 * f8 * dc8 * ir8;
         phi = (P.s6 * c6 * (df6 - T(6) * f6 * ir) * ir6 + P.s8 * c8 * (df8 - T(8) * f8 * ir) * ir8) * ir;
     } else {
-        // BJ: f_n r^-n = 1 / (r^n + a^n), so with rq = 3 s8 r2r4_A r2r4_B, rR = a^6, rX = a^8:
-        //   t6 = s6 / (r^6 + a^6), t8 = 3 s8 q / (r^8 + a^8)
-        //   E+ = C6 (t6 + t8),  dE+/dCN = dC6 (t6 + t8),
-        //   (1/r) dE+/dr = -C6 (6 t6 r^4 / (r^6 + a^6) + 8 t8 r^6 / (r^8 + a^8))
-        // (no 1/r anywhere; a denominator that underflows to 0 contributes nothing)
-        const T r4 = r2 * r2, r6 = r4 * r2, r8 = r4 * r4;
-        const T d6 = finite_or_zero(prcp(r6 + rR[tj])), d8 = finite_or_zero(prcp(r8 + rX[tj]));
-        const T t6 = P.s6 * d6, t8 = rq[tj] * d8;
-        const T tt = t6 + t8;
-        e = T(0.5) * c6 * tt;
+        // BJ: f_n r^-n = 1 / (r^n + a^n), so with rq = 3 s8 r2r4_A r2r4_B, rR = a^6, rX = a^8,
+        // d6 = 1 / (r^6 + a^6), d8 = 1 / (r^8 + a^8), t8 = rq d8:
+        //   E+ = C6 (s6 d6 + t8),  dE+/dCN = dC6 (s6 d6 + t8),
+        //   (1/r) dE+/dr = -C6 (6 s6 d6^2 r^4 + 8 t8 d8 r^6) = -C6 r^4 (8 t8 d8 r^2 + 6 s6 d6^2)
+        // (no 1/r anywhere; a denominator below the smallest normal number -- it can only have
+        // underflowed -- contributes nothing; tested before the reciprocal, so the select does
+        // not wait for it)
+        const T r4 = r2 * r2;
+        const T n6 = r4 * r2 + rR[tj], n8 = r4 * r4 + rX[tj];
+        const T tiny = sizeof(T) == sizeof(float) ? T(FLT_MIN) : T(DBL_MIN);
+        const T d6 = n6 >= tiny ? prcp(n6) : T(0), d8 = n8 >= tiny ? prcp(n8) : T(0);
+        const T t8 = rq[tj] * d8;
+        const T tt = P.s6 * d6 + t8;
+        e = c6 * tt;
         g = dc6 * tt;
-        phi = -c6 * (T(6) * t6 * d6 * r4 + T(8) * t8 * d8 * r6);
+        const T u8 = T(8) * t8 * d8, u6 = (T(6) * P.s6) * d6 * d6;
+        phi = -(c6 * r4) * (u8 * r2 + u6);
     }
-    const T h = T(-0.5) * phi;
-    const T vv[6] = {h * dx * dx, h * dy * dy, h * dz * dz, h * dx * dy, h * dx * dz, h * dy * dz};
-    A.add(e, g, phi * dx, phi * dy, phi * dz, vv);
+    const T fx = phi * dx, fy = phi * dy, fz = phi * dz;
+    const T vv[6] = {fx * dx, fy * dy, fz * dz, fx * dy, fx * dz, fy * dz};
+    A.add(e, g, fx, fy, fz, vv);
 }
 
 template <class T>
@@ -455,7 +468,7 @@ __host__ __device__ constexpr size_t twobody_wave_lds(int ne, int nref) {
     return ((sizeof(Queue<T>) + (size_t)(2 * ne * nref + 3 * ne) * sizeof(T)) + 15) / 16 * 16;
 }
 
-template <class T, int NR>
+template <class T, int NR, bool ONE>
 __global__ __launch_bounds__(BLOCK) void k_2b(Params<T> P) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int wv = threadIdx.x >> 6, lane = lane_id();
@@ -466,11 +479,41 @@ __global__ __launch_bounds__(BLOCK) void k_2b(Params<T> P) {
     Queue<T> &Q = *reinterpret_cast<Queue<T> *>(base);
     T *y = reinterpret_cast<T *>(base + sizeof(Queue<T>));
     T *dyv = y + ynr, *rq = dyv + ynr, *rR = rq + P.ne, *rX = rR + P.ne;
+    T y1[ONE ? NR : 1], dy1[ONE ? NR : 1], rq1[1], rR1[1], rX1[1];  // ONE: the same, in registers
 
     Centre<T> C;
     C.load(P, i);
     const T cni = NR == 0 ? P.cn[i] : T(0);
-    if constexpr (NR > 0) {
+    if constexpr (ONE) {
+        static_assert(NR > 0, "one element type: factorised rows only");
+        T wi[NR], dwi[NR];
+        for (int a = 0; a < NR; ++a) {
+            wi[a] = P.w[a * P.n + i];
+            dwi[a] = P.dw[a * P.n + i];
+        }
+#pragma unroll
+        for (int bb = 0; bb < NR; ++bb) {
+            T sy = T(0), sdy = T(0);
+            for (int a = 0; a < NR; ++a) {
+                const T c = P.c6[a * NR + bb];
+                sy += c * wi[a];
+                sdy += c * dwi[a];
+            }
+            y1[bb] = sy;
+            dy1[bb] = sdy;
+        }
+        const T q = P.q4[0], R = P.rdamp[0];
+        if (P.damping == 0) {
+            rq1[0] = q;
+            rR1[0] = R;
+            rX1[0] = T(0);
+        } else {
+            const T a2 = R * R, a6 = a2 * a2 * a2;
+            rq1[0] = T(3) * P.s8 * q;
+            rR1[0] = a6;
+            rX1[0] = a6 * a2;
+        }
+    } else if constexpr (NR > 0) {
         T wi[NR], dwi[NR];
         for (int a = 0; a < NR; ++a) {
             wi[a] = P.w[a * P.n + i];
@@ -488,16 +531,18 @@ __global__ __launch_bounds__(BLOCK) void k_2b(Params<T> P) {
             dyv[idx] = sdy;
         }
     }
-    for (int B = lane; B < P.ne; B += WAVE) {
-        const T q = P.q4[C.ti * P.ne + B], R = P.rdamp[C.ti * P.ne + B];
-        if (P.damping == 0) {  // zero damping: q and R0
-            rq[B] = q;
-            rR[B] = R;
-        } else {  // BJ: 3 s8 q, a^6, a^8
-            const T a2 = R * R, a6 = a2 * a2 * a2;
-            rq[B] = T(3) * P.s8 * q;
-            rR[B] = a6;
-            rX[B] = a6 * a2;
+    if constexpr (!ONE) {
+        for (int B = lane; B < P.ne; B += WAVE) {
+            const T q = P.q4[C.ti * P.ne + B], R = P.rdamp[C.ti * P.ne + B];
+            if (P.damping == 0) {  // zero damping: q and R0
+                rq[B] = q;
+                rR[B] = R;
+            } else {  // BJ: 3 s8 q, a^6, a^8
+                const T a2 = R * R, a6 = a2 * a2 * a2;
+                rq[B] = T(3) * P.s8 * q;
+                rR[B] = a6;
+                rX[B] = a6 * a2;
+            }
         }
     }
     wave_sync();
@@ -508,15 +553,21 @@ __global__ __launch_bounds__(BLOCK) void k_2b(Params<T> P) {
     // this wave's share of the stencil (all of it with one part per centre)
     const int sa = C.S->st_off[CUT_2B], sz = C.S->st_cnt[CUT_2B], per = (sz + nparts - 1) / nparts;
     const int s0 = sa + min(sz, part * per), s1 = sa + min(sz, (part + 1) * per);
+    const T *wr[NR > 0 ? NR : 1];  // rows of the weights, one per reference
+    for (int b = 0; b < NR; ++b) wr[b] = sbase(P.w + (size_t)b * P.n);
     walk_pairs<T, true>(P, C, s0, s1, xsqrt(P.rc_22), P.rc_22, Q, [&](int j, T dx, T dy, T dz) {
-        pair_2b<T, NR>(P, j, dx, dy, dz, y, dyv, rq, rR, rX, C.ti, cni, A);
+        if constexpr (ONE)
+            pair_2b<T, NR, true>(P, wr, j, dx, dy, dz, y1, dy1, rq1, rR1, rX1, 0, cni, A);
+        else
+            pair_2b<T, NR, false>(P, wr, j, dx, dy, dz, y, dyv, rq, rR, rX, C.ti, cni, A);
         cnt += 1.0f;
     });
     A.fold();
-    const double E = wave_total(A.e), G = wave_total(A.g);
+    // the centre's half of each pair's energy and virial (scaling by 1/2 is exact)
+    const double E = 0.5 * wave_total(A.e), G = wave_total(A.g);
     double F[3], V[6];
     for (int k = 0; k < 3; ++k) F[k] = wave_total(A.f[k]);
-    for (int k = 0; k < 6; ++k) V[k] = wave_total(A.v[k]);
+    for (int k = 0; k < 6; ++k) V[k] = -0.5 * wave_total(A.v[k]);
     cnt = wave_sumf(cnt);
     if (lane == 0) {
         const double iv = C.S->inv_vol;
@@ -627,10 +678,10 @@ void Launch<T>::cn(const Params<T> &P, hipStream_t s) {
     hipLaunchKernelGGL(k_cn<T>, dim3((P.n + WPB - 1) / WPB), dim3(BLOCK), 0, s, P);
 }
 
-template <class T, int... NR>
+template <class T, bool ONE, int... NR>
 const void *twobody_pick(int nref, std::integer_sequence<int, NR...>) {
     const void *fn = nullptr;
-    ((fn = (nref == NR + 1) ? reinterpret_cast<const void *>(&k_2b<T, NR + 1>) : fn), ...);
+    ((fn = (nref == NR + 1) ? reinterpret_cast<const void *>(&k_2b<T, NR + 1, ONE>) : fn), ...);
     return fn;
 }
 
@@ -638,8 +689,10 @@ template <class T>
 void Launch<T>::twobody(const Params<T> &P, hipStream_t s) {
     if (P.n == 0) return;
     const size_t lds = WPB * twobody_wave_lds<T>(P.ne, P.c6_pairwise ? 0 : P.nref);
-    const void *fn = P.c6_pairwise ? reinterpret_cast<const void *>(&k_2b<T, 0>)
-                                   : twobody_pick<T>(P.nref, std::make_integer_sequence<int, MAX_NREF>{});
+    const auto seq = std::make_integer_sequence<int, MAX_NREF>{};
+    const void *fn = P.c6_pairwise ? reinterpret_cast<const void *>(&k_2b<T, 0, false>)
+                     : P.ne == 1   ? twobody_pick<T, true>(P.nref, seq)
+                                   : twobody_pick<T, false>(P.nref, seq);
     if (!fn) return;  // not reached: a handle's reference count is 1..MAX_NREF
     Params<T> arg = P;
     void *args[] = {&arg};
