@@ -226,6 +226,13 @@ __device__ __forceinline__ void walk_pairs(const Params<T> &P, const Centre<T> &
     }
 }
 
+// Reciprocal and reciprocal square root of the pair math: the FP32 hardware instructions
+// (1 ulp), correctly rounded FP64.
+__device__ __forceinline__ float prcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double prcp(double x) { return 1.0 / x; }
+__device__ __forceinline__ float prsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ double prsqrt(double x) { return rsqrt(x); }
+
 // Per-lane sums of N terms: plain partial sums over at most FOLD pairs, folded into the
 // compensated totals (two-sum in FP32) -- one compensated add per FOLD pairs instead of per pair.
 template <class T, int N>
@@ -275,22 +282,24 @@ __global__ __launch_bounds__(BLOCK) void k_cn(Params<T> P) {
     Centre<T> C;
     C.load(P, i);
     const T rci = P.rcov[C.ti];
-    Acc<T> acc;
+    AccN<T, 1> acc;
     acc.init();
     float cnt = 0.0f;
     const int s0 = C.S->st_off[CUT_CN], s1 = s0 + C.S->st_cnt[CUT_CN];
     walk_pairs<T, false>(P, C, s0, s1, P.rc_cn, P.rc_cn2, Qs[wv], [&](int j, T dx, T dy, T dz) {
-        const T r = xsqrt(dx * dx + dy * dy + dz * dz);
-        const T ex = xexp(-T(K1) * ((rci + P.rcov[at32(P.type_s, (unsigned)j)]) / r - T(1)));
-        T c = T(1) / (T(1) + ex);
+        const T r2 = dx * dx + dy * dy + dz * dz;
+        const T ir = prsqrt(r2);
+        const T ex = xexp(-T(K1) * ((rci + P.rcov[at32(P.type_s, (unsigned)j)]) * ir - T(1)));
+        T c[1] = {prcp(T(1) + ex)};
         if (P.cn_smooth) {
             T unused = T(0);
-            smooth_cn_term(P.rc_cn, r, T(1) / r, c, unused);
+            smooth_cn_term(P.rc_cn, r2 * ir, ir, c[0], unused);
         }
         acc.add(c);
         cnt += 1.0f;
     });
-    const double cnd = wave_total(acc);
+    if constexpr (sizeof(T) == sizeof(float)) acc.fold();
+    const double cnd = wave_total(acc.t[0]);
     cnt = wave_sumf(cnt);
     if (lane == 0) {
         const T cn = (T)cnd;
@@ -388,13 +397,6 @@ struct Acc2b {
         }
     }
 };
-
-// Reciprocal and reciprocal square root of the pair math: the FP32 hardware instructions
-// (1 ulp), correctly rounded FP64.
-__device__ __forceinline__ float prcp(float x) { return __builtin_amdgcn_rcpf(x); }
-__device__ __forceinline__ double prcp(double x) { return 1.0 / x; }
-__device__ __forceinline__ float prsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
-__device__ __forceinline__ double prsqrt(double x) { return rsqrt(x); }
 
 // NR = the handle's reference count: the weight gathers of a pair are unrolled and issued
 // together.  NR = 0: tables that do not factorise per element; C6 and dC6/dCN_i are interpolated
@@ -626,14 +628,14 @@ __global__ __launch_bounds__(BLOCK) void k_chain(Params<T> P) {
     const int s0 = C.S->st_off[CUT_CN], s1 = s0 + C.S->st_cnt[CUT_CN];
     walk_pairs<T, false>(P, C, s0, s1, P.rc_cn, P.rc_cn2, Qs[wv], [&](int j, T dx, T dy, T dz) {
         const T r2 = dx * dx + dy * dy + dz * dz;
-        const T ir = xrsqrt(r2);
+        const T ir = prsqrt(r2);
         const T rr = rci + P.rcov[at32(P.type_s, (unsigned)j)];
         const T ex = xexp(-T(K1) * (rr * ir - T(1)));
-        const T ope = T(1) + ex;
-        T dcn = -T(K1) * ex * rr * ir * ir * ir / (ope * ope);
+        const T c = prcp(T(1) + ex);
+        T dcn = -T(K1) * ex * rr * ir * ir * ir * c * c;
         if (P.cn_smooth) {
-            T c = T(1) / ope;
-            smooth_cn_term(P.rc_cn, r2 * ir, ir, c, dcn);
+            T cs = c;
+            smooth_cn_term(P.rc_cn, r2 * ir, ir, cs, dcn);
         }
         const T sc = finite_or_zero((gi + (T)at32(P.g, (unsigned)j)) * dcn);
         const T h = T(-0.5) * sc;
