@@ -273,7 +273,8 @@ struct AccN {
 // With CN references that depend only on (element, reference) -- checked at handle init -- the
 // reference's 25-term Z/W of src/d3_kernel.cu:709-800 equals w_A^T C6ref w_B exactly.
 // ---------------------------------------------------------------------------------------------
-template <class T>
+// ONE: a handle of one element type, Rcov_i + Rcov_j = 2 Rcov_i (no per-pair gathers).
+template <class T, bool ONE>
 __global__ __launch_bounds__(BLOCK) void k_cn(Params<T> P) {
     __shared__ Queue<T> Qs[WPB];
     const int lane = lane_id(), wv = threadIdx.x >> 6;
@@ -289,7 +290,8 @@ __global__ __launch_bounds__(BLOCK) void k_cn(Params<T> P) {
     walk_pairs<T, false>(P, C, s0, s1, P.rc_cn, P.rc_cn2, Qs[wv], [&](int j, T dx, T dy, T dz) {
         const T r2 = dx * dx + dy * dy + dz * dz;
         const T ir = prsqrt(r2);
-        const T ex = xexp(-T(K1) * ((rci + P.rcov[at32(P.type_s, (unsigned)j)]) * ir - T(1)));
+        const T rr = ONE ? rci + rci : rci + P.rcov[at32(P.type_s, (unsigned)j)];
+        const T ex = xexp(-T(K1) * (rr * ir - T(1)));
         T c[1] = {prcp(T(1) + ex)};
         if (P.cn_smooth) {
             T unused = T(0);
@@ -613,7 +615,7 @@ __global__ void k_2b_parts(Params<T> P) {
 // pair set, F_i += (g_i + g_j) c'(r)/r d,  vir += -(g_i + g_j) c'(r)/r d d^T / 2,
 // c'(r)/r = -k1 e (Rcov_i + Rcov_j) / ((1 + e)^2 r^3).
 // ---------------------------------------------------------------------------------------------
-template <class T>
+template <class T, bool ONE>
 __global__ __launch_bounds__(BLOCK) void k_chain(Params<T> P) {
     __shared__ Queue<T> Qs[WPB];
     const int lane = lane_id(), wv = threadIdx.x >> 6;
@@ -627,9 +629,10 @@ __global__ __launch_bounds__(BLOCK) void k_chain(Params<T> P) {
     A.init();
     const int s0 = C.S->st_off[CUT_CN], s1 = s0 + C.S->st_cnt[CUT_CN];
     walk_pairs<T, false>(P, C, s0, s1, P.rc_cn, P.rc_cn2, Qs[wv], [&](int j, T dx, T dy, T dz) {
+        const T gij = gi + (T)at32(P.g, (unsigned)j);  // issued first: its latency overlaps the rest
         const T r2 = dx * dx + dy * dy + dz * dz;
         const T ir = prsqrt(r2);
-        const T rr = rci + P.rcov[at32(P.type_s, (unsigned)j)];
+        const T rr = ONE ? rci + rci : rci + P.rcov[at32(P.type_s, (unsigned)j)];
         const T ex = xexp(-T(K1) * (rr * ir - T(1)));
         const T c = prcp(T(1) + ex);
         T dcn = -T(K1) * ex * rr * ir * ir * ir * c * c;
@@ -637,7 +640,7 @@ __global__ __launch_bounds__(BLOCK) void k_chain(Params<T> P) {
             T cs = c;
             smooth_cn_term(P.rc_cn, r2 * ir, ir, cs, dcn);
         }
-        const T sc = finite_or_zero((gi + (T)at32(P.g, (unsigned)j)) * dcn);
+        const T sc = finite_or_zero(gij * dcn);
         const T h = T(-0.5) * sc;
         const T hx = h * dx, hy = h * dy, hz = h * dz;
         const T v[9] = {sc * dx, sc * dy, sc * dz, hx * dx, hy * dy, hz * dz, hx * dy, hx * dz, hy * dz};
@@ -677,7 +680,10 @@ void Launch<T>::regather(const Params<T> &P, hipStream_t s) {
 template <class T>
 void Launch<T>::cn(const Params<T> &P, hipStream_t s) {
     if (P.n == 0) return;
-    hipLaunchKernelGGL(k_cn<T>, dim3((P.n + WPB - 1) / WPB), dim3(BLOCK), 0, s, P);
+    if (P.ne == 1)
+        hipLaunchKernelGGL((k_cn<T, true>), dim3((P.n + WPB - 1) / WPB), dim3(BLOCK), 0, s, P);
+    else
+        hipLaunchKernelGGL((k_cn<T, false>), dim3((P.n + WPB - 1) / WPB), dim3(BLOCK), 0, s, P);
 }
 
 template <class T, bool ONE, int... NR>
@@ -706,7 +712,10 @@ void Launch<T>::twobody(const Params<T> &P, hipStream_t s) {
 template <class T>
 void Launch<T>::chain(const Params<T> &P, hipStream_t s) {
     if (P.n == 0) return;
-    hipLaunchKernelGGL(k_chain<T>, dim3((P.n + WPB - 1) / WPB), dim3(BLOCK), 0, s, P);
+    if (P.ne == 1)
+        hipLaunchKernelGGL((k_chain<T, true>), dim3((P.n + WPB - 1) / WPB), dim3(BLOCK), 0, s, P);
+    else
+        hipLaunchKernelGGL((k_chain<T, false>), dim3((P.n + WPB - 1) / WPB), dim3(BLOCK), 0, s, P);
 }
 
 template struct Launch<double>;
