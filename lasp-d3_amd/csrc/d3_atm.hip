@@ -669,6 +669,7 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
         auto steps1 = [&](KTile &K, auto diag) __attribute__((always_inline)) {
             constexpr bool DIAG = decltype(diag)::value;
             constexpr int Q0 = DIAG ? 1 : 0, Q1 = DIAG ? AW / 2 + 1 : AW;
+            unsigned tc = 0;  // pairs within rc3 in this tile pair (a scalar register)
 #pragma unroll 1
             for (int q = Q0; q < Q1; ++q) {
                 // FP32: lane l updates a j slot that another lane updated one step earlier
@@ -687,7 +688,7 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                     }
                 }
                 if (m == 0) continue;  // uniform
-                cnt += __popcll(m);
+                tc += (unsigned)__popcll(m);
                 if (!ok) continue;
                 const T2<T> j2 = at(2, jb);
                 T lamjk, sc6j, yx[NR > 0 ? NR : 1], yy[NR > 0 ? NR : 1];
@@ -704,10 +705,12 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                     body(K, jb, j1.y, j2, vx, vy, vz, c, lamjk, sc6j, yx, yy, p0, p1, p2);
                 }
             }
+            cnt += tc;
         };
         // Two k tiles against the staged j tile (off-diagonal for both): the j slot is read, and
         // its sums written back, once for two pairs.
         auto steps2 = [&](KTile &K1, KTile &K2) __attribute__((always_inline)) {
+            unsigned tc = 0;
 #pragma unroll 1
             for (int q = 0; q < AW; ++q) {
                 if constexpr (F32) wave_sync();
@@ -720,7 +723,7 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                 test(K1, j0, j1, vx1, vy1, vz1, c1, ok1, m1);
                 test(K2, j0, j1, vx2, vy2, vz2, c2, ok2, m2);
                 if ((m1 | m2) == 0) continue;  // uniform
-                cnt += __popcll(m1) + __popcll(m2);
+                tc += (unsigned)(__popcll(m1) + __popcll(m2));
                 const T2<T> j2 = at(2, jb);
                 T lamjk, sc6j, yx[NR > 0 ? NR : 1], yy[NR > 0 ? NR : 1];
                 // one element type: the rows do not depend on k
@@ -745,6 +748,7 @@ __global__ __launch_bounds__(AW, (atm_min_waves<T, NR>())) void k_atm(Params<T> 
                     at(JP + 2, jb) = p2;
                 }
             }
+            cnt += tc;
         };
 
         // FP32: k tiles two at a time (K1 = tile kb, K2 = tile kb + 1 when this work item has it):
