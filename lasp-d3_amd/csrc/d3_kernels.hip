@@ -472,8 +472,11 @@ __host__ __device__ constexpr size_t twobody_wave_lds(int ne, int nref) {
     return ((sizeof(Queue<T>) + (size_t)(2 * ne * nref + 3 * ne) * sizeof(T)) + 15) / 16 * 16;
 }
 
+// At least 5 waves per SIMD (96 VGPRs; the few values spilled live outside the pair loop): the
+// gathers of a pair step wait less (Cu two-body FP32 -4.5 %, FP64 -3 %; 6 waves gain little
+// more in FP32 and cost FP64 its pair loop).
 template <class T, int NR, bool ONE>
-__global__ __launch_bounds__(BLOCK) void k_2b(Params<T> P) {
+__global__ __launch_bounds__(BLOCK, 5) void k_2b(Params<T> P) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int wv = threadIdx.x >> 6, lane = lane_id();
     const int w = blockIdx.x * WPB + wv, nparts = P.tb_parts, i = w / nparts, part = w - i * nparts;
@@ -615,8 +618,11 @@ __global__ void k_2b_parts(Params<T> P) {
 // pair set, F_i += (g_i + g_j) c'(r)/r d,  vir += -(g_i + g_j) c'(r)/r d d^T / 2,
 // c'(r)/r = -k1 e (Rcov_i + Rcov_j) / ((1 + e)^2 r^3).
 // ---------------------------------------------------------------------------------------------
+template <class T>
+constexpr int chain_min_waves() { return sizeof(T) == sizeof(float) ? 5 : 1; }  // FP32 -14 % on Cu
+
 template <class T, bool ONE>
-__global__ __launch_bounds__(BLOCK) void k_chain(Params<T> P) {
+__global__ __launch_bounds__(BLOCK, chain_min_waves<T>()) void k_chain(Params<T> P) {
     __shared__ Queue<T> Qs[WPB];
     const int lane = lane_id(), wv = threadIdx.x >> 6;
     const int i = blockIdx.x * WPB + wv;
