@@ -410,27 +410,20 @@ __device__ __forceinline__ void pair_2b(const Params<T> &P, const T *const *wr, 
                                         const T *dy_, const T *rq, const T *rR, const T *rX, int ti, T cni,
                                         Acc2b<T> &A) {
     const int tj = ONE ? 0 : at32(P.type_s, (unsigned)j);
-    T c6 = T(0), dc6 = T(0);
-    if constexpr (NR == 0) {
-        T dc6j;
-        c6_pairwise(P.c6g, P.ne, P.nref, ti, tj, cni, at32(P.cn, (unsigned)j), c6, dc6, dc6j);
-    } else {
-        T wj[NR];
+    // the partner's weights are gathered first; the damping factors below do not need them, so
+    // their arithmetic covers the gathers' latency
+    T wj[NR > 0 ? NR : 1];
+    if constexpr (NR > 0) {
 #pragma unroll
         for (int b = 0; b < NR; ++b) wj[b] = at32(wr[b], (unsigned)j);
-#pragma unroll
-        for (int b = 0; b < NR; ++b) {
-            c6 += y[tj * NR + b] * wj[b];
-            dc6 += dy_[tj * NR + b] * wj[b];
-        }
     }
     const T r2 = dx * dx + dy * dy + dz * dz;
-    // e = E+ of the whole pair (the centre's share is half of it: the sums are halved once, in
-    // k_2b), dE+/dCN_i, phi = (1/r) dE+/dr; the virial sums collect phi d_a d_b (times -1/2 there)
-    T e, g, phi;
+    // e = E1 C6 is E+ of the whole pair (the centre's share is half of it: the sums are halved
+    // once, in k_2b), g = E1 dC6/dCN_i, phi = P1 C6 = (1/r) dE+/dr; the virial sums collect
+    // phi d_a d_b (times -1/2 there)
+    T E1, P1;
     if (P.damping == 0) {
-        const T q = rq[tj], R = rR[tj];
-        const T c8 = T(3) * c6 * q, dc8 = T(3) * dc6 * q;
+        const T q3 = T(3) * rq[tj], R = rR[tj];  // C8 = 3 q C6
         const T ir = prsqrt(r2);
         const T ir2 = ir * ir;
         const T ir6 = ir2 * ir2 * ir2, ir8 = ir6 * ir2;
@@ -440,9 +433,8 @@ __device__ __forceinline__ void pair_2b(const Params<T> &P, const T *const *wr, 
         const T b8_2 = b8 * b8, b8_4 = b8_2 * b8_2, b8_8 = b8_4 * b8_4, b8_16 = b8_8 * b8_8;
         const T f6 = finite_or_zero(prcp(T(1) + T(6) * b6_14)), f8 = finite_or_zero(prcp(T(1) + T(6) * b8_16));
         const T df6 = finite_or_zero(T(84) * f6 * f6 * b6_14 * ir), df8 = finite_or_zero(T(96) * f8 * f8 * b8_16 * ir);
-        e = P.s6 * c6 * f6 * ir6 + P.s8 * c8 * f8 * ir8;
-        g = P.s6 * f6 * dc6 * ir6 + P.s8 * f8 * dc8 * ir8;
-        phi = (P.s6 * c6 * (df6 - T(6) * f6 * ir) * ir6 + P.s8 * c8 * (df8 - T(8) * f8 * ir) * ir8) * ir;
+        E1 = P.s6 * f6 * ir6 + P.s8 * q3 * f8 * ir8;
+        P1 = (P.s6 * (df6 - T(6) * f6 * ir) * ir6 + P.s8 * q3 * (df8 - T(8) * f8 * ir) * ir8) * ir;
     } else {
         // BJ: f_n r^-n = 1 / (r^n + a^n), so with rq = 3 s8 r2r4_A r2r4_B, rR = a^6, rX = a^8,
         // d6 = 1 / (r^6 + a^6), d8 = 1 / (r^8 + a^8), t8 = rq d8:
@@ -456,12 +448,22 @@ __device__ __forceinline__ void pair_2b(const Params<T> &P, const T *const *wr, 
         const T tiny = sizeof(T) == sizeof(float) ? T(FLT_MIN) : T(DBL_MIN);
         const T d6 = n6 >= tiny ? prcp(n6) : T(0), d8 = n8 >= tiny ? prcp(n8) : T(0);
         const T t8 = rq[tj] * d8;
-        const T tt = P.s6 * d6 + t8;
-        e = c6 * tt;
-        g = dc6 * tt;
+        E1 = P.s6 * d6 + t8;
         const T u8 = T(8) * t8 * d8, u6 = (T(6) * P.s6) * d6 * d6;
-        phi = -(c6 * r4) * (u8 * r2 + u6);
+        P1 = -r4 * (u8 * r2 + u6);
     }
+    T c6 = T(0), dc6 = T(0);
+    if constexpr (NR == 0) {
+        T dc6j;
+        c6_pairwise(P.c6g, P.ne, P.nref, ti, tj, cni, at32(P.cn, (unsigned)j), c6, dc6, dc6j);
+    } else {
+#pragma unroll
+        for (int b = 0; b < NR; ++b) {
+            c6 += y[tj * NR + b] * wj[b];
+            dc6 += dy_[tj * NR + b] * wj[b];
+        }
+    }
+    const T e = c6 * E1, g = dc6 * E1, phi = c6 * P1;
     const T fx = phi * dx, fy = phi * dy, fz = phi * dz;
     const T vv[6] = {fx * dx, fy * dy, fz * dz, fx * dy, fx * dz, fy * dz};
     A.add(e, g, fx, fy, fz, vv);
