@@ -169,8 +169,8 @@ __global__ void k_regather(Params<T> P) {
 // at a time against the cutoff (0 < r^2 <= rc^2, or < rc^2 when strict), the accepted pairs go to
 // the wave's ring queue in LDS (ballot + mbcnt), and every 64 queued pairs are evaluated by the
 // whole wave, one pair per lane (the remainder at the end by the lanes that have one).  The
-// loads of a candidate are clamped to the bin (no lane-dependent branch) and addressed by 32-bit
-// offsets.
+// loads of a candidate are clamped to the bin (no lane-dependent branch), addressed by 32-bit
+// offsets, and issued one chunk ahead.
 // ---------------------------------------------------------------------------------------------
 template <class T>
 struct Queue {
@@ -178,45 +178,98 @@ struct Queue {
     T dx[QCAP], dy[QCAP], dz[QCAP];
 };
 
-template <class T, bool STRICT, class Eval>
+// AHEAD: the coordinates of the next chunk of candidates are loaded before the current chunk is
+// tested (more registers; kernels choose it per precision by measurement).
+template <class T, bool STRICT, bool AHEAD, class Eval>
 __device__ __forceinline__ void walk_pairs(const Params<T> &P, const Centre<T> &C, int s0, int s1, T rc, T rc2,
                                            Queue<T> &Q, Eval &&eval) {
     const int lane = lane_id();
     int qt = 0, qh = 0;  // pairs queued, pairs evaluated (ring positions modulo QCAP)
+    // One chunk: up to 64 candidates of one bin (this lane's atom index j, the bin's end j1, the
+    // centre's position plus the image vector, the candidate's coordinates).
+    struct Chunk {
+        unsigned j, j1;
+        T ox, oy, oz, x, y, z;
+    };
+    auto test = [&](const Chunk &K) __attribute__((always_inline)) {
+        const T dx = K.ox - K.x, dy = K.oy - K.y, dz = K.oz - K.z;
+        const T r2 = dx * dx + dy * dy + dz * dz;
+        // one ballot per compare: each folds into its v_cmp (a ballot of the combined condition
+        // costs a select and a compare more per candidate step)
+        const bool c0 = K.j < K.j1, c1 = STRICT ? r2 < rc2 : r2 <= rc2, c2 = r2 > T(0);
+        const bool in = c0 && c1 && c2;
+        const uint64_t m = ballot64(c0) & ballot64(c1) & ballot64(c2);
+        if (m == 0) return;
+        if (in) {
+            const int pos = (qt + mbcnt64(m)) & (QCAP - 1);
+            Q.j[pos] = (int)K.j;
+            Q.dx[pos] = dx;
+            Q.dy[pos] = dy;
+            Q.dz[pos] = dz;
+        }
+        qt += __popcll(m);
+        if (qt - qh >= WAVE) {
+            wave_sync();
+            const int pos = (qh + lane) & (QCAP - 1);
+            eval(Q.j[pos], Q.dx[pos], Q.dy[pos], Q.dz[pos]);
+            qh += WAVE;
+            wave_sync();  // the next pairs may go to the slots just read
+        }
+    };
     for (int sb = s0; sb < s1; sb += WAVE) {
         const auto Bn = C.bins(P, sb, s1, rc);
-        const int nsb = s1 - sb < WAVE ? s1 - sb : WAVE;
-        for (int sk = 0; sk < nsb; ++sk) {
-            int j0, j1;
-            T ox, oy, oz;
-            Bn.get(sk, j0, j1, ox, oy, oz);
-            for (int jb = j0; jb < j1; jb += WAVE) {
-                const unsigned j = (unsigned)(jb + lane);
-                const unsigned jj = min(j, (unsigned)(j1 - 1));
-                const T dx = ox - at32(P.lx, jj), dy = oy - at32(P.ly, jj), dz = oz - at32(P.lz, jj);
-                const T r2 = dx * dx + dy * dy + dz * dz;
-                // one ballot per compare: each folds into its v_cmp (a ballot of the combined
-                // condition costs a select and a compare more per candidate step)
-                const bool c0 = j < (unsigned)j1, c1 = STRICT ? r2 < rc2 : r2 <= rc2, c2 = r2 > T(0);
-                const bool in = c0 && c1 && c2;
-                const uint64_t m = ballot64(c0) & ballot64(c1) & ballot64(c2);
-                if (m == 0) continue;
-                if (in) {
-                    const int pos = (qt + mbcnt64(m)) & (QCAP - 1);
-                    Q.j[pos] = (int)j;
-                    Q.dx[pos] = dx;
-                    Q.dy[pos] = dy;
-                    Q.dz[pos] = dz;
-                }
-                qt += __popcll(m);
-                if (qt - qh >= WAVE) {
-                    wave_sync();
-                    const int pos = (qh + lane) & (QCAP - 1);
-                    eval(Q.j[pos], Q.dx[pos], Q.dy[pos], Q.dz[pos]);
-                    qh += WAVE;
-                    wave_sync();  // the next pairs may go to the slots just read
+        if constexpr (!AHEAD) {
+            const int nsb = s1 - sb < WAVE ? s1 - sb : WAVE;
+            for (int sk = 0; sk < nsb; ++sk) {
+                int j0, j1;
+                Chunk K;
+                Bn.get(sk, j0, j1, K.ox, K.oy, K.oz);
+                K.j1 = (unsigned)j1;
+                for (int jb = j0; jb < j1; jb += WAVE) {
+                    K.j = (unsigned)(jb + lane);
+                    const unsigned jj = min(K.j, K.j1 - 1);  // loads clamped to the bin
+                    K.x = at32(P.lx, jj);
+                    K.y = at32(P.ly, jj);
+                    K.z = at32(P.lz, jj);
+                    test(K);
                 }
             }
+            continue;
+        }
+        // the group's chunks in stencil order: lane l's entry has nch of them, incl chunks end
+        // with it (a scan over the lanes), so chunk c belongs to the first lane with incl > c
+        const int nch = (Bn.j1 - Bn.j0 + WAVE - 1) / WAVE;
+        int incl = nch;
+#pragma unroll
+        for (int o = 1; o < WAVE; o <<= 1) {
+            const int v = __shfl_up(incl, o, WAVE);
+            if (lane >= o) incl += v;
+        }
+        const int excl = incl - nch, total = __builtin_amdgcn_readlane(incl, WAVE - 1);
+        auto fetch = [&](int c, Chunk &K) __attribute__((always_inline)) {
+            const int k = (int)__builtin_ctzll(ballot64(incl > c));
+            int j0, j1;
+            Bn.get(k, j0, j1, K.ox, K.oy, K.oz);
+            K.j = (unsigned)(j0 + (c - __builtin_amdgcn_readlane(excl, k)) * WAVE + lane);
+            K.j1 = (unsigned)j1;
+            const unsigned jj = min(K.j, K.j1 - 1);  // loads clamped to the bin: no lane branch
+            K.x = at32(P.lx, jj);
+            K.y = at32(P.ly, jj);
+            K.z = at32(P.lz, jj);
+        };
+        // two chunks in flight: the next chunk's coordinates are loaded before the current one
+        // is tested, so their latency overlaps the test and any evaluation it triggers (the loads
+        // are issued on every path -- past the end they repeat the last chunk -- so that the
+        // wait before a test covers only the older chunk)
+        if (total == 0) continue;
+        Chunk A, B;
+        fetch(0, A);
+        for (int c = 0; c < total; c += 2) {
+            fetch(min(c + 1, total - 1), B);
+            test(A);
+            if (c + 1 >= total) break;
+            fetch(min(c + 2, total - 1), A);
+            test(B);
         }
     }
     wave_sync();
@@ -273,9 +326,10 @@ struct AccN {
 // With CN references that depend only on (element, reference) -- checked at handle init -- the
 // reference's 25-term Z/W of src/d3_kernel.cu:709-800 equals w_A^T C6ref w_B exactly.
 // ---------------------------------------------------------------------------------------------
-// ONE: a handle of one element type, Rcov_i + Rcov_j = 2 Rcov_i (no per-pair gathers).
+// ONE: a handle of one element type, Rcov_i + Rcov_j = 2 Rcov_i (no per-pair gathers).  FP32 at
+// 6 waves per SIMD (78 VGPRs with the look-ahead walk, no spills).
 template <class T, bool ONE>
-__global__ __launch_bounds__(BLOCK) void k_cn(Params<T> P) {
+__global__ __launch_bounds__(BLOCK, sizeof(T) == sizeof(float) ? 6 : 1) void k_cn(Params<T> P) {
     __shared__ Queue<T> Qs[WPB];
     const int lane = lane_id(), wv = threadIdx.x >> 6;
     const int i = blockIdx.x * WPB + wv;
@@ -287,7 +341,7 @@ __global__ __launch_bounds__(BLOCK) void k_cn(Params<T> P) {
     acc.init();
     float cnt = 0.0f;
     const int s0 = C.S->st_off[CUT_CN], s1 = s0 + C.S->st_cnt[CUT_CN];
-    walk_pairs<T, false>(P, C, s0, s1, P.rc_cn, P.rc_cn2, Qs[wv], [&](int j, T dx, T dy, T dz) {
+    walk_pairs<T, false, true>(P, C, s0, s1, P.rc_cn, P.rc_cn2, Qs[wv], [&](int j, T dx, T dy, T dz) {
         const T r2 = dx * dx + dy * dy + dz * dz;
         const T ir = prsqrt(r2);
         const T rr = ONE ? rci + rci : rci + P.rcov[at32(P.type_s, (unsigned)j)];
@@ -564,7 +618,8 @@ __global__ __launch_bounds__(BLOCK, 5) void k_2b(Params<T> P) {
     const int s0 = sa + min(sz, part * per), s1 = sa + min(sz, (part + 1) * per);
     const T *wr[NR > 0 ? NR : 1];  // rows of the weights, one per reference
     for (int b = 0; b < NR; ++b) wr[b] = sbase(P.w + (size_t)b * P.n);
-    walk_pairs<T, true>(P, C, s0, s1, xsqrt(P.rc_22), P.rc_22, Q, [&](int j, T dx, T dy, T dz) {
+    walk_pairs<T, true, sizeof(T) == sizeof(float)>(P, C, s0, s1, xsqrt(P.rc_22), P.rc_22, Q,
+                                                     [&](int j, T dx, T dy, T dz) {
         if constexpr (ONE)
             pair_2b<T, NR, true>(P, wr, j, dx, dy, dz, y1, dy1, rq1, rR1, rX1, 0, cni, A);
         else
@@ -636,7 +691,8 @@ __global__ __launch_bounds__(BLOCK, chain_min_waves<T>()) void k_chain(Params<T>
     AccN<T, 9> A;  // F (3), virial (6)
     A.init();
     const int s0 = C.S->st_off[CUT_CN], s1 = s0 + C.S->st_cnt[CUT_CN];
-    walk_pairs<T, false>(P, C, s0, s1, P.rc_cn, P.rc_cn2, Qs[wv], [&](int j, T dx, T dy, T dz) {
+    walk_pairs<T, false, sizeof(T) == sizeof(float)>(P, C, s0, s1, P.rc_cn, P.rc_cn2, Qs[wv],
+                                                      [&](int j, T dx, T dy, T dz) {
         const T gij = gi + (T)at32(P.g, (unsigned)j);  // issued first: its latency overlaps the rest
         const T r2 = dx * dx + dy * dy + dz * dz;
         const T ir = prsqrt(r2);
