@@ -172,24 +172,41 @@ __global__ void k_regather(Params<T> P) {
 // loads of a candidate are clamped to the bin (no lane-dependent branch), addressed by 32-bit
 // offsets, and issued one chunk ahead.
 // ---------------------------------------------------------------------------------------------
-template <class T>
+// NPAY: values per candidate loaded with its coordinates and queued with it (the evaluation reads
+// them from LDS instead of gathering them after the pair was accepted).
+template <class T, int NPAY = 0>
 struct Queue {
+    int j[QCAP];
+    T dx[QCAP], dy[QCAP], dz[QCAP];
+    T pay[NPAY][QCAP];
+};
+template <class T>
+struct Queue<T, 0> {
     int j[QCAP];
     T dx[QCAP], dy[QCAP], dz[QCAP];
 };
 
 // AHEAD: the coordinates of the next chunk of candidates are loaded before the current chunk is
 // tested (more registers; kernels choose it per precision by measurement).
-template <class T, bool STRICT, bool AHEAD, class Eval>
+// pay(jj, v): the NPAY payload values of candidate jj; eval(j, dx, dy, dz, v) evaluates a pair.
+template <class T, bool STRICT, bool AHEAD, int NPAY, class Pay, class Eval>
 __device__ __forceinline__ void walk_pairs(const Params<T> &P, const Centre<T> &C, int s0, int s1, T rc, T rc2,
-                                           Queue<T> &Q, Eval &&eval) {
+                                           Queue<T, NPAY> &Q, Pay &&pay, Eval &&eval) {
     const int lane = lane_id();
     int qt = 0, qh = 0;  // pairs queued, pairs evaluated (ring positions modulo QCAP)
     // One chunk: up to 64 candidates of one bin (this lane's atom index j, the bin's end j1, the
-    // centre's position plus the image vector, the candidate's coordinates).
+    // centre's position plus the image vector, the candidate's coordinates and payload).
     struct Chunk {
         unsigned j, j1;
         T ox, oy, oz, x, y, z;
+        T v[NPAY > 0 ? NPAY : 1];
+    };
+    auto pop = [&](int pos) __attribute__((always_inline)) {
+        T v[NPAY > 0 ? NPAY : 1];
+        if constexpr (NPAY > 0)
+#pragma unroll
+            for (int k = 0; k < NPAY; ++k) v[k] = Q.pay[k][pos];
+        eval(Q.j[pos], Q.dx[pos], Q.dy[pos], Q.dz[pos], v);
     };
     auto test = [&](const Chunk &K) __attribute__((always_inline)) {
         const T dx = K.ox - K.x, dy = K.oy - K.y, dz = K.oz - K.z;
@@ -206,12 +223,14 @@ __device__ __forceinline__ void walk_pairs(const Params<T> &P, const Centre<T> &
             Q.dx[pos] = dx;
             Q.dy[pos] = dy;
             Q.dz[pos] = dz;
+            if constexpr (NPAY > 0)
+#pragma unroll
+                for (int k = 0; k < NPAY; ++k) Q.pay[k][pos] = K.v[k];
         }
         qt += __popcll(m);
         if (qt - qh >= WAVE) {
             wave_sync();
-            const int pos = (qh + lane) & (QCAP - 1);
-            eval(Q.j[pos], Q.dx[pos], Q.dy[pos], Q.dz[pos]);
+            pop((qh + lane) & (QCAP - 1));
             qh += WAVE;
             wave_sync();  // the next pairs may go to the slots just read
         }
@@ -231,6 +250,7 @@ __device__ __forceinline__ void walk_pairs(const Params<T> &P, const Centre<T> &
                     K.x = at32(P.lx, jj);
                     K.y = at32(P.ly, jj);
                     K.z = at32(P.lz, jj);
+                    if constexpr (NPAY > 0) pay(jj, K.v);
                     test(K);
                 }
             }
@@ -256,6 +276,7 @@ __device__ __forceinline__ void walk_pairs(const Params<T> &P, const Centre<T> &
             K.x = at32(P.lx, jj);
             K.y = at32(P.ly, jj);
             K.z = at32(P.lz, jj);
+            if constexpr (NPAY > 0) pay(jj, K.v);
         };
         // two chunks in flight: the next chunk's coordinates are loaded before the current one
         // is tested, so their latency overlaps the test and any evaluation it triggers (the loads
@@ -273,11 +294,14 @@ __device__ __forceinline__ void walk_pairs(const Params<T> &P, const Centre<T> &
         }
     }
     wave_sync();
-    if (lane < qt - qh) {
-        const int pos = (qh + lane) & (QCAP - 1);
-        eval(Q.j[pos], Q.dx[pos], Q.dy[pos], Q.dz[pos]);
-    }
+    if (lane < qt - qh) pop((qh + lane) & (QCAP - 1));
 }
+
+// no payload
+template <class T>
+struct NoPay {
+    __device__ void operator()(unsigned, T *) const {}
+};
 
 // Reciprocal and reciprocal square root of the pair math: the FP32 hardware instructions
 // (1 ulp), correctly rounded FP64.
@@ -341,7 +365,8 @@ __global__ __launch_bounds__(BLOCK, sizeof(T) == sizeof(float) ? 6 : 1) void k_c
     acc.init();
     float cnt = 0.0f;
     const int s0 = C.S->st_off[CUT_CN], s1 = s0 + C.S->st_cnt[CUT_CN];
-    walk_pairs<T, false, true>(P, C, s0, s1, P.rc_cn, P.rc_cn2, Qs[wv], [&](int j, T dx, T dy, T dz) {
+    walk_pairs<T, false, true, 0>(P, C, s0, s1, P.rc_cn, P.rc_cn2, Qs[wv], NoPay<T>{},
+                                  [&](int j, T dx, T dy, T dz, const T *) {
         const T r2 = dx * dx + dy * dy + dz * dz;
         const T ir = prsqrt(r2);
         const T rr = ONE ? rci + rci : rci + P.rcov[at32(P.type_s, (unsigned)j)];
@@ -459,17 +484,18 @@ struct Acc2b {
 // per pair from the pair's reference entries (c6_pairwise) at the CNs of i (cni) and j.
 // ONE: a handle of one element type -- the partner's type is 0 (not loaded) and the centre's C6
 // row and damping constants are registers, not LDS.
-template <class T, int NR, bool ONE>
-__device__ __forceinline__ void pair_2b(const Params<T> &P, const T *const *wr, int j, T dx, T dy, T dz, const T *y,
-                                        const T *dy_, const T *rq, const T *rR, const T *rX, int ti, T cni,
-                                        Acc2b<T> &A) {
+// NPAY = NR: the partner's weights wq came with the candidate; else gathered here from the rows wr.
+template <class T, int NR, bool ONE, int NPAY>
+__device__ __forceinline__ void pair_2b(const Params<T> &P, const T *const *wr, const T *wq, int j, T dx, T dy,
+                                        T dz, const T *y, const T *dy_, const T *rq, const T *rR, const T *rX,
+                                        int ti, T cni, Acc2b<T> &A) {
     const int tj = ONE ? 0 : at32(P.type_s, (unsigned)j);
     // the partner's weights are gathered first; the damping factors below do not need them, so
     // their arithmetic covers the gathers' latency
     T wj[NR > 0 ? NR : 1];
     if constexpr (NR > 0) {
 #pragma unroll
-        for (int b = 0; b < NR; ++b) wj[b] = at32(wr[b], (unsigned)j);
+        for (int b = 0; b < NR; ++b) wj[b] = NPAY == NR ? wq[b] : at32(wr[b], (unsigned)j);
     }
     const T r2 = dx * dx + dy * dy + dz * dz;
     // e = E1 C6 is E+ of the whole pair (the centre's share is half of it: the sums are halved
@@ -523,9 +549,18 @@ __device__ __forceinline__ void pair_2b(const Params<T> &P, const T *const *wr, 
     A.add(e, g, fx, fy, fz, vv);
 }
 
+// FP32 with up to 3 references: the partner's CN weights travel with the candidate (queue payload)
+template <class T>
+__host__ __device__ constexpr int twobody_npay(int nref) {
+    return sizeof(T) == sizeof(float) && nref >= 1 && nref <= 3 ? nref : 0;
+}
+template <class T>
+__host__ __device__ constexpr size_t twobody_queue_bytes(int nref) {
+    return sizeof(Queue<T, 0>) + (size_t)twobody_npay<T>(nref) * QCAP * sizeof(T);
+}
 template <class T>
 __host__ __device__ constexpr size_t twobody_wave_lds(int ne, int nref) {
-    return ((sizeof(Queue<T>) + (size_t)(2 * ne * nref + 3 * ne) * sizeof(T)) + 15) / 16 * 16;
+    return ((twobody_queue_bytes<T>(nref) + (size_t)(2 * ne * nref + 3 * ne) * sizeof(T)) + 15) / 16 * 16;
 }
 
 // At least 5 waves per SIMD (96 VGPRs; the few values spilled live outside the pair loop): the
@@ -538,9 +573,11 @@ __global__ __launch_bounds__(BLOCK, 5) void k_2b(Params<T> P) {
     const int w = blockIdx.x * WPB + wv, nparts = P.tb_parts, i = w / nparts, part = w - i * nparts;
     if (i >= P.n) return;
     const int ynr = P.ne * NR;
+    constexpr int NPAY = twobody_npay<T>(NR);
     unsigned char *base = smem + wv * twobody_wave_lds<T>(P.ne, NR);
-    Queue<T> &Q = *reinterpret_cast<Queue<T> *>(base);
-    T *y = reinterpret_cast<T *>(base + sizeof(Queue<T>));
+    Queue<T, NPAY> &Q = *reinterpret_cast<Queue<T, NPAY> *>(base);
+    static_assert(sizeof(Queue<T, NPAY>) == twobody_queue_bytes<T>(NR), "queue layout");
+    T *y = reinterpret_cast<T *>(base + sizeof(Queue<T, NPAY>));
     T *dyv = y + ynr, *rq = dyv + ynr, *rR = rq + P.ne, *rX = rR + P.ne;
     T y1[ONE ? NR : 1], dy1[ONE ? NR : 1], rq1[1], rR1[1], rX1[1];  // ONE: the same, in registers
 
@@ -618,12 +655,16 @@ __global__ __launch_bounds__(BLOCK, 5) void k_2b(Params<T> P) {
     const int s0 = sa + min(sz, part * per), s1 = sa + min(sz, (part + 1) * per);
     const T *wr[NR > 0 ? NR : 1];  // rows of the weights, one per reference
     for (int b = 0; b < NR; ++b) wr[b] = sbase(P.w + (size_t)b * P.n);
-    walk_pairs<T, true, sizeof(T) == sizeof(float)>(P, C, s0, s1, xsqrt(P.rc_22), P.rc_22, Q,
-                                                     [&](int j, T dx, T dy, T dz) {
+    auto pay = [&](unsigned jj, T *v) __attribute__((always_inline)) {
+#pragma unroll
+        for (int b = 0; b < NPAY; ++b) v[b] = at32(wr[b], jj);
+    };
+    walk_pairs<T, true, sizeof(T) == sizeof(float), NPAY>(P, C, s0, s1, xsqrt(P.rc_22), P.rc_22, Q, pay,
+                                                           [&](int j, T dx, T dy, T dz, const T *wq) {
         if constexpr (ONE)
-            pair_2b<T, NR, true>(P, wr, j, dx, dy, dz, y1, dy1, rq1, rR1, rX1, 0, cni, A);
+            pair_2b<T, NR, true, NPAY>(P, wr, wq, j, dx, dy, dz, y1, dy1, rq1, rR1, rX1, 0, cni, A);
         else
-            pair_2b<T, NR, false>(P, wr, j, dx, dy, dz, y, dyv, rq, rR, rX, C.ti, cni, A);
+            pair_2b<T, NR, false, NPAY>(P, wr, wq, j, dx, dy, dz, y, dyv, rq, rR, rX, C.ti, cni, A);
         cnt += 1.0f;
     });
     A.fold();
@@ -691,8 +732,8 @@ __global__ __launch_bounds__(BLOCK, chain_min_waves<T>()) void k_chain(Params<T>
     AccN<T, 9> A;  // F (3), virial (6)
     A.init();
     const int s0 = C.S->st_off[CUT_CN], s1 = s0 + C.S->st_cnt[CUT_CN];
-    walk_pairs<T, false, sizeof(T) == sizeof(float)>(P, C, s0, s1, P.rc_cn, P.rc_cn2, Qs[wv],
-                                                      [&](int j, T dx, T dy, T dz) {
+    walk_pairs<T, false, sizeof(T) == sizeof(float), 0>(P, C, s0, s1, P.rc_cn, P.rc_cn2, Qs[wv], NoPay<T>{},
+                                                         [&](int j, T dx, T dy, T dz, const T *) {
         const T gij = gi + (T)at32(P.g, (unsigned)j);  // issued first: its latency overlaps the rest
         const T r2 = dx * dx + dy * dy + dz * dz;
         const T ir = prsqrt(r2);
