@@ -169,8 +169,8 @@ __global__ void k_regather(Params<T> P) {
 // at a time against the cutoff (0 < r^2 <= rc^2, or < rc^2 when strict), the accepted pairs go to
 // the wave's ring queue in LDS (ballot + mbcnt), and every 64 queued pairs are evaluated by the
 // whole wave, one pair per lane (the remainder at the end by the lanes that have one).  The
-// loads of a candidate are clamped to the bin (no lane-dependent branch), addressed by 32-bit
-// offsets, and issued one chunk ahead.
+// loads of a candidate are clamped to the bin (no lane-dependent branch) and addressed by 32-bit
+// offsets; the kernels that choose it (AHEAD) issue them one chunk ahead.
 // ---------------------------------------------------------------------------------------------
 // NPAY: values per candidate loaded with its coordinates and queued with it (the evaluation reads
 // them from LDS instead of gathering them after the pair was accepted).
